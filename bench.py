@@ -1,0 +1,164 @@
+"""Throughput benchmark of the crt path tracer on one MI355X.
+
+Workload (one step = one sample pass of the render hot path): random triangle soup of
+1,000,000 triangles (centroids uniform in the unit cube, vertex offsets N(0, 0.005^2)),
+1920x1080 pinhole camera, 1 spp, primary rays plus 3 cosine-weighted diffuse bounces
+(depth 4), compat estimator, constant sky, sun off.  Geometry and rays live in HBM
+before the timed region starts; the BVH build is timed separately.
+
+Output: one JSON line.  ``value`` is Mrays/s = (extension + shadow rays traced) / wall
+time of the timed passes.  ``roofline`` covers the dominant kernel (closest-hit
+traversal): algorithmic bytes = 48 B per ray (32 B ray in, 16 B hit out) over the
+kernel's mean launch time measured with HIP events on the library's stream.
+``cpu_baseline`` times the CPU oracle's restatement of the reference loop (FP32
+binned-SAH BVH, one task per row, barrier per pass) on a band of rows of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s at 1080p; equal-time MSE vs 4096-spp render, plain vs neural cache"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_RAY = 48
+
+
+def cpu_baseline(spec, res, bounces, seconds_target: float, threads: int) -> dict:
+    """Oracle ("port") render of a band of rows of the same frame on host cores."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O  # CPU baseline leg only
+
+    t0 = time.perf_counter()
+    bvh = O.Bvh(spec.corners())
+    build_s = time.perf_counter() - t0
+    W, H = res
+    # calibrate on a few rows, then size the band for the target CPU time
+    cal_rows = 8
+    r = O.render(spec, res=res, spp=1, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh,
+                 rows=(H // 2, H // 2 + cal_rows), nthreads=threads, aovs=False)
+    per_row = max(r["stats"]["seconds"], 1e-6) / cal_rows
+    nrows = int(min(H, max(cal_rows, seconds_target / per_row)))
+    r0 = max(0, H // 2 - nrows // 2)
+    r = O.render(spec, res=res, spp=1, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh,
+                 rows=(r0, r0 + nrows), nthreads=threads, aovs=False)
+    st = r["stats"]
+    rays = st["ext_rays"] + st["shadow_rays"]
+    return {
+        "value": rays / st["seconds"] / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"rows {r0}-{r0 + nrows} of the {W}x{H} frame ({nrows * W} paths, {rays} rays, "
+                  f"{st['seconds']:.1f} s); CPU BVH (FP32 binned SAH, built in {build_s:.1f} s, untimed); "
+                  f"Embree is not available on this image",
+        "ref_counting_mrays_s": st["ref_segments"] / st["seconds"] / 1e6,
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=0, help="BVH width (0 = library default)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
+              file=sys.stderr)
+        return 2
+
+    from crender_amd import Context, Renderer, scene_from_spec, scenes
+
+    res, bounces = (1920, 1080), 4
+    spec = scenes.soup(args.tris, seed=2, res=res)
+    ctx = Context(0, bvh_width=args.width, phase_timing=True)
+    dev = ctx.device_name()
+    sc = scene_from_spec(ctx, spec)
+    info = sc.bvh_info()
+    r = Renderer(ctx, sc, res[0], res[1], bounces)
+    r.set_resolution(*res)
+    r.set_options(mode=0, rr_start=0, seed=0)
+    r.render(args.warmup)
+    s0 = r.current_stats()
+    t0 = time.perf_counter()
+    r.render(args.steps)          # synchronous: returns after the last pass completed
+    t1 = time.perf_counter()
+    s1 = r.current_stats()
+    wall = t1 - t0
+    ext = s1["extension_rays"] - s0["extension_rays"]
+    shadow = s1["shadow_rays"] - s0["shadow_rays"]
+    segs = s1["total_rays"] - s0["total_rays"]
+    rays = ext + shadow
+    ms_ext = s1["ms_extend"] - s0["ms_extend"]
+    launches = s1["extend_launches"] - s0["extend_launches"]
+    gpu_ms = s1["ms_render"] - s0["ms_render"]
+    achieved = BYTES_PER_RAY * ext / (ms_ext * 1e-3) / 1e9 if ms_ext > 0 else 0.0
+    traffic = None
+    pmc = ROOT / "profiles" / "trace_closest_soup1m_1080p_pmc.json"
+    if pmc.exists():
+        traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": rays / wall / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"triangle soup {args.tris} tris, 1920x1080, 1 spp, primary + 3 diffuse bounces "
+                        f"(depth 4), compat estimator, sun off, constant sky",
+            "bvh_width": info["width"],
+            "device": dev,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_trace (closest hit)",
+            "bytes_per_launch": BYTES_PER_RAY * ext / max(launches, 1),
+            "mean_launch_ms": ms_ext / max(launches, 1),
+        },
+        "rays": {"extension": ext, "shadow": shadow, "ref_counting_segments": segs},
+        "samples_per_s": res[0] * res[1] * args.steps / wall,
+        "ref_counting_mrays_s": segs / wall / 1e6,
+        "gpu_ms_per_step": gpu_ms / args.steps,
+        "bvh": {"build_ms": info["build_ms"], "mtris_per_s": args.tris / (info["build_ms"] * 1e-3) / 1e6,
+                "nodes": info["node_count"], "depth": info["depth"], "sah_cost": info["sah_cost"],
+                "bytes": info["bytes"]},
+    }
+    r.close()
+    sc.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(spec, res, bounces, args.cpu_seconds, threads)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,12 @@
+"""crender_amd -- MI355X-native path tracer for CRender's render hot path.
+
+The compute path is the HIP library ``libcrt.so`` (C ABI in include/crt.h); this package
+holds its Python mirror of the reference's scene / renderer interface and the seeded
+scene generators.  Importing the package does not load the library; the first use of
+:class:`Context` does, and fails loudly when the library is missing.
+"""
+from ._abi import CrtError, header_functions, lib  # noqa: F401
+from .render import Context, Renderer, Scene, scene_from_spec  # noqa: F401
+from . import scenes  # noqa: F401
+
+__all__ = ["Context", "Scene", "Renderer", "scene_from_spec", "scenes", "CrtError", "lib", "header_functions"]

@@ -1,0 +1,1332 @@
+// api.cpp -- C ABI of the crt path tracer (declared in include/crt.h).
+//
+// Host side of the drop-in boundary for CRender's render hot path: scene-load
+// (cr::scene / cr::registry), the Embree query (rtcIntersect1 via scene::cast_ray) and
+// the renderer (cr::renderer: start / pause / update / current_* / current_stats).
+// Every entry point returns a status code and records a thread-local message; nothing
+// here exits the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/crt.h"
+#include "crt_internal.h"
+
+using namespace crt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? CRT_ERR_OUT_OF_MEMORY : CRT_ERR_HIP,      \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)p, sizeof(T) * count);
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+uint64_t fnv1a(uint64_t h, const void *data, size_t n)
+{
+    const uint8_t *p = static_cast<const uint8_t *>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct ModelHost {
+    std::string               name;
+    std::vector<float>        corners;   // 9 per triangle
+    std::vector<float>        uv;        // 6 per triangle (empty: no uv)
+    std::vector<uint32_t>     mat_idx;   // local material index per triangle
+    std::vector<crt_material> mats;
+    std::vector<std::vector<float>> tex_data;
+    std::vector<std::pair<uint32_t, uint32_t>> tex_dims;
+    std::vector<float>        instances; // 16 per instance
+};
+
+}  // namespace
+
+struct crt_context {
+    int         device = 0;
+    crt_options opt{};
+    hipStream_t stream = nullptr;
+    uint32_t    grid   = 0;
+    uint32_t   *spill  = nullptr;
+    // staging for crt_intersect / crt_occluded
+    uint32_t    stage_cap = 0;
+    float      *st_ray[8] = {};
+    float      *st_t = nullptr, *st_u = nullptr, *st_v = nullptr;
+    uint32_t   *st_tri = nullptr, *st_prim = nullptr, *st_geom = nullptr;
+    float      *st_ng[3] = {};
+    uint8_t    *st_occ = nullptr;
+    uint32_t   *st_cnt = nullptr;   // [0] count, [8..15] work counters, [16] error
+    std::mutex  mu;                 // serialises queries on the context stream
+};
+
+struct crt_scene {
+    crt_context           *ctx = nullptr;
+    std::vector<ModelHost> models;
+    bool                   has_sky = false;
+    std::vector<float>     sky;
+    uint32_t               sky_w = 0, sky_h = 0;
+    float                  sky_rot[2] = {0.0f, 0.0f};
+    int                    sun_enabled = 1;   // scene.h:45
+    crt_sun                sun{};
+    crt_camera             cam{};
+    bool                   committed = false;
+    bool                   dirty     = true;
+    // device
+    SceneDev               dev{};
+    void                  *d_nodes = nullptr;
+    float4                *d_tri = nullptr;
+    uint32_t              *d_mat_idx = nullptr;
+    float2                *d_uv = nullptr;
+    MaterialDev           *d_mats = nullptr;
+    float4                *d_texels = nullptr;
+    TextureDev            *d_texs = nullptr;
+    uint32_t              *d_model_first = nullptr;
+    crt_bvh_info           info{};
+    uint64_t               hash = 0;
+    uint32_t               users = 0;  // renderers referencing this scene
+};
+
+struct crt_renderer {
+    crt_context *ctx = nullptr;
+    crt_scene   *scene = nullptr;
+    uint32_t     W = 0, H = 0, bounces = 0;
+    float        aspect = 1.0f;      // renderer.h:80: 1 until set_resolution
+    uint64_t     target = 0;         // renderer.h:81: 0 = unlimited
+    crt_render_options opt{};
+    std::atomic<uint64_t> samples{0};
+    PathBuffers  b{};
+    float4      *display = nullptr;
+    std::vector<void *> allocs;
+    // background passes
+    std::thread              worker;
+    std::mutex               mu;
+    std::condition_variable  cv;
+    bool                     running = false;
+    std::atomic<bool>        stop{false};
+    int                      worker_status = CRT_OK;
+    std::string              worker_error;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    // phase timing
+    // phase timing: [0] start, [1] after generate, per bounce i < kTimedBounces:
+    // [2+4i] before extend, [3+4i] after extend, [4+4i] after shade, [5+4i] after shadow,
+    // [kEvAcc] before accumulate, [kEvEnd] end of pass
+    static constexpr int kTimedBounces = 15, kEvAcc = 62, kEvEnd = 63;
+    hipEvent_t ev[64] = {};
+    double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
+    uint64_t extend_launches = 0;
+};
+
+namespace {
+
+int ensure_spill(crt_context *ctx)
+{
+    if (ctx->spill) return CRT_OK;
+    HIP_CHECK(hipMalloc((void **)&ctx->spill, trace_spill_bytes(ctx->grid)));
+    return CRT_OK;
+}
+
+void identity16(float *m)
+{
+    for (int i = 0; i < 16; ++i) m[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+}
+
+bool is_identity(const float *m)
+{
+    for (int i = 0; i < 16; ++i)
+        if (m[i] != ((i % 5 == 0) ? 1.0f : 0.0f)) return false;
+    return true;
+}
+
+void free_scene_device(crt_scene *s)
+{
+    dfree(s->d_nodes);
+    dfree(s->d_tri);
+    dfree(s->d_mat_idx);
+    dfree(s->d_uv);
+    dfree(s->d_mats);
+    dfree(s->d_texels);
+    dfree(s->d_texs);
+    dfree(s->d_model_first);
+    s->d_nodes = nullptr;
+    s->d_tri = nullptr;
+    s->d_mat_idx = nullptr;
+    s->d_uv = nullptr;
+    s->d_mats = nullptr;
+    s->d_texels = nullptr;
+    s->d_texs = nullptr;
+    s->d_model_first = nullptr;
+    s->committed = false;
+}
+
+void free_renderer_buffers(crt_renderer *r)
+{
+    for (void *p : r->allocs) dfree(p);
+    r->allocs.clear();
+    r->b       = PathBuffers{};
+    r->display = nullptr;
+}
+
+template <class T>
+int ralloc(crt_renderer *r, T **p, size_t count)
+{
+    HIP_CHECK(dalloc(p, count));
+    r->allocs.push_back(*p);
+    return CRT_OK;
+}
+
+int alloc_rays(crt_renderer *r, RaySoA &q, size_t n)
+{
+    float **f[8] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.tn, &q.tf};
+    for (float **p : f) {
+        const int st = ralloc(r, p, n);
+        if (st) return st;
+    }
+    return CRT_OK;
+}
+
+int alloc_renderer_buffers(crt_renderer *r)
+{
+    free_renderer_buffers(r);
+    PathBuffers &b = r->b;
+    b.W = r->W;
+    b.H = r->H;
+    b.P = r->W * r->H;
+    const size_t P = b.P;
+    int st;
+#define RA(ptr, n)                          \
+    if ((st = ralloc(r, ptr, n)) != CRT_OK) return st
+    if ((st = alloc_rays(r, b.q[0], P))) return st;
+    if ((st = alloc_rays(r, b.q[1], P))) return st;
+    if ((st = alloc_rays(r, b.sq, 2 * P))) return st;
+    RA(&b.qpath[0], P);
+    RA(&b.qpath[1], P);
+    RA(&b.hits.t, P);
+    RA(&b.hits.u, P);
+    RA(&b.hits.v, P);
+    RA(&b.hits.tri, P);
+    RA(&b.shits.t, P);
+    RA(&b.shits.u, P);
+    RA(&b.shits.v, P);
+    RA(&b.shits.tri, P);
+    RA(&b.spath, 2 * P);
+    RA(&b.scontrib, 2 * P);
+    RA(&b.occ, P);
+    RA(&b.thr, P);
+    RA(&b.rad, P);
+    RA(&b.raw, 3 * P);
+    RA(&b.aov_alb, P);
+    RA(&b.aov_nrm, P);
+    RA(&b.aov_dep, P);
+    RA(&r->display, P);
+    RA(&b.cnt, (size_t)kCntWords);
+    RA(&b.stats, (size_t)4);
+#undef RA
+    hipStream_t s = r->ctx->stream;
+    HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * P, s));
+    HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4, s));
+    // image.h:30-33: new images are filled with FLT_MAX
+    HIP_CHECK(launch_fill(b.aov_alb, P, 3.402823466e+38f, s));
+    HIP_CHECK(launch_fill(b.aov_nrm, P, 3.402823466e+38f, s));
+    HIP_CHECK(launch_fill(b.aov_dep, P, 3.402823466e+38f, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return CRT_OK;
+}
+
+camera_dev to_camera_dev(const crt_camera &c)
+{
+    camera_dev d{};
+    for (int k = 0; k < 3; ++k) d.pos[k] = c.position[k];
+    d.fov = c.fov;
+    for (int k = 0; k < 16; ++k) d.m[k] = c.matrix[k];
+    d.mode  = c.mode;
+    d.scale = c.scale;
+    return d;
+}
+
+TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *count, uint32_t *work, uint32_t *err)
+{
+    TraceQueue q;
+    q.rays     = rays;
+    q.hits     = hits;
+    q.count    = count;
+    q.counters = work;
+    q.error    = err;
+    return q;
+}
+
+struct PhaseTimer {
+    crt_renderer *r;
+    bool          on;
+    int           next = 0;
+    void mark()
+    {
+        if (on && next < 16) (void)hipEventRecord(r->ev[next++], r->ctx->stream);
+    }
+};
+
+// One sample pass: renderer.cpp:121-125 (one barrier per pass) as a kernel sequence.
+// With CRT_FLAG_PHASE_TIMING, HIP events are recorded on the context stream around each
+// kernel and read after the pass's final synchronisation, so timing adds no host waits.
+int render_pass(crt_renderer *r)
+{
+    crt_context       *ctx = r->ctx;
+    const SceneDev    &sc  = r->scene->dev;
+    const PathBuffers &b   = r->b;
+    hipStream_t        s   = ctx->stream;
+    RenderParams       rp;
+    rp.cam         = to_camera_dev(r->scene->cam);
+    rp.aspect      = r->aspect;
+    rp.max_bounces = r->bounces;
+    rp.rr_start    = r->opt.rr_start;
+    rp.mode        = r->opt.mode;
+    rp.seed        = r->opt.seed;
+    rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
+    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0;
+    auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(r->ev[i], s) : hipSuccess; };
+    HIP_CHECK(hipEventRecord(r->ev[0], s));
+    HIP_CHECK(launch_generate(sc, b, rp, s));
+    HIP_CHECK(mark(1));
+    uint32_t cur = 0;
+    const uint32_t timed = std::min<uint32_t>(r->bounces, crt_renderer::kTimedBounces);
+    for (uint32_t i = 0; i < r->bounces; ++i) {
+        const bool t = timing && i < timed;
+        HIP_CHECK(launch_bounce_prep(b, cur, s));
+        if (t) HIP_CHECK(mark(2 + 4 * (int)i));
+        TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
+        HIP_CHECK(launch_trace_closest(sc, q, b.P, ctx->spill, ctx->grid, s));
+        if (t) HIP_CHECK(mark(3 + 4 * (int)i));
+        HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
+        if (t) HIP_CHECK(mark(4 + 4 * (int)i));
+        if (sc.sun_enabled) {
+            HIP_CHECK(launch_shadow_prep(b, s));
+            if (!sc.any_alpha) {
+                TraceQueue qs = make_queue(b.sq, HitSoA{}, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
+                HIP_CHECK(launch_trace_any(sc, qs, b.P, b.occ, ctx->spill, ctx->grid, s));
+                HIP_CHECK(launch_shadow_resolve(b, s));
+            } else {
+                // closest hit + alpha pass-through, renderer.cpp:330-345 (bounded at 64 steps)
+                for (int step = 0; step < 64; ++step) {
+                    TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
+                    HIP_CHECK(launch_trace_closest(sc, qs, b.P, ctx->spill, ctx->grid, s));
+                    HIP_CHECK(launch_shadow_alpha_step(sc, b, s));
+                    uint32_t left = 0;
+                    HIP_CHECK(hipMemcpyAsync(&left, b.cnt + kCntShadow, sizeof left, hipMemcpyDeviceToHost, s));
+                    HIP_CHECK(hipStreamSynchronize(s));
+                    if (left == 0) break;
+                }
+            }
+        }
+        if (t) HIP_CHECK(mark(5 + 4 * (int)i));
+        cur ^= 1u;
+    }
+    HIP_CHECK(mark(crt_renderer::kEvAcc));
+    HIP_CHECK(launch_accumulate(b, s));
+    HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvEnd], s));
+    HIP_CHECK(hipEventSynchronize(r->ev[crt_renderer::kEvEnd]));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[crt_renderer::kEvEnd]));
+    r->ms_phase[5] += ms;
+    if (timing) {
+        HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+        r->ms_phase[0] += ms;
+        for (uint32_t i = 0; i < timed; ++i) {
+            const int k = 2 + 4 * (int)i;
+            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k], r->ev[k + 1]));
+            r->ms_phase[1] += ms;
+            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k + 1], r->ev[k + 2]));
+            r->ms_phase[2] += ms;
+            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k + 2], r->ev[k + 3]));
+            r->ms_phase[3] += ms;
+        }
+        r->extend_launches += timed;
+        HIP_CHECK(hipEventElapsedTime(&ms, r->ev[crt_renderer::kEvAcc], r->ev[crt_renderer::kEvEnd]));
+        r->ms_phase[4] += ms;
+    }
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    r->samples.fetch_add(1);
+    return CRT_OK;
+}
+
+int check_ready(crt_renderer *r)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    if (!r->scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    return CRT_OK;
+}
+
+void worker_loop(crt_renderer *r)
+{
+    int st = CRT_OK;
+    for (;;) {
+        if (r->stop.load()) break;
+        if (r->target != 0 && r->samples.load() >= r->target) break;
+        st = render_pass(r);
+        if (st != CRT_OK) break;
+    }
+    std::lock_guard<std::mutex> g(r->mu);
+    r->worker_status = st;
+    if (st != CRT_OK) r->worker_error = g_err;
+    r->running = false;
+    r->cv.notify_all();
+}
+
+int join_worker(crt_renderer *r)
+{
+    if (r->worker.joinable()) r->worker.join();
+    return CRT_OK;
+}
+
+int reset_accum(crt_renderer *r)
+{
+    hipStream_t s = r->ctx->stream;
+    HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.P, s));
+    HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4, s));
+    HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    r->samples.store(0);
+    for (double &m : r->ms_phase) m = 0.0;
+    r->extend_launches = 0;
+    r->t0 = std::chrono::steady_clock::now();
+    return CRT_OK;
+}
+
+int stage_reserve(crt_context *ctx, uint32_t n)
+{
+    if (n <= ctx->stage_cap && ctx->st_cnt) return CRT_OK;
+    for (float *&p : ctx->st_ray) dfree(p), p = nullptr;
+    dfree(ctx->st_t);
+    dfree(ctx->st_u);
+    dfree(ctx->st_v);
+    dfree(ctx->st_tri);
+    dfree(ctx->st_prim);
+    dfree(ctx->st_geom);
+    for (float *&p : ctx->st_ng) dfree(p), p = nullptr;
+    dfree(ctx->st_occ);
+    dfree(ctx->st_cnt);
+    const uint32_t cap = std::max<uint32_t>(n, 1024);
+    for (float *&p : ctx->st_ray) HIP_CHECK(dalloc(&p, cap));
+    HIP_CHECK(dalloc(&ctx->st_t, cap));
+    HIP_CHECK(dalloc(&ctx->st_u, cap));
+    HIP_CHECK(dalloc(&ctx->st_v, cap));
+    HIP_CHECK(dalloc(&ctx->st_tri, cap));
+    HIP_CHECK(dalloc(&ctx->st_prim, cap));
+    HIP_CHECK(dalloc(&ctx->st_geom, cap));
+    for (float *&p : ctx->st_ng) HIP_CHECK(dalloc(&p, cap));
+    HIP_CHECK(dalloc(&ctx->st_occ, cap));
+    HIP_CHECK(dalloc(&ctx->st_cnt, 32));
+    ctx->stage_cap = cap;
+    return CRT_OK;
+}
+
+int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, uint8_t *occluded, int memory)
+{
+    if (!scene || !rays) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (!scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    if (memory != CRT_MEM_HOST && memory != CRT_MEM_DEVICE) return fail(CRT_ERR_INVALID_ARG, "bad memory kind");
+    const float *in[8] = {rays->org_x, rays->org_y, rays->org_z, rays->dir_x, rays->dir_y, rays->dir_z, rays->tnear, rays->tfar};
+    for (const float *p : in)
+        if (!p && n) return fail(CRT_ERR_INVALID_ARG, "null ray component");
+    if (hits && (!hits->t || !hits->prim_id) && n) return fail(CRT_ERR_INVALID_ARG, "hits.t and hits.prim_id are required");
+    if (n == 0) return CRT_OK;
+    crt_context *ctx = scene->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int st = stage_reserve(ctx, n);
+    if (st) return st;
+    if ((st = ensure_spill(ctx))) return st;
+    hipStream_t s = ctx->stream;
+    RaySoA      rs;
+    float     **dst[8] = {&rs.ox, &rs.oy, &rs.oz, &rs.dx, &rs.dy, &rs.dz, &rs.tn, &rs.tf};
+    for (int k = 0; k < 8; ++k) {
+        if (memory == CRT_MEM_HOST) {
+            HIP_CHECK(hipMemcpyAsync(ctx->st_ray[k], in[k], sizeof(float) * n, hipMemcpyHostToDevice, s));
+            *dst[k] = ctx->st_ray[k];
+        } else {
+            *dst[k] = const_cast<float *>(in[k]);
+        }
+    }
+    HIP_CHECK(hipMemsetAsync(ctx->st_cnt, 0, sizeof(uint32_t) * 32, s));
+    HIP_CHECK(hipMemcpyAsync(ctx->st_cnt, &n, sizeof n, hipMemcpyHostToDevice, s));
+    if (hits) {
+        HitSoA h;
+        const bool dev = memory == CRT_MEM_DEVICE;
+        h.t   = dev ? hits->t : ctx->st_t;
+        h.u   = dev && hits->u ? hits->u : ctx->st_u;
+        h.v   = dev && hits->v ? hits->v : ctx->st_v;
+        h.tri = ctx->st_tri;
+        TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
+        HIP_CHECK(launch_trace_closest(scene->dev, q, n, ctx->spill, ctx->grid, s));
+        uint32_t *prim = dev ? hits->prim_id : ctx->st_prim;
+        uint32_t *geom = dev ? hits->geom_id : (hits->geom_id ? ctx->st_geom : nullptr);
+        float    *ng[3] = {hits->ng_x, hits->ng_y, hits->ng_z};
+        float    *ngd[3];
+        for (int k = 0; k < 3; ++k) ngd[k] = ng[k] ? (dev ? ng[k] : ctx->st_ng[k]) : nullptr;
+        HIP_CHECK(launch_hits_to_api(scene->dev, h, n, prim, geom, ngd[0], ngd[1], ngd[2], s));
+        if (!dev) {
+            HIP_CHECK(hipMemcpyAsync(hits->t, ctx->st_t, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            if (hits->u) HIP_CHECK(hipMemcpyAsync(hits->u, ctx->st_u, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            if (hits->v) HIP_CHECK(hipMemcpyAsync(hits->v, ctx->st_v, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(hits->prim_id, ctx->st_prim, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+            if (hits->geom_id)
+                HIP_CHECK(hipMemcpyAsync(hits->geom_id, ctx->st_geom, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+            for (int k = 0; k < 3; ++k)
+                if (ng[k]) HIP_CHECK(hipMemcpyAsync(ng[k], ctx->st_ng[k], sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        }
+    } else {
+        TraceQueue q = make_queue(rs, HitSoA{}, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
+        uint8_t   *occ = memory == CRT_MEM_DEVICE ? occluded : ctx->st_occ;
+        HIP_CHECK(launch_trace_any(scene->dev, q, n, occ, ctx->spill, ctx->grid, s));
+        if (memory == CRT_MEM_HOST) HIP_CHECK(hipMemcpyAsync(occluded, ctx->st_occ, n, hipMemcpyDeviceToHost, s));
+    }
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpyAsync(&err, ctx->st_cnt + 16, sizeof err, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *crt_last_error(void) { return g_err.c_str(); }
+int         crt_abi_version(void) { return CRT_ABI_VERSION; }
+
+void crt_options_default(crt_options *opt)
+{
+    if (!opt) return;
+    std::memset(opt, 0, sizeof *opt);
+    opt->device    = 0;
+    opt->bvh_width = 0;
+    opt->max_leaf  = 0;
+}
+
+int crt_context_create(const crt_options *opt, crt_context **out)
+{
+    if (!out) return fail(CRT_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    crt_options o;
+    crt_options_default(&o);
+    if (opt) o = *opt;
+    if (o.bvh_width != 0 && o.bvh_width != 4 && o.bvh_width != 8)
+        return fail(CRT_ERR_INVALID_ARG, "bvh_width must be 0, 4 or 8");
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (o.device < 0 || o.device >= ndev) return fail(CRT_ERR_INVALID_ARG, "no such HIP device");
+    HIP_CHECK(hipSetDevice(o.device));
+    auto *ctx   = new crt_context();
+    ctx->device = o.device;
+    ctx->opt    = o;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(CRT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    ctx->grid = trace_grid(o.device, o.trace_waves_per_cu ? (o.trace_waves_per_cu + 3) / 4 : 0);
+    *out      = ctx;
+    return CRT_OK;
+}
+
+int crt_context_destroy(crt_context *ctx)
+{
+    if (!ctx) return CRT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->spill);
+    for (float *p : ctx->st_ray) dfree(p);
+    dfree(ctx->st_t);
+    dfree(ctx->st_u);
+    dfree(ctx->st_v);
+    dfree(ctx->st_tri);
+    dfree(ctx->st_prim);
+    dfree(ctx->st_geom);
+    for (float *p : ctx->st_ng) dfree(p);
+    dfree(ctx->st_occ);
+    dfree(ctx->st_cnt);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CRT_OK;
+}
+
+int crt_context_device_name(crt_context *ctx, char *buf, size_t len)
+{
+    if (!ctx || !buf || len == 0) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    hipDeviceProp_t p;
+    HIP_CHECK(hipGetDeviceProperties(&p, ctx->device));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return CRT_OK;
+}
+
+void crt_camera_default(crt_camera *cam)
+{
+    if (!cam) return;
+    std::memset(cam, 0, sizeof *cam);
+    cam->position[0] = 5.0f;
+    cam->position[1] = 5.0f;
+    cam->fov         = 75.0f;
+    cam->scale       = 1.0f;
+    identity16(cam->matrix);
+}
+
+// glm::rotate(m, angle, axis) = m * R(angle, axis) (Rodrigues, axis normalised)
+static void rot_mul(float *m, float deg, float ax, float ay, float az)
+{
+    const float a = deg * 0.01745329251994329576923690768489f;
+    const float c = std::cos(a), s = std::sin(a);
+    const float t[3] = {(1.0f - c) * ax, (1.0f - c) * ay, (1.0f - c) * az};
+    float R[9];  // column-major 3x3
+    R[0] = c + t[0] * ax;
+    R[1] = t[0] * ay + s * az;
+    R[2] = t[0] * az - s * ay;
+    R[3] = t[1] * ax - s * az;
+    R[4] = c + t[1] * ay;
+    R[5] = t[1] * az + s * ax;
+    R[6] = t[2] * ax + s * ay;
+    R[7] = t[2] * ay - s * ax;
+    R[8] = c + t[2] * az;
+    float out[16];
+    std::memcpy(out, m, sizeof out);
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 4; ++row)
+            out[col * 4 + row] = m[0 * 4 + row] * R[col * 3 + 0] + m[1 * 4 + row] * R[col * 3 + 1] + m[2 * 4 + row] * R[col * 3 + 2];
+    std::memcpy(m, out, sizeof out);
+}
+
+void crt_camera_update(crt_camera *cam)
+{
+    if (!cam) return;
+    float m[16];
+    identity16(m);
+    // glm::translate(m, p): column 3 += m * p
+    for (int row = 0; row < 4; ++row)
+        m[12 + row] = m[0 + row] * cam->position[0] + m[4 + row] * cam->position[1] + m[8 + row] * cam->position[2] + m[12 + row];
+    rot_mul(m, cam->rotation[0], 0.0f, 1.0f, 0.0f);
+    rot_mul(m, cam->rotation[1], 1.0f, 0.0f, 0.0f);
+    rot_mul(m, cam->rotation[2], 0.0f, 0.0f, 1.0f);
+    std::memcpy(cam->matrix, m, sizeof m);
+}
+
+void crt_sun_default(crt_sun *sun)
+{
+    if (!sun) return;
+    sun->size      = 3.14159265359f / 48.0f;
+    sun->intensity = 100.0f;
+    const float inv = 1.0f / std::sqrt(0.8f * 0.8f + 1.0f);
+    sun->direction[0] = 0.8f * inv;
+    sun->direction[1] = -1.0f * inv;
+    sun->direction[2] = 0.0f;
+    sun->colour[0] = 1.0f;
+    sun->colour[1] = 0.9f;
+    sun->colour[2] = 0.7f;
+}
+
+void crt_material_default(crt_material *m)
+{
+    if (!m) return;
+    m->type           = CRT_SMOOTH;
+    m->ior            = 1.5f;
+    m->roughness      = 0.5f;
+    m->reflectiveness = 1.0f;
+    m->emission       = 0.0f;
+    for (int k = 0; k < 4; ++k) m->colour[k] = 1.0f;
+    m->tex = -1;
+}
+
+int crt_scene_create(crt_context *ctx, crt_scene **out)
+{
+    if (!ctx || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    auto *s = new crt_scene();
+    s->ctx  = ctx;
+    crt_sun_default(&s->sun);
+    crt_camera_default(&s->cam);
+    *out = s;
+    return CRT_OK;
+}
+
+int crt_scene_destroy(crt_scene *scene)
+{
+    if (!scene) return CRT_OK;
+    if (scene->users) return fail(CRT_ERR_STATE, "scene still used by a renderer");
+    (void)hipSetDevice(scene->ctx->device);
+    free_scene_device(scene);
+    delete scene;
+    return CRT_OK;
+}
+
+int crt_scene_add_model(crt_scene *scene, const crt_model_data *md, uint32_t *model_id)
+{
+    if (!scene || !md) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    const uint32_t n = md->num_triangles;
+    if (n && (!md->vertices || !md->vertex_indices || !md->material_indices))
+        return fail(CRT_ERR_INVALID_ARG, "vertices, vertex_indices and material_indices are required");
+    if (md->num_materials && !md->materials) return fail(CRT_ERR_INVALID_ARG, "null materials");
+    if (md->num_textures && !md->textures) return fail(CRT_ERR_INVALID_ARG, "null textures");
+    ModelHost m;
+    m.name = md->name ? md->name : "";
+    m.mats.assign(md->materials, md->materials + md->num_materials);
+    for (uint32_t i = 0; i < md->num_materials; ++i) {
+        const crt_material &mt = m.mats[i];
+        if (mt.type < 0 || mt.type > 2) return fail(CRT_ERR_INVALID_ARG, "material type out of range");
+        if (mt.tex >= 0 && (uint32_t)mt.tex >= md->num_textures)
+            return fail(CRT_ERR_INVALID_ARG, "material texture index out of range");
+    }
+    for (uint32_t i = 0; i < md->num_textures; ++i) {
+        const crt_image &im = md->textures[i];
+        if (!im.rgba || im.width == 0 || im.height == 0) return fail(CRT_ERR_INVALID_ARG, "empty texture");
+        m.tex_data.emplace_back(im.rgba, im.rgba + (size_t)im.width * im.height * 4);
+        m.tex_dims.emplace_back(im.width, im.height);
+    }
+    // registry.cpp:13-24,62-66: expand per corner, primID = triangle order
+    m.corners.resize((size_t)n * 9);
+    m.mat_idx.resize(n);
+    const bool has_uv = md->texture_coords && md->texture_indices && md->num_texture_coords;
+    if (has_uv) m.uv.resize((size_t)n * 6);
+    for (uint32_t t = 0; t < n; ++t) {
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t vi = md->vertex_indices[3 * (size_t)t + c];
+            if (vi >= md->num_vertices) return fail(CRT_ERR_INVALID_MESH, "vertex index out of range at triangle " + std::to_string(t));
+            for (int k = 0; k < 3; ++k) {
+                const float x = md->vertices[3 * (size_t)vi + k];
+                if (!std::isfinite(x)) return fail(CRT_ERR_INVALID_MESH, "non-finite vertex at triangle " + std::to_string(t));
+                m.corners[9 * (size_t)t + 3 * c + k] = x;
+            }
+            if (has_uv) {
+                const uint32_t ti = md->texture_indices[3 * (size_t)t + c];
+                if (ti >= md->num_texture_coords) return fail(CRT_ERR_INVALID_MESH, "texture index out of range at triangle " + std::to_string(t));
+                m.uv[6 * (size_t)t + 2 * c + 0] = md->texture_coords[2 * (size_t)ti + 0];
+                m.uv[6 * (size_t)t + 2 * c + 1] = md->texture_coords[2 * (size_t)ti + 1];
+            }
+        }
+        const uint32_t mi = md->material_indices[t];
+        if (mi >= md->num_materials) return fail(CRT_ERR_INVALID_MESH, "material index out of range at triangle " + std::to_string(t));
+        m.mat_idx[t] = mi;
+    }
+    m.instances.resize(16);
+    identity16(m.instances.data());
+    scene->models.push_back(std::move(m));
+    scene->dirty = true;
+    if (model_id) *model_id = (uint32_t)scene->models.size() - 1;
+    return CRT_OK;
+}
+
+int crt_scene_set_instances(crt_scene *scene, uint32_t model_id, const float *transforms, uint32_t count)
+{
+    if (!scene || (!transforms && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (model_id >= scene->models.size()) return fail(CRT_ERR_INVALID_ARG, "no such model");
+    if (count != 1 || !is_identity(transforms))
+        return fail(CRT_ERR_UNSUPPORTED, "only the single identity instance is supported");
+    scene->models[model_id].instances.assign(transforms, transforms + 16 * (size_t)count);
+    scene->dirty = true;
+    return CRT_OK;
+}
+
+int crt_scene_set_materials(crt_scene *scene, uint32_t model_id, const crt_material *mats, uint32_t count)
+{
+    if (!scene || (!mats && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (model_id >= scene->models.size()) return fail(CRT_ERR_INVALID_ARG, "no such model");
+    ModelHost &m = scene->models[model_id];
+    for (uint32_t t = 0; t < m.mat_idx.size(); ++t)
+        if (m.mat_idx[t] >= count) return fail(CRT_ERR_INVALID_ARG, "material table smaller than the indices need");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (mats[i].type < 0 || mats[i].type > 2) return fail(CRT_ERR_INVALID_ARG, "material type out of range");
+        if (mats[i].tex >= 0 && (size_t)mats[i].tex >= m.tex_data.size())
+            return fail(CRT_ERR_INVALID_ARG, "material texture index out of range");
+    }
+    m.mats.assign(mats, mats + count);
+    scene->dirty = true;
+    return CRT_OK;
+}
+
+int crt_scene_set_skybox(crt_scene *scene, const crt_image *sky)
+{
+    if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
+    if (!sky || !sky->rgba) {
+        scene->has_sky = false;
+        scene->sky.clear();
+    } else {
+        if (sky->width == 0 || sky->height == 0) return fail(CRT_ERR_INVALID_ARG, "empty skybox");
+        scene->has_sky = true;
+        scene->sky.assign(sky->rgba, sky->rgba + (size_t)sky->width * sky->height * 4);
+        scene->sky_w = sky->width;
+        scene->sky_h = sky->height;
+    }
+    scene->dirty = true;
+    return CRT_OK;
+}
+
+int crt_scene_set_skybox_rotation(crt_scene *scene, float x, float y)
+{
+    if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
+    scene->sky_rot[0] = x;
+    scene->sky_rot[1] = y;
+    scene->dev.sky_rot[0] = x;
+    scene->dev.sky_rot[1] = y;
+    return CRT_OK;
+}
+
+int crt_scene_set_sun_enabled(crt_scene *scene, int enabled)
+{
+    if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
+    scene->sun_enabled     = enabled ? 1 : 0;
+    scene->dev.sun_enabled = scene->sun_enabled;
+    return CRT_OK;
+}
+
+int crt_scene_set_sun(crt_scene *scene, const crt_sun *sun)
+{
+    if (!scene || !sun) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (!(sun->size > 0.0f)) return fail(CRT_ERR_INVALID_ARG, "sun size must be positive");
+    scene->sun = *sun;
+    scene->dev.sun.size      = sun->size;
+    scene->dev.sun.intensity = sun->intensity;
+    scene->dev.sun.dir       = mk3(sun->direction[0], sun->direction[1], sun->direction[2]);
+    scene->dev.sun.colour    = mk3(sun->colour[0], sun->colour[1], sun->colour[2]);
+    return CRT_OK;
+}
+
+int crt_scene_set_camera(crt_scene *scene, const crt_camera *camera)
+{
+    if (!scene || !camera) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    scene->cam = *camera;
+    return CRT_OK;
+}
+
+int crt_scene_commit(crt_scene *scene)
+{
+    if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
+    crt_context *ctx = scene->ctx;
+    HIP_CHECK(hipSetDevice(ctx->device));
+    free_scene_device(scene);
+    // concatenate models (scene.cpp:83-95 loops over them; one BVH over all here)
+    std::vector<float>       corners, uv;
+    std::vector<uint32_t>    mat_idx, model_first;
+    std::vector<MaterialDev> mats;
+    std::vector<TextureDev>  texs;
+    std::vector<float>       texels;
+    bool any_uv = false;
+    for (auto &m : scene->models) any_uv |= !m.uv.empty();
+    bool any_alpha = false;
+    uint64_t h = 1469598103934665603ull;
+    for (auto &m : scene->models) {
+        const uint32_t mat_base = (uint32_t)mats.size();
+        const uint32_t tex_base = (uint32_t)texs.size();
+        model_first.push_back((uint32_t)mat_idx.size());
+        for (size_t i = 0; i < m.tex_data.size(); ++i) {
+            TextureDev td;
+            td.offset = texels.size() / 4;
+            td.w      = m.tex_dims[i].first;
+            td.h      = m.tex_dims[i].second;
+            texs.push_back(td);
+            texels.insert(texels.end(), m.tex_data[i].begin(), m.tex_data[i].end());
+        }
+        for (auto &mt : m.mats) {
+            MaterialDev d;
+            d.type           = mt.type;
+            d.ior            = mt.ior;
+            d.roughness      = mt.roughness;
+            d.reflectiveness = mt.reflectiveness;
+            d.emission       = mt.emission;
+            for (int k = 0; k < 4; ++k) d.colour[k] = mt.colour[k];
+            d.tex = mt.tex >= 0 ? (int32_t)(tex_base + (uint32_t)mt.tex) : -1;
+            if (mt.tex >= 0 || mt.colour[3] == 0.0f) any_alpha = true;
+            mats.push_back(d);
+        }
+        corners.insert(corners.end(), m.corners.begin(), m.corners.end());
+        for (uint32_t t : m.mat_idx) mat_idx.push_back(mat_base + t);
+        if (any_uv) {
+            if (m.uv.empty()) uv.insert(uv.end(), m.corners.size() / 9 * 6, 0.0f);
+            else uv.insert(uv.end(), m.uv.begin(), m.uv.end());
+        }
+        h = fnv1a(h, m.corners.data(), m.corners.size() * sizeof(float));
+        h = fnv1a(h, m.mat_idx.data(), m.mat_idx.size() * sizeof(uint32_t));
+        h = fnv1a(h, m.mats.data(), m.mats.size() * sizeof(crt_material));
+    }
+    int32_t sky_tex = -1;
+    if (scene->has_sky) {
+        TextureDev td;
+        td.offset = texels.size() / 4;
+        td.w      = scene->sky_w;
+        td.h      = scene->sky_h;
+        sky_tex   = (int32_t)texs.size();
+        texs.push_back(td);
+        texels.insert(texels.end(), scene->sky.begin(), scene->sky.end());
+        h = fnv1a(h, scene->sky.data(), scene->sky.size() * sizeof(float));
+    }
+    const uint32_t ntri = (uint32_t)mat_idx.size();
+    if (corners.size() / 9 >= (1u << 27)) return fail(CRT_ERR_UNSUPPORTED, "more than 2^27 triangles");
+    hipStream_t s = ctx->stream;
+    float *d_corners = nullptr;
+    HIP_CHECK(dalloc(&d_corners, corners.size()));
+    if (!corners.empty()) HIP_CHECK(hipMemcpyAsync(d_corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice, s));
+    const uint32_t width = ctx->opt.bvh_width ? ctx->opt.bvh_width : 8u;
+    const uint32_t max_leaf = ctx->opt.max_leaf ? ctx->opt.max_leaf : 4u;
+    BuildResult br{};
+    hipError_t  be = build_bvh(d_corners, ntri, width, max_leaf, s, &br);
+    dfree(d_corners);
+    if (be != hipSuccess) return fail(CRT_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(be));
+    scene->d_nodes = br.nodes;
+    scene->d_tri   = br.tri;
+    HIP_CHECK(dalloc(&scene->d_mat_idx, mat_idx.size()));
+    HIP_CHECK(dalloc(&scene->d_mats, std::max<size_t>(mats.size(), 1)));
+    if (!mat_idx.empty()) HIP_CHECK(hipMemcpyAsync(scene->d_mat_idx, mat_idx.data(), sizeof(uint32_t) * mat_idx.size(), hipMemcpyHostToDevice, s));
+    if (!mats.empty()) HIP_CHECK(hipMemcpyAsync(scene->d_mats, mats.data(), sizeof(MaterialDev) * mats.size(), hipMemcpyHostToDevice, s));
+    if (any_uv && !uv.empty()) {
+        HIP_CHECK(dalloc(&scene->d_uv, uv.size() / 2));
+        HIP_CHECK(hipMemcpyAsync(scene->d_uv, uv.data(), sizeof(float) * uv.size(), hipMemcpyHostToDevice, s));
+    }
+    if (!texs.empty()) {
+        HIP_CHECK(dalloc(&scene->d_texs, texs.size()));
+        HIP_CHECK(dalloc(&scene->d_texels, texels.size() / 4));
+        HIP_CHECK(hipMemcpyAsync(scene->d_texs, texs.data(), sizeof(TextureDev) * texs.size(), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(scene->d_texels, texels.data(), sizeof(float) * texels.size(), hipMemcpyHostToDevice, s));
+    }
+    HIP_CHECK(dalloc(&scene->d_model_first, std::max<size_t>(model_first.size(), 1)));
+    if (!model_first.empty())
+        HIP_CHECK(hipMemcpyAsync(scene->d_model_first, model_first.data(), sizeof(uint32_t) * model_first.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+
+    const uint32_t max_stack = ntri ? (width - 1) * br.depth + 1 : 0;
+    if (max_stack > kStackLds + kSpillDepth) {
+        free_scene_device(scene);
+        return fail(CRT_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
+    }
+    SceneDev &d = scene->dev;
+    d = SceneDev{};
+    d.tri         = scene->d_tri;
+    d.nodes       = scene->d_nodes;
+    d.width       = width;
+    d.ntri        = ntri;
+    d.root        = 0;
+    d.mat_idx     = scene->d_mat_idx;
+    d.uv          = scene->d_uv;
+    d.mats        = scene->d_mats;
+    d.nmat        = (uint32_t)mats.size();
+    d.texels      = scene->d_texels;
+    d.texs        = scene->d_texs;
+    d.ntex        = (uint32_t)texs.size() - (sky_tex >= 0 ? 1u : 0u);
+    d.sky_tex     = sky_tex;
+    d.sky_rot[0]  = scene->sky_rot[0];
+    d.sky_rot[1]  = scene->sky_rot[1];
+    d.sun_enabled = scene->sun_enabled;
+    d.sun.size      = scene->sun.size;
+    d.sun.intensity = scene->sun.intensity;
+    d.sun.dir       = mk3(scene->sun.direction[0], scene->sun.direction[1], scene->sun.direction[2]);
+    d.sun.colour    = mk3(scene->sun.colour[0], scene->sun.colour[1], scene->sun.colour[2]);
+    d.any_alpha   = any_alpha ? 1 : 0;
+    d.model_first = scene->d_model_first;
+    d.nmodels     = (uint32_t)model_first.size();
+    crt_bvh_info &bi = scene->info;
+    bi.width           = width;
+    bi.node_count      = br.node_count;
+    bi.bvh2_node_count = br.bvh2_nodes;
+    bi.depth           = br.depth;
+    bi.max_stack       = max_stack;
+    bi.sah_cost        = br.sah;
+    bi.build_ms        = br.ms;
+    bi.bytes           = (uint64_t)br.node_count * 128u + (uint64_t)ntri * 48u;
+    scene->hash      = h;
+    scene->committed = true;
+    scene->dirty     = false;
+    return ensure_spill(ctx);
+}
+
+int crt_scene_bvh_info(crt_scene *scene, crt_bvh_info *info)
+{
+    if (!scene || !info) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (!scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    *info = scene->info;
+    return CRT_OK;
+}
+
+int crt_scene_bvh_export(crt_scene *scene, void *nodes, uint64_t *node_bytes, uint32_t *prims, uint32_t *nprims)
+{
+    if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
+    if (!scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    const uint64_t nb = (uint64_t)scene->info.node_count * 128u;
+    const uint32_t np = scene->dev.ntri;
+    if (nodes && node_bytes && *node_bytes >= nb && nb)
+        HIP_CHECK(hipMemcpy(nodes, scene->d_nodes, nb, hipMemcpyDeviceToHost));
+    if (prims && nprims && *nprims >= np && np) {
+        std::vector<float4> tri((size_t)np * 3);
+        HIP_CHECK(hipMemcpy(tri.data(), scene->d_tri, sizeof(float4) * tri.size(), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < np; ++i) {
+            uint32_t id;
+            std::memcpy(&id, &tri[3 * (size_t)i].w, 4);
+            prims[i] = id;
+        }
+    }
+    if (node_bytes) *node_bytes = nb;
+    if (nprims) *nprims = np;
+    return CRT_OK;
+}
+
+int crt_intersect(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, int memory)
+{
+    if (!hits) return fail(CRT_ERR_INVALID_ARG, "null hits");
+    return query(scene, rays, n, hits, nullptr, memory);
+}
+
+int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory)
+{
+    if (!occluded && n) return fail(CRT_ERR_INVALID_ARG, "null occluded");
+    return query(scene, rays, n, nullptr, occluded, memory);
+}
+
+int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint32_t res_y, uint32_t bounces,
+                        crt_renderer **out)
+{
+    if (!ctx || !scene || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (res_x == 0 || res_y == 0) return fail(CRT_ERR_INVALID_ARG, "resolution must be positive");
+    if ((uint64_t)res_x * res_y > (1ull << 30)) return fail(CRT_ERR_INVALID_ARG, "resolution too large");
+    if (scene->ctx != ctx) return fail(CRT_ERR_INVALID_ARG, "scene belongs to another context");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    auto *r    = new crt_renderer();
+    r->ctx     = ctx;
+    r->scene   = scene;
+    r->W       = res_x;
+    r->H       = res_y;
+    r->bounces = bounces;
+    r->opt     = crt_render_options{CRT_MODE_COMPAT, 0u, 0u, 0u};
+    for (hipEvent_t &e : r->ev) {
+        hipError_t ee = hipEventCreate(&e);
+        if (ee != hipSuccess) {
+            delete r;
+            return fail(CRT_ERR_HIP, "hipEventCreate");
+        }
+    }
+    const int st = alloc_renderer_buffers(r);
+    if (st != CRT_OK) {
+        free_renderer_buffers(r);
+        for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
+        delete r;
+        return st;
+    }
+    scene->users++;
+    *out = r;
+    return CRT_OK;
+}
+
+int crt_renderer_destroy(crt_renderer *r)
+{
+    if (!r) return CRT_OK;
+    r->stop.store(true);
+    join_worker(r);
+    (void)hipSetDevice(r->ctx->device);
+    free_renderer_buffers(r);
+    for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
+    r->scene->users--;
+    delete r;
+    return CRT_OK;
+}
+
+int crt_renderer_start(crt_renderer *r)
+{
+    int st = check_ready(r);
+    if (st) return st;
+    {
+        std::lock_guard<std::mutex> g(r->mu);
+        if (r->running) return CRT_OK;  // renderer.cpp:156: start() only acts when paused
+    }
+    join_worker(r);
+    if ((st = reset_accum(r))) return st;
+    r->stop.store(false);
+    r->worker_status = CRT_OK;
+    {
+        std::lock_guard<std::mutex> g(r->mu);
+        r->running = true;
+    }
+    r->worker = std::thread(worker_loop, r);
+    return CRT_OK;
+}
+
+int crt_renderer_pause(crt_renderer *r)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    r->stop.store(true);
+    join_worker(r);
+    if (r->worker_status != CRT_OK) return fail(r->worker_status, r->worker_error);
+    return CRT_OK;
+}
+
+int crt_renderer_cancel(crt_renderer *r)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    r->stop.store(true);
+    return CRT_OK;
+}
+
+int crt_renderer_is_running(crt_renderer *r, int *running)
+{
+    if (!r || !running) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(r->mu);
+    *running = r->running ? 1 : 0;
+    return CRT_OK;
+}
+
+int crt_renderer_wait(crt_renderer *r)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    if (r->target == 0) {
+        std::lock_guard<std::mutex> g(r->mu);
+        if (r->running) return fail(CRT_ERR_STATE, "no target spp set: wait would never return");
+    }
+    join_worker(r);
+    if (r->worker_status != CRT_OK) return fail(r->worker_status, r->worker_error);
+    return CRT_OK;
+}
+
+static int require_paused(crt_renderer *r)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    std::lock_guard<std::mutex> g(r->mu);
+    if (r->running) return fail(CRT_ERR_STATE, "renderer is running; pause it first");
+    return CRT_OK;
+}
+
+int crt_renderer_set_resolution(crt_renderer *r, uint32_t x, uint32_t y)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    if (x == 0 || y == 0) return fail(CRT_ERR_INVALID_ARG, "resolution must be positive");
+    join_worker(r);
+    r->W      = x;
+    r->H      = y;
+    r->aspect = (float)x / (float)y;   // renderer.cpp:199
+    if ((st = alloc_renderer_buffers(r))) return st;
+    r->samples.store(0);
+    return CRT_OK;
+}
+
+int crt_renderer_set_max_bounces(crt_renderer *r, uint32_t bounces)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    r->bounces = bounces;
+    return CRT_OK;
+}
+
+int crt_renderer_set_target_spp(crt_renderer *r, uint64_t target)
+{
+    if (!r) return fail(CRT_ERR_INVALID_ARG, "null renderer");
+    r->target = target;
+    return CRT_OK;
+}
+
+int crt_renderer_set_options(crt_renderer *r, const crt_render_options *opt)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    if (!opt) return fail(CRT_ERR_INVALID_ARG, "null options");
+    if (opt->mode != CRT_MODE_COMPAT && opt->mode != CRT_MODE_PBR) return fail(CRT_ERR_INVALID_ARG, "bad mode");
+    r->opt = *opt;
+    return CRT_OK;
+}
+
+int crt_renderer_get_options(crt_renderer *r, crt_render_options *opt)
+{
+    if (!r || !opt) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    *opt = r->opt;
+    return CRT_OK;
+}
+
+int crt_renderer_render(crt_renderer *r, uint32_t passes)
+{
+    int st = check_ready(r);
+    if (st) return st;
+    if ((st = require_paused(r))) return st;
+    join_worker(r);
+    HIP_CHECK(hipSetDevice(r->ctx->device));
+    r->stop.store(false);
+    for (uint32_t i = 0; i < passes; ++i) {
+        if (r->stop.load()) return fail(CRT_ERR_CANCELLED, "cancelled");
+        if ((st = render_pass(r))) return st;
+    }
+    return CRT_OK;
+}
+
+int crt_renderer_reset(crt_renderer *r)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    join_worker(r);
+    return reset_accum(r);
+}
+
+int crt_renderer_current_sample_count(crt_renderer *r, uint64_t *count)
+{
+    if (!r || !count) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    *count = r->samples.load();
+    return CRT_OK;
+}
+
+int crt_renderer_current_resolution(crt_renderer *r, uint32_t *x, uint32_t *y)
+{
+    if (!r || !x || !y) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    *x = r->W;
+    *y = r->H;
+    return CRT_OK;
+}
+
+static int buffer_ptr(crt_renderer *r, int buffer, void **ptr, size_t *bytes, bool refresh)
+{
+    const size_t P = r->b.P;
+    switch (buffer) {
+    case CRT_BUFFER_PROGRESS:
+        if (refresh) {
+            HIP_CHECK(launch_display(r->b.raw, r->samples.load(), r->b.P, r->display, r->ctx->stream));
+        }
+        *ptr   = r->display;
+        *bytes = P * 16;
+        return CRT_OK;
+    case CRT_BUFFER_NORMALS: *ptr = r->b.aov_nrm; *bytes = P * 16; return CRT_OK;
+    case CRT_BUFFER_ALBEDOS: *ptr = r->b.aov_alb; *bytes = P * 16; return CRT_OK;
+    case CRT_BUFFER_DEPTHS: *ptr = r->b.aov_dep; *bytes = P * 16; return CRT_OK;
+    case CRT_BUFFER_RAW: *ptr = r->b.raw; *bytes = P * 12; return CRT_OK;
+    default: return fail(CRT_ERR_INVALID_ARG, "unknown buffer");
+    }
+}
+
+int crt_renderer_read(crt_renderer *r, int buffer, float *out)
+{
+    if (!r || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    HIP_CHECK(hipSetDevice(r->ctx->device));
+    void  *p = nullptr;
+    size_t bytes = 0;
+    int    st = buffer_ptr(r, buffer, &p, &bytes, true);
+    if (st) return st;
+    HIP_CHECK(hipMemcpyAsync(out, p, bytes, hipMemcpyDeviceToHost, r->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
+    return CRT_OK;
+}
+
+int crt_renderer_device_buffer(crt_renderer *r, int buffer, void **dev_ptr)
+{
+    if (!r || !dev_ptr) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    size_t bytes = 0;
+    int    st = buffer_ptr(r, buffer, dev_ptr, &bytes, true);
+    if (st) return st;
+    HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
+    return CRT_OK;
+}
+
+int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
+{
+    if (!r || !stats) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    unsigned long long s[4] = {0, 0, 0, 0};
+    HIP_CHECK(hipMemcpy(s, r->b.stats, sizeof s, hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof *stats);
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - r->t0).count();
+    stats->running_time       = t;
+    stats->samples            = r->samples.load();
+    stats->total_rays         = s[2];
+    stats->extension_rays     = s[0];
+    stats->shadow_rays        = s[1];
+    stats->rays_per_second    = t > 0 ? (uint64_t)((double)s[2] / t) : 0;
+    stats->samples_per_second = t > 0 ? (uint64_t)((double)stats->samples / t) : 0;
+    stats->ms_generate   = r->ms_phase[0];
+    stats->ms_extend     = r->ms_phase[1];
+    stats->ms_shade      = r->ms_phase[2];
+    stats->ms_shadow     = r->ms_phase[3];
+    stats->ms_accumulate = r->ms_phase[4];
+    stats->ms_render     = r->ms_phase[5];
+    stats->extend_launches = r->extend_launches;
+    return CRT_OK;
+}
+
+struct CkptHeader {
+    char     magic[8];
+    uint32_t version, W, H, bounces, mode, rr_start, seed, pad;
+    uint64_t samples, sample_offset, scene_hash;
+};
+
+int crt_renderer_checkpoint_size(crt_renderer *r, uint64_t *bytes)
+{
+    if (!r || !bytes) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    *bytes = sizeof(CkptHeader) + (uint64_t)r->b.P * 3 * sizeof(float);
+    return CRT_OK;
+}
+
+int crt_renderer_checkpoint_save(crt_renderer *r, void *buf, uint64_t bytes)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    uint64_t need = 0;
+    crt_renderer_checkpoint_size(r, &need);
+    if (!buf || bytes < need) return fail(CRT_ERR_INVALID_ARG, "checkpoint buffer too small");
+    CkptHeader h{};
+    std::memcpy(h.magic, "CRTCKPT1", 8);
+    h.version       = 1;
+    h.W             = r->W;
+    h.H             = r->H;
+    h.bounces       = r->bounces;
+    h.mode          = r->opt.mode;
+    h.rr_start      = r->opt.rr_start;
+    h.seed          = r->opt.seed;
+    h.samples       = r->samples.load();
+    h.sample_offset = r->opt.sample_offset;
+    h.scene_hash    = r->scene->hash;
+    std::memcpy(buf, &h, sizeof h);
+    HIP_CHECK(hipMemcpy(static_cast<char *>(buf) + sizeof h, r->b.raw, (size_t)r->b.P * 12, hipMemcpyDeviceToHost));
+    return CRT_OK;
+}
+
+int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t bytes)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    if (!buf || bytes < sizeof(CkptHeader)) return fail(CRT_ERR_CHECKPOINT, "checkpoint truncated");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, "CRTCKPT1", 8) != 0 || h.version != 1) return fail(CRT_ERR_CHECKPOINT, "not a crt checkpoint");
+    if (h.W != r->W || h.H != r->H) return fail(CRT_ERR_CHECKPOINT, "checkpoint resolution differs");
+    if (h.scene_hash != r->scene->hash) return fail(CRT_ERR_CHECKPOINT, "checkpoint scene differs");
+    if (bytes < sizeof h + (uint64_t)r->b.P * 12) return fail(CRT_ERR_CHECKPOINT, "checkpoint truncated");
+    if ((st = reset_accum(r))) return st;
+    HIP_CHECK(hipMemcpy(r->b.raw, static_cast<const char *>(buf) + sizeof h, (size_t)r->b.P * 12, hipMemcpyHostToDevice));
+    r->bounces           = h.bounces;
+    r->opt.mode          = h.mode;
+    r->opt.rr_start      = h.rr_start;
+    r->opt.seed          = h.seed;
+    r->opt.sample_offset = (uint32_t)h.sample_offset;
+    r->samples.store(h.samples);
+    return CRT_OK;
+}
+
+}  // extern "C"

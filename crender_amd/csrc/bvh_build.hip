@@ -1,0 +1,1006 @@
+// bvh_build.hip -- device BVH build for the crt path tracer (gfx950).
+//
+// Replaces the BVH that Embree builds inside rtcCommitScene (CRender
+// src/objects/model.cpp:92-94).  Three stages, all on the device:
+//   1. binned SAH (16 bins per axis over centroid bounds) top-down.  Segments larger than
+//      one chunk (2048 primitives) are binned by many workgroups (LDS bins merged with
+//      global atomics) and partitioned stably chunk by chunk; smaller segments are built
+//      to completion by one workgroup each, with the primitive ids held in LDS.
+//      Node slots are derived from segment ranges, so the tree is deterministic.
+//   2. collapse of the binary tree into a 4- or 8-wide tree (greedy: open the child with
+//      the largest surface area until the node is full), level by level.
+//   3. encoding into 128-byte nodes (FP32 boxes for 4-wide, 8-bit quantised boxes for
+//      8-wide) and reordering of the triangles into leaf order.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+#include <vector>
+#include "crt_internal.h"
+
+namespace crt {
+
+namespace {
+
+constexpr int kBinWords = 3 * kNumBins * 13;
+constexpr float kInf = __builtin_huge_valf();
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ float area3(float lx, float ly, float lz, float hx, float hy, float hz)
+{
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    if (!(dx >= 0.0f) || !(dy >= 0.0f) || !(dz >= 0.0f)) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Bin index of centroid coordinate c for a segment with centroid range [lo, hi].  Both
+// the binning and the partition call this, so they agree bit for bit.
+__device__ __forceinline__ int bin_of(float c, float lo, float hi)
+{
+    const float ext = hi - lo;
+    if (!(ext > 0.0f)) return 0;
+    const float k = (float)kNumBins * (1.0f - 1e-6f) / ext;
+    int b = (int)((c - lo) * k);
+    return b < 0 ? 0 : (b >= kNumBins ? kNumBins - 1 : b);
+}
+
+__device__ __forceinline__ void bins_init(int *bins, int tid, int nthreads)
+{
+    for (int i = tid; i < kBinWords; i += nthreads) {
+        const int w = i % 13;
+        bins[i] = w == 0 ? 0 : ((w >= 1 && w <= 3) || (w >= 7 && w <= 9)) ? f2ord(kInf) : f2ord(-kInf);
+    }
+}
+
+__device__ __forceinline__ void bins_add(int *bins, float4 c, float4 lo, float4 hi, const float *cblo,
+                                         const float *cbhi)
+{
+    const float cc[3] = {c.x, c.y, c.z};
+    for (int a = 0; a < 3; ++a) {
+        const int b = bin_of(cc[a], cblo[a], cbhi[a]);
+        int *p = bins + (a * kNumBins + b) * 13;
+        atomicAdd(p + 0, 1);
+        atomicMin(p + 1, f2ord(lo.x));
+        atomicMin(p + 2, f2ord(lo.y));
+        atomicMin(p + 3, f2ord(lo.z));
+        atomicMax(p + 4, f2ord(hi.x));
+        atomicMax(p + 5, f2ord(hi.y));
+        atomicMax(p + 6, f2ord(hi.z));
+        atomicMin(p + 7, f2ord(c.x));
+        atomicMin(p + 8, f2ord(c.y));
+        atomicMin(p + 9, f2ord(c.z));
+        atomicMax(p + 10, f2ord(c.x));
+        atomicMax(p + 11, f2ord(c.y));
+        atomicMax(p + 12, f2ord(c.z));
+    }
+}
+
+struct BinVal {
+    float n;
+    float lo[3], hi[3], clo[3], chi[3];
+};
+
+__device__ __forceinline__ BinVal bv_identity()
+{
+    BinVal v;
+    v.n = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        v.lo[k] = kInf;
+        v.hi[k] = -kInf;
+        v.clo[k] = kInf;
+        v.chi[k] = -kInf;
+    }
+    return v;
+}
+
+__device__ __forceinline__ BinVal bv_join(const BinVal &a, const BinVal &b)
+{
+    BinVal r;
+    r.n = a.n + b.n;
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k]  = fminf(a.lo[k], b.lo[k]);
+        r.hi[k]  = fmaxf(a.hi[k], b.hi[k]);
+        r.clo[k] = fminf(a.clo[k], b.clo[k]);
+        r.chi[k] = fmaxf(a.chi[k], b.chi[k]);
+    }
+    return r;
+}
+
+template <int kDir>  // 0: shfl_up, 1: shfl_down
+__device__ __forceinline__ BinVal bv_shfl(const BinVal &v, int d)
+{
+    BinVal r;
+    if (kDir == 0) {
+        r.n = __shfl_up(v.n, d, 16);
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k]  = __shfl_up(v.lo[k], d, 16);
+            r.hi[k]  = __shfl_up(v.hi[k], d, 16);
+            r.clo[k] = __shfl_up(v.clo[k], d, 16);
+            r.chi[k] = __shfl_up(v.chi[k], d, 16);
+        }
+    } else {
+        r.n = __shfl_down(v.n, d, 16);
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k]  = __shfl_down(v.lo[k], d, 16);
+            r.hi[k]  = __shfl_down(v.hi[k], d, 16);
+            r.clo[k] = __shfl_down(v.clo[k], d, 16);
+            r.chi[k] = __shfl_down(v.chi[k], d, 16);
+        }
+    }
+    return r;
+}
+
+// Result of evaluating one segment's bins.
+struct SahOut {
+    float    lo[3], hi[3];        // node box
+    uint32_t n;
+    int32_t  axis, bin;           // best binned split (axis -1: none)
+    uint32_t nleft;
+    float    cost;                // AL*NL + AR*NR
+    float    lcblo[3], lcbhi[3], rcblo[3], rcbhi[3];
+};
+
+// One wave evaluates all 3 x 15 candidate planes with segmented prefix/suffix scans.
+// Must be called by all 64 lanes of one wave.
+__device__ void sah_wave(const int *bins, SahOut *out)
+{
+    const int lane = threadIdx.x & 63;
+    const int axis = lane >> 4, b = lane & 15;
+    BinVal    v = bv_identity();
+    if (lane < 48) {
+        const int *p = bins + (axis * kNumBins + b) * 13;
+        v.n = (float)p[0];
+        if (p[0] > 0) {
+            for (int k = 0; k < 3; ++k) {
+                v.lo[k]  = ord2f(p[1 + k]);
+                v.hi[k]  = ord2f(p[4 + k]);
+                v.clo[k] = ord2f(p[7 + k]);
+                v.chi[k] = ord2f(p[10 + k]);
+            }
+        }
+    }
+    BinVal pre = v, suf = v;
+    for (int d = 1; d < 16; d <<= 1) {
+        const BinVal o = bv_shfl<0>(pre, d);
+        if (b >= d) pre = bv_join(pre, o);
+    }
+    for (int d = 1; d < 16; d <<= 1) {
+        const BinVal o = bv_shfl<1>(suf, d);
+        if (b + d < 16) suf = bv_join(suf, o);
+    }
+    const BinVal rs = bv_shfl<1>(suf, 1);
+    float cost = kInf;
+    if (lane < 48 && b < 15 && pre.n > 0.0f && rs.n > 0.0f) {
+        cost = area3(pre.lo[0], pre.lo[1], pre.lo[2], pre.hi[0], pre.hi[1], pre.hi[2]) * pre.n +
+               area3(rs.lo[0], rs.lo[1], rs.lo[2], rs.hi[0], rs.hi[1], rs.hi[2]) * rs.n;
+    }
+    // argmin (cost, lane)
+    float bc = cost;
+    int   bl = lane;
+    for (int o = 32; o > 0; o >>= 1) {
+        const float oc = __shfl_xor(bc, o);
+        const int   ol = __shfl_xor(bl, o);
+        if (oc < bc || (oc == bc && ol < bl)) {
+            bc = oc;
+            bl = ol;
+        }
+    }
+    // node box and count: full prefix of axis 0 (lane 15)
+    const float tn = __shfl(pre.n, 15);
+    float       nb[6];
+    for (int k = 0; k < 3; ++k) {
+        nb[k]     = __shfl(pre.lo[k], 15);
+        nb[3 + k] = __shfl(pre.hi[k], 15);
+    }
+    if (lane == 0) {
+        for (int k = 0; k < 3; ++k) {
+            out->lo[k] = nb[k];
+            out->hi[k] = nb[3 + k];
+        }
+        out->n = (uint32_t)tn;
+        out->cost = bc;
+        if (bc == kInf) {
+            out->axis = -1;
+            out->bin  = -1;
+        } else {
+            out->axis = bl >> 4;
+            out->bin  = bl & 15;
+        }
+    }
+    if (bc != kInf && lane == bl) {
+        out->nleft = (uint32_t)pre.n;
+        for (int k = 0; k < 3; ++k) {
+            out->lcblo[k] = pre.clo[k];
+            out->lcbhi[k] = pre.chi[k];
+            out->rcblo[k] = rs.clo[k];
+            out->rcbhi[k] = rs.chi[k];
+        }
+    }
+}
+
+// Leaf / split decision (C_trav = 1, C_isect = 1).  Returns 1 for a leaf; otherwise
+// sets nleft and whether the split is by position (axis < 0).
+__device__ __forceinline__ int decide(const SahOut &s, uint32_t max_leaf, uint32_t &nleft, int &axis)
+{
+    const uint32_t n = s.n;
+    if (n <= 1) return 1;
+    const float pa = area3(s.lo[0], s.lo[1], s.lo[2], s.hi[0], s.hi[1], s.hi[2]);
+    if (s.axis >= 0 && pa > 0.0f) {
+        const float split_cost = 1.0f + s.cost / pa;
+        if (n <= max_leaf && (float)n <= split_cost) return 1;
+        nleft = s.nleft;
+        axis  = s.axis;
+        return 0;
+    }
+    if (n <= max_leaf) return 1;
+    nleft = n / 2;
+    axis  = -1;
+    return 0;
+}
+
+__device__ __forceinline__ void write_inner(Bvh2Node *nodes, uint32_t slot, const SahOut &s, uint32_t nleft)
+{
+    Bvh2Node nd;
+    for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = s.lo[k];
+        nd.hi[k] = s.hi[k];
+    }
+    nd.a        = slot + 1;
+    nd.b        = slot + 2 * nleft;
+    nodes[slot] = nd;
+}
+
+__device__ __forceinline__ void write_leaf(Bvh2Node *nodes, uint32_t slot, const SahOut &s, uint32_t first, uint32_t n)
+{
+    Bvh2Node nd;
+    for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = s.lo[k];
+        nd.hi[k] = s.hi[k];
+    }
+    nd.a        = first;
+    nd.b        = kLeafBit | n;
+    nodes[slot] = nd;
+}
+
+// ---- stage 0: primitive boxes, centroids, root bounds ---------------------------------
+__global__ __launch_bounds__(256) void k_prim_bounds(const float *__restrict__ tri, uint32_t n, float4 *plo,
+                                                     float4 *phi, float4 *cen, uint32_t *idx, int *root)
+{
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+    float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float *p = tri + 9 * (size_t)i;
+        float        l[3], h[3];
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fminf(fminf(p[k], p[3 + k]), p[6 + k]);
+            h[k] = fmaxf(fmaxf(p[k], p[3 + k]), p[6 + k]);
+        }
+        const float c[3] = {0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2])};
+        plo[i] = make_float4(l[0], l[1], l[2], 0.0f);
+        phi[i] = make_float4(h[0], h[1], h[2], 0.0f);
+        cen[i] = make_float4(c[0], c[1], c[2], 0.0f);
+        idx[i] = i;
+        for (int k = 0; k < 3; ++k) {
+            lo[k]  = fminf(lo[k], l[k]);
+            hi[k]  = fmaxf(hi[k], h[k]);
+            clo[k] = fminf(clo[k], c[k]);
+            chi[k] = fmaxf(chi[k], c[k]);
+        }
+    }
+    __shared__ float red[4][12];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        lo[k]  = wave_min(lo[k]);
+        hi[k]  = wave_max(hi[k]);
+        clo[k] = wave_min(clo[k]);
+        chi[k] = wave_max(chi[k]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            red[w][k]     = lo[k];
+            red[w][3 + k] = hi[k];
+            red[w][6 + k] = clo[k];
+            red[w][9 + k] = chi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        float     v = red[0][k];
+        for (int j = 1; j < (int)(blockDim.x >> 6); ++j) v = ((k / 3) % 2 == 0) ? fminf(v, red[j][k]) : fmaxf(v, red[j][k]);
+        if ((k / 3) % 2 == 0) atomicMin(root + k, f2ord(v));
+        else atomicMax(root + k, f2ord(v));
+    }
+}
+
+__global__ void k_root_task(const int *root, uint32_t n, BuildTask *large, BuildTask *small, uint32_t *counts)
+{
+    BuildTask t;
+    t.begin = 0;
+    t.end   = n;
+    t.slot  = 0;
+    t.pad   = 0;
+    for (int k = 0; k < 3; ++k) {
+        t.cblo[k] = ord2f(root[6 + k]);
+        t.cbhi[k] = ord2f(root[9 + k]);
+    }
+    if (n > kChunk) {
+        large[0]  = t;
+        counts[0] = 1;  // large
+        counts[1] = 0;  // small
+    } else {
+        small[0]  = t;
+        counts[0] = 0;
+        counts[1] = 1;
+    }
+}
+
+// ---- stage 1a: segments larger than one chunk --------------------------------------------
+__global__ void k_make_chunks(const BuildTask *tasks, uint32_t ntasks, uint32_t *chunk_task,
+                              uint32_t *chunk_begin, uint32_t *task_chunk0, uint32_t *nchunks)
+{
+    if (threadIdx.x != 0) return;
+    uint32_t c = 0;
+    for (uint32_t t = 0; t < ntasks; ++t) {
+        task_chunk0[t] = c;
+        for (uint32_t b = tasks[t].begin; b < tasks[t].end; b += kChunk) {
+            chunk_task[c]  = t;
+            chunk_begin[c] = b;
+            ++c;
+        }
+    }
+    task_chunk0[ntasks] = c;
+    *nchunks            = c;
+}
+
+__global__ void k_clear_bins(int *bins, uint32_t ntasks)
+{
+    const uint32_t t = blockIdx.x;
+    if (t < ntasks) bins_init(bins + (size_t)t * kBinWords, threadIdx.x, blockDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_bin_large(const BuildTask *tasks, const uint32_t *chunk_task,
+                                                   const uint32_t *chunk_begin, const uint32_t *idx,
+                                                   const float4 *plo, const float4 *phi, const float4 *cen,
+                                                   int *gbins)
+{
+    __shared__ int bins[kBinWords];
+    const uint32_t  c = blockIdx.x;
+    const BuildTask t = tasks[chunk_task[c]];
+    const uint32_t  b = chunk_begin[c];
+    const uint32_t  e = min(b + kChunk, t.end);
+    bins_init(bins, threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t id = idx[i];
+        bins_add(bins, cen[id], plo[id], phi[id], t.cblo, t.cbhi);
+    }
+    __syncthreads();
+    int *g = gbins + (size_t)chunk_task[c] * kBinWords;
+    for (int i = threadIdx.x; i < kBinWords; i += blockDim.x) {
+        const int w = i % 13;
+        const int v = bins[i];
+        if (w == 0) {
+            if (v) atomicAdd(g + i, v);
+        } else if ((w >= 1 && w <= 3) || (w >= 7 && w <= 9)) {
+            atomicMin(g + i, v);
+        } else {
+            atomicMax(g + i, v);
+        }
+    }
+}
+
+// one wave per large segment: choose the split, write the node, emit the children
+__global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, uint32_t ntasks, const int *gbins,
+                                                    SplitInfo *split, Bvh2Node *nodes, BuildTask *next_large,
+                                                    BuildTask *small, uint32_t *counts, uint32_t max_leaf)
+{
+    __shared__ SahOut s;
+    const uint32_t t = blockIdx.x;
+    if (t >= ntasks) return;
+    sah_wave(gbins + (size_t)t * kBinWords, &s);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const BuildTask tk = tasks[t];
+    uint32_t        nleft = 0;
+    int             axis  = -1;
+    SplitInfo       si;
+    if (decide(s, max_leaf, nleft, axis)) {
+        // cannot happen for n > kChunk >= max_leaf; keep a valid tree regardless
+        nleft = s.n / 2;
+        axis  = -1;
+    }
+    si.axis  = axis;
+    si.bin   = axis >= 0 ? s.bin : -1;
+    si.nleft = nleft;
+    si.leaf  = 0;
+    split[t] = si;
+    write_inner(nodes, tk.slot, s, nleft);
+    BuildTask l, r;
+    l.begin = tk.begin;
+    l.end   = tk.begin + nleft;
+    l.slot  = tk.slot + 1;
+    l.pad   = 0;
+    r.begin = tk.begin + nleft;
+    r.end   = tk.end;
+    r.slot  = tk.slot + 2 * nleft;
+    r.pad   = 0;
+    for (int k = 0; k < 3; ++k) {
+        l.cblo[k] = axis >= 0 ? s.lcblo[k] : tk.cblo[k];
+        l.cbhi[k] = axis >= 0 ? s.lcbhi[k] : tk.cbhi[k];
+        r.cblo[k] = axis >= 0 ? s.rcblo[k] : tk.cblo[k];
+        r.cbhi[k] = axis >= 0 ? s.rcbhi[k] : tk.cbhi[k];
+    }
+    const BuildTask ch[2] = {l, r};
+    for (int j = 0; j < 2; ++j) {
+        if (ch[j].end - ch[j].begin > kChunk) next_large[atomicAdd(counts + 0, 1u)] = ch[j];
+        else small[atomicAdd(counts + 1, 1u)] = ch[j];
+    }
+}
+
+__device__ __forceinline__ bool goes_left(const SplitInfo &si, const BuildTask &t, uint32_t pos, uint32_t id,
+                                          const float4 *cen)
+{
+    if (si.axis < 0) return pos < t.begin + si.nleft;
+    const float4 c  = cen[id];
+    const float  cc = si.axis == 0 ? c.x : (si.axis == 1 ? c.y : c.z);
+    return bin_of(cc, t.cblo[si.axis], t.cbhi[si.axis]) <= si.bin;
+}
+
+__global__ __launch_bounds__(256) void k_part_count(const BuildTask *tasks, const SplitInfo *split,
+                                                    const uint32_t *chunk_task, const uint32_t *chunk_begin,
+                                                    const uint32_t *idx, const float4 *cen, uint32_t *chunk_left)
+{
+    __shared__ uint32_t cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const uint32_t  c  = blockIdx.x;
+    const uint32_t  ti = chunk_task[c];
+    const BuildTask t  = tasks[ti];
+    const SplitInfo si = split[ti];
+    const uint32_t  b  = chunk_begin[c];
+    const uint32_t  e  = min(b + kChunk, t.end);
+    uint32_t        local = 0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) local += goes_left(si, t, i, idx[i], cen) ? 1u : 0u;
+    atomicAdd(&cnt, local);
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_left[c] = cnt;
+}
+
+__global__ void k_part_offsets(uint32_t ntasks, const uint32_t *task_chunk0, const uint32_t *chunk_left,
+                               uint32_t *chunk_loff)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    uint32_t acc = 0;
+    for (uint32_t c = task_chunk0[t]; c < task_chunk0[t + 1]; ++c) {
+        chunk_loff[c] = acc;
+        acc += chunk_left[c];
+    }
+}
+
+// Stable partition of one chunk: lefts in order, then rights in order.
+__global__ __launch_bounds__(256) void k_part_scatter(const BuildTask *tasks, const SplitInfo *split,
+                                                      const uint32_t *chunk_task, const uint32_t *chunk_begin,
+                                                      const uint32_t *chunk_loff, const uint32_t *idx,
+                                                      const float4 *cen, uint32_t *tmp)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t  c  = blockIdx.x;
+    const uint32_t  ti = chunk_task[c];
+    const BuildTask t  = tasks[ti];
+    const SplitInfo si = split[ti];
+    const uint32_t  b  = chunk_begin[c];
+    const uint32_t  e  = min(b + kChunk, t.end);
+    const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t        lbefore = chunk_loff[c];            // lefts of this task before this chunk
+    uint32_t        rbefore = (b - t.begin) - lbefore;  // rights before this chunk
+    for (uint32_t base = b; base < e; base += blockDim.x) {
+        const uint32_t i  = base + threadIdx.x;
+        const bool     in = i < e;
+        const uint32_t id = in ? idx[i] : 0u;
+        const bool     f  = in && goes_left(si, t, i, id, cen);
+        const uint64_t bal = __ballot(f);
+        const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(bal);
+        __syncthreads();
+        uint32_t off = 0, tot = 0;
+        for (int j = 0; j < 4; ++j) {
+            off += j < w ? wsum[j] : 0u;
+            tot += wsum[j];
+        }
+        if (in) {
+            const uint32_t lrank = off + rank;
+            const uint32_t pos   = f ? t.begin + lbefore + lrank
+                                     : t.begin + si.nleft + rbefore + (threadIdx.x - lrank);
+            tmp[pos] = id;
+        }
+        const uint32_t m = min((uint32_t)blockDim.x, e - base);
+        lbefore += tot;
+        rbefore += m - tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_chunks(const BuildTask *tasks, const uint32_t *chunk_task,
+                                                     const uint32_t *chunk_begin, const uint32_t *tmp, uint32_t *idx)
+{
+    const uint32_t  c = blockIdx.x;
+    const BuildTask t = tasks[chunk_task[c]];
+    const uint32_t  b = chunk_begin[c];
+    const uint32_t  e = min(b + kChunk, t.end);
+    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) idx[i] = tmp[i];
+}
+
+// ---- stage 1b: one workgroup builds a whole small segment -------------------------------
+struct LocalTask {
+    uint32_t b, e, slot, pad;
+    float    cblo[3], cbhi[3];
+};
+
+__global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t ntasks, uint32_t *idx,
+                                               const float4 *plo, const float4 *phi, const float4 *cen,
+                                               Bvh2Node *nodes, uint32_t max_leaf)
+{
+    __shared__ uint32_t  ids[kChunk];
+    __shared__ uint32_t  tmp[kChunk];
+    __shared__ int       bins[kBinWords];
+    __shared__ SahOut    s;
+    __shared__ LocalTask stack[48];
+    __shared__ int       sp;
+    __shared__ LocalTask cur;
+    __shared__ uint32_t  wsum[4];
+    __shared__ int       dec_leaf, dec_axis;
+    __shared__ uint32_t  dec_nleft;
+
+    const uint32_t ti = blockIdx.x;
+    if (ti >= ntasks) return;
+    const BuildTask task = tasks[ti];
+    const uint32_t  gb   = task.begin;
+    const uint32_t  n    = task.end - task.begin;
+    const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) ids[i] = idx[gb + i];
+    if (threadIdx.x == 0) {
+        LocalTask r;
+        r.b    = 0;
+        r.e    = n;
+        r.slot = task.slot;
+        r.pad  = 0;
+        for (int k = 0; k < 3; ++k) {
+            r.cblo[k] = task.cblo[k];
+            r.cbhi[k] = task.cbhi[k];
+        }
+        stack[0] = r;
+        sp       = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0 && sp > 0) cur = stack[--sp];
+        __syncthreads();
+        if (cur.e == 0xFFFFFFFFu) break;
+        bins_init(bins, threadIdx.x, blockDim.x);
+        __syncthreads();
+        for (uint32_t i = cur.b + threadIdx.x; i < cur.e; i += blockDim.x) {
+            const uint32_t id = ids[i];
+            bins_add(bins, cen[id], plo[id], phi[id], cur.cblo, cur.cbhi);
+        }
+        __syncthreads();
+        if (w == 0) sah_wave(bins, &s);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t nl = 0;
+            int      ax = -1;
+            dec_leaf    = decide(s, max_leaf, nl, ax);
+            dec_nleft   = nl;
+            dec_axis    = ax;
+            if (dec_leaf) write_leaf(nodes, cur.slot, s, gb + cur.b, cur.e - cur.b);
+            else write_inner(nodes, cur.slot, s, nl);
+        }
+        __syncthreads();
+        if (!dec_leaf && dec_axis >= 0) {
+            uint32_t lcount = 0, rcount = 0;
+            const int ax = dec_axis;
+            for (uint32_t base = cur.b; base < cur.e; base += blockDim.x) {
+                const uint32_t i  = base + threadIdx.x;
+                const bool     in = i < cur.e;
+                bool           f  = false;
+                uint32_t       id = 0;
+                if (in) {
+                    id = ids[i];
+                    const float4 c  = cen[id];
+                    const float  cc = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+                    f = bin_of(cc, cur.cblo[ax], cur.cbhi[ax]) <= s.bin;
+                }
+                const uint64_t bal  = __ballot(f);
+                const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
+                if (lane == 0) wsum[w] = __popcll(bal);
+                __syncthreads();
+                uint32_t off = 0, tot = 0;
+                for (int j = 0; j < 4; ++j) {
+                    off += j < w ? wsum[j] : 0u;
+                    tot += wsum[j];
+                }
+                if (in) {
+                    const uint32_t lr = off + rank;
+                    tmp[f ? cur.b + lcount + lr : cur.b + dec_nleft + rcount + (threadIdx.x - lr)] = id;
+                }
+                const uint32_t m = min((uint32_t)blockDim.x, cur.e - base);
+                lcount += tot;
+                rcount += m - tot;
+                __syncthreads();
+            }
+            for (uint32_t i = cur.b + threadIdx.x; i < cur.e; i += blockDim.x) ids[i] = tmp[i];
+        }
+        if (threadIdx.x == 0) {
+            if (!dec_leaf) {
+                LocalTask l, r;
+                l.b    = cur.b;
+                l.e    = cur.b + dec_nleft;
+                l.slot = cur.slot + 1;
+                l.pad  = 0;
+                r.b    = cur.b + dec_nleft;
+                r.e    = cur.e;
+                r.slot = cur.slot + 2 * dec_nleft;
+                r.pad  = 0;
+                for (int k = 0; k < 3; ++k) {
+                    l.cblo[k] = dec_axis >= 0 ? s.lcblo[k] : cur.cblo[k];
+                    l.cbhi[k] = dec_axis >= 0 ? s.lcbhi[k] : cur.cbhi[k];
+                    r.cblo[k] = dec_axis >= 0 ? s.rcblo[k] : cur.cblo[k];
+                    r.cbhi[k] = dec_axis >= 0 ? s.rcbhi[k] : cur.cbhi[k];
+                }
+                // larger child first so the smaller is processed next: stack depth <= log2(n)+1
+                if (l.e - l.b >= r.e - r.b) {
+                    stack[sp++] = l;
+                    stack[sp++] = r;
+                } else {
+                    stack[sp++] = r;
+                    stack[sp++] = l;
+                }
+            }
+            if (sp == 0) cur.e = 0xFFFFFFFFu;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) idx[gb + i] = ids[i];
+}
+
+// ---- stage 2/3: collapse + encode ------------------------------------------------------
+struct Frontier {
+    uint32_t slot;   // BVH2 slot
+    uint32_t wide;   // wide node index
+};
+
+__device__ __forceinline__ bool b2_leaf(const Bvh2Node &n) { return (n.b & kLeafBit) != 0u; }
+__device__ __forceinline__ float b2_area(const Bvh2Node &n)
+{
+    return area3(n.lo[0], n.lo[1], n.lo[2], n.hi[0], n.hi[1], n.hi[2]);
+}
+
+// biased exponent e such that 255 * 2^(e-127) >= ext
+__device__ __forceinline__ uint32_t quant_exp(double ext)
+{
+    if (!(ext > 0.0)) return 1u;
+    int e;
+    frexp(ext / 255.0, &e);   // ext/255 = m * 2^e, m in [0.5, 1) -> 2^e >= ext/255
+    int be = e + 127;
+    if (be < 1) be = 1;
+    if (be > 254) be = 254;
+    return (uint32_t)be;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Frontier *in, uint32_t nin,
+                                                  Frontier *out, uint32_t *nout, uint32_t *wide_count,
+                                                  void *wide_nodes)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    const Frontier f = in[i];
+    const Bvh2Node root = b2[f.slot];
+    uint32_t       ch[W];
+    int            nc = 0;
+    if (b2_leaf(root)) {
+        ch[nc++] = f.slot;
+    } else {
+        ch[nc++] = root.a;
+        ch[nc++] = root.b;
+        while (nc < W) {
+            int   best = -1;
+            float ba   = -1.0f;
+            for (int j = 0; j < nc; ++j) {
+                const Bvh2Node n = b2[ch[j]];
+                if (!b2_leaf(n)) {
+                    const float a = b2_area(n);
+                    if (a > ba) {
+                        ba   = a;
+                        best = j;
+                    }
+                }
+            }
+            if (best < 0) break;
+            const Bvh2Node n = b2[ch[best]];
+            ch[best]  = n.a;
+            ch[nc++]  = n.b;
+        }
+    }
+    uint32_t ref[W];
+    float    lo[W][3], hi[W][3];
+    for (int j = 0; j < nc; ++j) {
+        const Bvh2Node n = b2[ch[j]];
+        for (int k = 0; k < 3; ++k) {
+            lo[j][k] = n.lo[k];
+            hi[j][k] = n.hi[k];
+        }
+        if (b2_leaf(n)) {
+            ref[j] = ref_leaf(n.a, n.b & ~kLeafBit);
+        } else {
+            const uint32_t wi = atomicAdd(wide_count, 1u);
+            ref[j] = wi;
+            out[atomicAdd(nout, 1u)] = Frontier{ch[j], wi};
+        }
+    }
+    if (W == 4) {
+        Node4 nd;
+        for (int j = 0; j < 4; ++j) {
+            const bool v = j < nc;
+            nd.lox[j]   = v ? lo[j][0] : 0.0f;
+            nd.loy[j]   = v ? lo[j][1] : 0.0f;
+            nd.loz[j]   = v ? lo[j][2] : 0.0f;
+            nd.hix[j]   = v ? hi[j][0] : 0.0f;
+            nd.hiy[j]   = v ? hi[j][1] : 0.0f;
+            nd.hiz[j]   = v ? hi[j][2] : 0.0f;
+            nd.child[j] = v ? ref[j] : kInvalidRef;
+        }
+        nd.nchild = (uint32_t)nc;
+        nd.pad0 = nd.pad1 = nd.pad2 = 0;
+        static_cast<Node4 *>(wide_nodes)[f.wide] = nd;
+    } else {
+        Node8Q nd;
+        float  olo[3] = {kInf, kInf, kInf}, ohi[3] = {-kInf, -kInf, -kInf};
+        for (int j = 0; j < nc; ++j)
+            for (int k = 0; k < 3; ++k) {
+                olo[k] = fminf(olo[k], lo[j][k]);
+                ohi[k] = fmaxf(ohi[k], hi[j][k]);
+            }
+        uint32_t e[3];
+        double   sc[3];
+        for (int k = 0; k < 3; ++k) {
+            e[k]  = quant_exp((double)ohi[k] - (double)olo[k]);
+            sc[k] = ldexp(1.0, (int)e[k] - 127);
+        }
+        nd.ox   = olo[0];
+        nd.oy   = olo[1];
+        nd.oz   = olo[2];
+        nd.meta = e[0] | (e[1] << 8) | (e[2] << 16) | ((uint32_t)nc << 24);
+        uint8_t *qlo[3] = {nd.qlox, nd.qloy, nd.qloz};
+        uint8_t *qhi[3] = {nd.qhix, nd.qhiy, nd.qhiz};
+        for (int j = 0; j < 8; ++j) {
+            nd.child[j] = j < nc ? ref[j] : kInvalidRef;
+            for (int k = 0; k < 3; ++k) {
+                if (j < nc) {
+                    double ql = floor(((double)lo[j][k] - (double)olo[k]) / sc[k]);
+                    double qh = ceil(((double)hi[j][k] - (double)olo[k]) / sc[k]);
+                    ql = ql < 0.0 ? 0.0 : (ql > 255.0 ? 255.0 : ql);
+                    qh = qh < 0.0 ? 0.0 : (qh > 255.0 ? 255.0 : qh);
+                    qlo[k][j] = (uint8_t)ql;
+                    qhi[k][j] = (uint8_t)qh;
+                } else {
+                    qlo[k][j] = 0;
+                    qhi[k][j] = 0;
+                }
+            }
+        }
+        for (int j = 0; j < 8; ++j) nd.pad[j] = 0;
+        static_cast<Node8Q *>(wide_nodes)[f.wide] = nd;
+    }
+}
+
+__global__ void k_reorder(const uint32_t *idx, uint32_t n, const float *tri, float4 *out)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t id = idx[p];
+    const float   *q  = tri + 9 * (size_t)id;
+    out[3 * (size_t)p + 0] = make_float4(q[0], q[1], q[2], __builtin_bit_cast(float, id));
+    out[3 * (size_t)p + 1] = make_float4(q[3], q[4], q[5], 0.0f);
+    out[3 * (size_t)p + 2] = make_float4(q[6], q[7], q[8], 0.0f);
+}
+
+__global__ void k_mark_unused(Bvh2Node *nodes, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) nodes[i].b = 0xFFFFFFFEu;
+}
+
+// SAH cost of the binary tree: sum(inner area) + sum(leaf area * count), / root area
+__global__ void k_sah_cost(const Bvh2Node *nodes, uint32_t nslots, double *acc, uint32_t *count)
+{
+    double   local = 0.0;
+    uint32_t used  = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) {
+        const Bvh2Node n = nodes[i];
+        if (n.b == 0xFFFFFFFEu) continue;
+        ++used;
+        const double a = (double)b2_area(n);
+        local += b2_leaf(n) ? a * (double)(n.b & ~kLeafBit) : a;
+    }
+    atomicAdd(acc, local);
+    atomicAdd(count, used);
+}
+
+#define HIP_TRY(x)                          \
+    do {                                    \
+        hipError_t e_ = (x);                \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+}  // namespace
+
+hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf, hipStream_t s,
+                     BuildResult *out)
+{
+    *out = BuildResult{};
+    if (ntri == 0) return hipSuccess;
+    if (width != 4 && width != 8) return hipErrorInvalidValue;
+    if (ntri >= (1u << 27)) return hipErrorInvalidValue;
+    if (max_leaf < 1) max_leaf = 1;
+    if (max_leaf > 16) max_leaf = 16;
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, s));
+
+    const uint32_t nslots = 2 * ntri;
+    float4 *plo, *phi, *cen;
+    uint32_t *idx, *tmp;
+    int *root;
+    Bvh2Node *nodes;
+    BuildTask *large_a, *large_b, *small;
+    uint32_t *counts;   // [0] large, [1] small, [2] nchunks, [3] frontier out, [4] wide count
+    const uint32_t max_large = ntri / kChunk + 2;
+    const uint32_t max_chunks = ntri / kChunk + 2 * max_large + 2;
+    uint32_t *chunk_task, *chunk_begin, *task_chunk0, *chunk_left, *chunk_loff;
+    SplitInfo *split;
+    int *gbins;
+    HIP_TRY(hipMallocAsync((void **)&plo, sizeof(float4) * ntri, s));
+    HIP_TRY(hipMallocAsync((void **)&phi, sizeof(float4) * ntri, s));
+    HIP_TRY(hipMallocAsync((void **)&cen, sizeof(float4) * ntri, s));
+    HIP_TRY(hipMallocAsync((void **)&idx, sizeof(uint32_t) * ntri, s));
+    HIP_TRY(hipMallocAsync((void **)&tmp, sizeof(uint32_t) * ntri, s));
+    HIP_TRY(hipMallocAsync((void **)&root, sizeof(int) * 12, s));
+    HIP_TRY(hipMallocAsync((void **)&nodes, sizeof(Bvh2Node) * nslots, s));
+    HIP_TRY(hipMallocAsync((void **)&large_a, sizeof(BuildTask) * max_large, s));
+    HIP_TRY(hipMallocAsync((void **)&large_b, sizeof(BuildTask) * max_large, s));
+    HIP_TRY(hipMallocAsync((void **)&small, sizeof(BuildTask) * (ntri + 1), s));
+    HIP_TRY(hipMallocAsync((void **)&counts, sizeof(uint32_t) * 8, s));
+    HIP_TRY(hipMallocAsync((void **)&chunk_task, sizeof(uint32_t) * max_chunks, s));
+    HIP_TRY(hipMallocAsync((void **)&chunk_begin, sizeof(uint32_t) * max_chunks, s));
+    HIP_TRY(hipMallocAsync((void **)&task_chunk0, sizeof(uint32_t) * (max_large + 1), s));
+    HIP_TRY(hipMallocAsync((void **)&chunk_left, sizeof(uint32_t) * max_chunks, s));
+    HIP_TRY(hipMallocAsync((void **)&chunk_loff, sizeof(uint32_t) * max_chunks, s));
+    HIP_TRY(hipMallocAsync((void **)&split, sizeof(SplitInfo) * max_large, s));
+    HIP_TRY(hipMallocAsync((void **)&gbins, sizeof(int) * kBinWords * max_large, s));
+
+    const int init_root[12] = {f2ord(kInf), f2ord(kInf), f2ord(kInf), f2ord(-kInf), f2ord(-kInf), f2ord(-kInf),
+                               f2ord(kInf), f2ord(kInf), f2ord(kInf), f2ord(-kInf), f2ord(-kInf), f2ord(-kInf)};
+    HIP_TRY(hipMemcpyAsync(root, init_root, sizeof init_root, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * 8, s));
+    k_mark_unused<<<(nslots + 255) / 256, 256, 0, s>>>(nodes, nslots);
+    const uint32_t nb = std::min<uint32_t>((ntri + 255) / 256, 2048u);
+    k_prim_bounds<<<nb, 256, 0, s>>>(d_tri, ntri, plo, phi, cen, idx, root);
+    k_root_task<<<1, 1, 0, s>>>(root, ntri, large_a, small, counts);
+    HIP_TRY(hipGetLastError());
+
+    uint32_t hc[8];
+    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint32_t nlarge = hc[0];
+    while (nlarge > 0) {
+        k_make_chunks<<<1, 64, 0, s>>>(large_a, nlarge, chunk_task, chunk_begin, task_chunk0, counts + 2);
+        k_clear_bins<<<nlarge, 256, 0, s>>>(gbins, nlarge);
+        HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const uint32_t nchunks = hc[2];
+        k_bin_large<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, idx, plo, phi, cen, gbins);
+        HIP_TRY(hipMemsetAsync(counts + 0, 0, sizeof(uint32_t), s));
+        k_split_large<<<nlarge, 64, 0, s>>>(large_a, nlarge, gbins, split, nodes, large_b, small, counts, max_leaf);
+        k_part_count<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, idx, cen, chunk_left);
+        k_part_offsets<<<(nlarge + 63) / 64, 64, 0, s>>>(nlarge, task_chunk0, chunk_left, chunk_loff);
+        k_part_scatter<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, chunk_loff, idx, cen, tmp);
+        k_copy_chunks<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, tmp, idx);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nlarge = hc[0];
+        std::swap(large_a, large_b);
+    }
+    const uint32_t nsmall = hc[1];
+    if (nsmall) k_small<<<nsmall, 256, 0, s>>>(small, nsmall, idx, plo, phi, cen, nodes, max_leaf);
+    HIP_TRY(hipGetLastError());
+
+    // SAH cost + used slots
+    double   *sah_acc;
+    uint32_t *used;
+    HIP_TRY(hipMallocAsync((void **)&sah_acc, sizeof(double), s));
+    HIP_TRY(hipMallocAsync((void **)&used, sizeof(uint32_t), s));
+    HIP_TRY(hipMemsetAsync(sah_acc, 0, sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(used, 0, sizeof(uint32_t), s));
+    k_sah_cost<<<std::min<uint32_t>((nslots + 255) / 256, 1024u), 256, 0, s>>>(nodes, nslots, sah_acc, used);
+
+    // collapse
+    const uint32_t max_wide = ntri + 1;
+    const size_t   node_bytes = 128;
+    void          *wide_tmp;
+    Frontier      *fa, *fb;
+    HIP_TRY(hipMallocAsync(&wide_tmp, node_bytes * max_wide, s));
+    HIP_TRY(hipMallocAsync((void **)&fa, sizeof(Frontier) * max_wide, s));
+    HIP_TRY(hipMallocAsync((void **)&fb, sizeof(Frontier) * max_wide, s));
+    const Frontier f0{0u, 0u};
+    HIP_TRY(hipMemcpyAsync(fa, &f0, sizeof f0, hipMemcpyHostToDevice, s));
+    const uint32_t one = 1;
+    HIP_TRY(hipMemcpyAsync(counts + 4, &one, sizeof one, hipMemcpyHostToDevice, s));
+    uint32_t nfront = 1, depth = 0;
+    while (nfront > 0) {
+        HIP_TRY(hipMemsetAsync(counts + 3, 0, sizeof(uint32_t), s));
+        if (width == 4)
+            k_collapse<4><<<(nfront + 255) / 256, 256, 0, s>>>(nodes, fa, nfront, fb, counts + 3, counts + 4, wide_tmp);
+        else
+            k_collapse<8><<<(nfront + 255) / 256, 256, 0, s>>>(nodes, fa, nfront, fb, counts + 3, counts + 4, wide_tmp);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nfront = hc[3];
+        ++depth;
+        std::swap(fa, fb);
+    }
+    const uint32_t nwide = hc[4];
+    void *wide;
+    HIP_TRY(hipMalloc(&wide, node_bytes * nwide));
+    HIP_TRY(hipMemcpyAsync(wide, wide_tmp, node_bytes * nwide, hipMemcpyDeviceToDevice, s));
+    float4 *tri_out;
+    HIP_TRY(hipMalloc((void **)&tri_out, sizeof(float4) * 3 * (size_t)ntri));
+    k_reorder<<<(ntri + 255) / 256, 256, 0, s>>>(idx, ntri, d_tri, tri_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev1, s));
+
+    double   sah_h = 0.0;
+    uint32_t used_h = 0;
+    HIP_TRY(hipMemcpyAsync(&sah_h, sah_acc, sizeof sah_h, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&used_h, used, sizeof used_h, hipMemcpyDeviceToHost, s));
+    Bvh2Node root_node;
+    HIP_TRY(hipMemcpyAsync(&root_node, nodes, sizeof root_node, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    const float rdx = root_node.hi[0] - root_node.lo[0], rdy = root_node.hi[1] - root_node.lo[1],
+                rdz = root_node.hi[2] - root_node.lo[2];
+    const double ra = 2.0 * ((double)rdx * rdy + (double)rdy * rdz + (double)rdz * rdx);
+
+    void *frees[] = {plo, phi, cen, idx, tmp, root, nodes, large_a, large_b, small, counts, chunk_task, chunk_begin,
+                     task_chunk0, chunk_left, chunk_loff, split, gbins, sah_acc, used, wide_tmp, fa, fb};
+    for (void *p : frees) HIP_TRY(hipFreeAsync(p, s));
+    HIP_TRY(hipStreamSynchronize(s));
+
+    out->nodes      = wide;
+    out->tri        = tri_out;
+    out->node_count = nwide;
+    out->bvh2_nodes = used_h;
+    out->depth      = depth;
+    out->root       = 0;
+    out->sah        = ra > 0.0 ? sah_h / ra : 0.0;
+    out->ms         = ms;
+    return hipSuccess;
+}
+
+}  // namespace crt

@@ -1,0 +1,201 @@
+// crt_internal.h -- device data layout and kernel entry points shared by the crt sources.
+//
+// HBM layout of a committed scene:
+//   tri      float4[3*ntri]   triangles in BVH leaf order: {v0,primID bits}{v1,0}{v2,0}
+//   nodes    128-byte wide nodes (Node4: FP32 boxes; Node8Q: 8-bit quantised boxes)
+//   mat_idx  u32[ntri]        material per primID (original order)
+//   uv       float2[3*ntri]   per-corner texture coordinates (original order)
+// Rays and hits are SoA float/u32 arrays, one element per ray (coalesced per wave).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "crt_math.h"
+
+namespace crt {
+
+constexpr uint32_t kLeafBit    = 0x80000000u;
+constexpr uint32_t kInvalidRef = 0xFFFFFFFFu;
+constexpr int      kNumBins    = 16;
+constexpr uint32_t kChunk      = 2048;   // prims per workgroup in the SAH builder
+
+// child reference: inner node index, or leaf = kLeafBit | (count-1) << 27 | first
+CRT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
+CRT_HD uint32_t ref_leaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 27) | first; }
+CRT_HD uint32_t leaf_first(uint32_t r) { return r & 0x07FFFFFFu; }
+CRT_HD uint32_t leaf_count(uint32_t r) { return ((r >> 27) & 0xFu) + 1u; }
+
+// Binary node written by the builder; slot 2*begin for a segment starting at begin.
+struct Bvh2Node {
+    float    lo[3];
+    uint32_t a;   // inner: left child slot; leaf: first primitive position
+    float    hi[3];
+    uint32_t b;   // inner: right child slot; leaf: kLeafBit | count
+};
+
+// 4-wide node, FP32 child boxes, one 128-byte line.
+struct alignas(128) Node4 {
+    float    lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t nchild, pad0, pad1, pad2;
+};
+static_assert(sizeof(Node4) == 128, "Node4 is one cache line");
+
+// 8-wide node, child boxes quantised to 8 bits per plane relative to the node origin:
+// lo = origin + q * 2^(e-127), one 128-byte line.
+struct alignas(128) Node8Q {
+    float    ox, oy, oz;
+    uint32_t meta;      // ex | ey << 8 | ez << 16 | nchild << 24 (biased exponents)
+    uint32_t child[8];
+    uint8_t  qlox[8], qhix[8], qloy[8], qhiy[8], qloz[8], qhiz[8];
+    uint32_t pad[8];
+};
+static_assert(sizeof(Node8Q) == 128, "Node8Q is one cache line");
+
+// A segment [begin, end) of the primitive order, its BVH2 slot and centroid bounds.
+// A node in slot p covering n primitives owns slots [p, p + 2n - 1): the left child is
+// slot p + 1, the right child slot p + 2 * nleft, so the tree layout is deterministic.
+struct BuildTask {
+    uint32_t begin, end, slot, pad;
+    float    cblo[3], cbhi[3];
+};
+
+struct BinSet {   // per axis per bin: count, AABB (ordered ints), centroid bounds (ordered ints)
+    int v[3][kNumBins][13];
+};
+
+struct SplitInfo {
+    int32_t  axis;     // -1: split by position (all centroids coincide)
+    int32_t  bin;
+    uint32_t nleft;
+    uint32_t leaf;     // 1: make a leaf
+};
+
+struct MaterialDev {
+    int32_t type;
+    float   ior, roughness, reflectiveness, emission;
+    float   colour[4];
+    int32_t tex;       // -1 or index into the texture table
+};
+
+struct TextureDev {
+    uint64_t offset;   // float4 offset into the texel pool
+    uint32_t w, h;
+};
+
+struct SunDev {
+    float size, intensity;
+    f3    dir, colour;
+};
+
+// Everything the kernels read about a committed scene.
+struct SceneDev {
+    const float4      *tri;
+    const void        *nodes;
+    uint32_t           width;      // 4 or 8
+    uint32_t           ntri;
+    uint32_t           root;       // root child ref
+    const uint32_t    *mat_idx;
+    const float2      *uv;         // may be null
+    const MaterialDev *mats;
+    uint32_t           nmat;
+    const float4      *texels;
+    const TextureDev  *texs;
+    uint32_t           ntex;
+    int32_t            sky_tex;    // -1: black sky
+    float              sky_rot[2];
+    int32_t            sun_enabled;
+    SunDev             sun;
+    int32_t            any_alpha;  // some material can be alpha (colour.w == 0 or textured)
+    const uint32_t    *model_first; // first primID of each model (for geomID)
+    uint32_t           nmodels;
+};
+
+// SoA ray buffer
+struct RaySoA {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tn, *tf;
+};
+struct HitSoA {
+    float    *t, *u, *v;
+    uint32_t *tri;   // leaf-order triangle index or kInvalidRef
+};
+
+// ---- builder ------------------------------------------------------------------------
+struct BuildResult {
+    void    *nodes;         // device
+    float4  *tri;           // device, leaf order
+    uint32_t node_count;
+    uint32_t bvh2_nodes;
+    uint32_t depth;
+    uint32_t root;
+    double   sah;
+    float    ms;
+};
+// Builds the BVH of ntri triangles (9 floats each, device memory) on stream s.
+hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf,
+                     hipStream_t s, BuildResult *out);
+
+// ---- traversal ----------------------------------------------------------------------
+struct TraceQueue {
+    RaySoA          rays;
+    HitSoA          hits;
+    const uint32_t *count;     // device: number of rays
+    uint32_t       *counters;  // device: work counters (kXcds entries), zeroed by the caller's kernel
+    uint32_t       *error;     // device: set to 1 if a traversal stack would overflow
+};
+hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t max_rays,
+                                uint32_t *spill, uint32_t grid, hipStream_t s);
+hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint32_t max_rays, uint8_t *occ,
+                            uint32_t *spill, uint32_t grid, hipStream_t s);
+uint32_t trace_grid(int device, uint32_t waves_per_cu);
+uint64_t trace_spill_bytes(uint32_t grid);
+constexpr uint32_t kTraceBlock = 256;
+constexpr uint32_t kXcds       = 8;
+constexpr uint32_t kStackLds   = 16;    // LDS stack entries per lane
+constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multiple of kStackLds)
+
+// ---- integrator ---------------------------------------------------------------------
+enum : uint32_t { kModeCompat = 0, kModePbr = 1 };
+
+struct PathBuffers {
+    uint32_t  W, H, P;          // image size, path slots (= W*H)
+    RaySoA    q[2];             // extension queues (ping-pong)
+    uint32_t *qpath[2];         // path slot of each queued ray
+    HitSoA    hits;
+    RaySoA    sq;               // shadow queue
+    uint32_t *spath;
+    float4   *scontrib;         // radiance added if the shadow ray escapes
+    uint8_t  *occ;
+    HitSoA    shits;            // closest hits of shadow rays (alpha pass-through)
+    float4   *thr, *rad;
+    float    *raw;              // W*H*3 running sum, flipped as renderer.cpp:358-365
+    float4   *aov_alb, *aov_nrm, *aov_dep;
+    uint32_t *cnt;              // see Cnt below
+    unsigned long long *stats;  // [0] extension rays, [1] shadow rays, [2] reference segments
+};
+// layout of PathBuffers::cnt
+enum : uint32_t {
+    kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4,
+    kCntWorkExt = 8, kCntWorkShadow = 16, kCntWords = 32
+};
+
+struct RenderParams {
+    camera_dev cam;
+    float      aspect;
+    uint32_t   max_bounces, rr_start, mode, seed, sample;
+};
+
+hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s);
+hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s);
+hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
+                        uint32_t cur, hipStream_t s);
+hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s);
+hipError_t launch_shadow_resolve(const PathBuffers &b, hipStream_t s);
+hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hipStream_t s);
+hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s);
+hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s);
+hipError_t launch_fill(float4 *p, uint64_t n, float v, hipStream_t s);
+hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
+                              float *ngx, float *ngy, float *ngz, hipStream_t s);
+
+}  // namespace crt

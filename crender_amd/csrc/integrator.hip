@@ -1,0 +1,619 @@
+// integrator.hip -- wavefront path-tracing kernels of the crt path tracer (gfx950).
+//
+// The reference runs CRender src/render/renderer.cpp:258-384 (_sample_pixel) per pixel on
+// CPU threads.  Here one sample pass is a sequence of kernels over SoA queues in HBM:
+//   generate  camera rays for all W*H pixels (renderer.cpp:260-263, camera.cpp:14-39)
+//   per bounce i < max_bounces:
+//     extend  closest hit of every queued ray (traverse.hip)
+//     shade   miss -> sky; hit -> process_hit (renderer.cpp:21-102), emission, AOVs on
+//             bounce 0, sun NEE ray (renderer.cpp:315-354), Russian roulette, and the
+//             next extension ray, appended with one atomic per wave
+//     shadow  any hit of the sun rays; unoccluded rays add their contribution
+//   accumulate  raw[flip(p)] += radiance (renderer.cpp:358-365)
+// Each path owns one slot (its pixel), so no two threads ever add into the same pixel and
+// the image is deterministic whatever order the queues are processed in.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include "crt_internal.h"
+
+namespace crt {
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+
+__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
+
+// Append to a queue with one atomic per wave: returns this lane's slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
+{
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0;
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t       base   = 0;
+    if (lane() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane()) - 1ull));
+}
+
+__device__ __forceinline__ void wave_add64(unsigned long long *p, uint32_t v)
+{
+    // wave reduction then one atomic
+    uint32_t s = v;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane() == 0 && s) atomicAdd(p, (unsigned long long)s);
+}
+
+// image.h:104-109 nearest texel with wrap-around
+__device__ __forceinline__ float4 tex_uv(const SceneDev &sc, int32_t t, float u, float v)
+{
+    const TextureDev td = sc.texs[t];
+    const int64_t    w = td.w, h = td.h;
+    int64_t          x = (int64_t)(u * (float)td.w);
+    int64_t          y = (int64_t)(v * (float)td.h);
+    x = ((x % w) + w) % w;
+    y = ((y % h) + h) % h;
+    return sc.texels[td.offset + (uint64_t)(x + y * w)];
+}
+
+// renderer.cpp:277-289, scene.cpp:67-77
+__device__ __forceinline__ f3 sky(const SceneDev &sc, f3 d)
+{
+    if (sc.sky_tex < 0) return mk3(0.0f, 0.0f, 0.0f);
+    const float dy = fminf(fmaxf(d.y, -1.0f), 1.0f);
+    const float u  = 0.5f + atan2f(d.z, d.x) * kInvTau;
+    const float v  = 0.5f - asinf(dy) * kInvPi;
+    const float4 c = tex_uv(sc, sc.sky_tex, u + sc.sky_rot[0], v + sc.sky_rot[1]);
+    return mk3(c.x, c.y, c.z);
+}
+
+__device__ __forceinline__ float4 mat_colour(const SceneDev &sc, const MaterialDev &m, float u, float v)
+{
+    if (m.tex >= 0 && (uint32_t)m.tex < sc.ntex) return tex_uv(sc, m.tex, u, v);
+    return make_float4(m.colour[0], m.colour[1], m.colour[2], m.colour[3]);
+}
+
+// sampling.h:53-57
+__device__ __forceinline__ f3 sun_radiance(const SunDev &s, f3 dir)
+{
+    const float c   = fminf(fmaxf(dot(dir, -s.dir), -1.0f), 1.0f);
+    const float ang = acosf(c);
+    return ang < s.size ? s.colour * s.intensity : mk3(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void aov_write(const PathBuffers &b, uint32_t p, f3 alb, f3 nrm, float dep)
+{
+    const uint32_t x = p % b.W, y = p / b.W;
+    const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
+    b.aov_alb[q] = make_float4(alb.x, alb.y, alb.z, 1.0f);
+    b.aov_nrm[q] = make_float4(nrm.x * 0.5f + 0.5f, nrm.y * 0.5f + 0.5f, nrm.z * 0.5f + 0.5f, 1.0f);
+    const float dd = fminf(dep, 200.0f) / 200.0f;
+    b.aov_dep[q] = make_float4(dd, dd, dd, 1.0f);
+}
+
+__device__ __forceinline__ void put_ray(const RaySoA &q, uint32_t k, f3 o, f3 d)
+{
+    q.ox[k] = o.x;
+    q.oy[k] = o.y;
+    q.oz[k] = o.z;
+    q.dx[k] = d.x;
+    q.dy[k] = d.y;
+    q.dz[k] = d.z;
+    q.tn[k] = 1e-5f;              // model.cpp:22
+    q.tf[k] = kInf;               // model.cpp:23
+}
+
+__global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, RenderParams rp)
+{
+    const uint32_t p    = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     live = p < b.P;
+    if (live) {
+        const uint32_t x = p % b.W, y = p / b.W;
+        const rnd4     j = rng4(rp.seed, p, rp.sample, kJitterBounce, 0);
+        f3 o, d;
+        // renderer.cpp:260-263
+        camera_ray(rp.cam, ((float)x + j.a) / (float)b.W, ((float)y + j.b) / (float)b.H, rp.aspect, o, d);
+        put_ray(b.q[0], p, o, d);
+        b.qpath[0][p] = p;
+        b.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        b.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (rp.max_bounces == 0) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+        if (p == 0) b.cnt[kCntQ0] = rp.max_bounces > 0 ? b.P : 0u;
+    }
+    // every path starts at total_bounces = 1 (renderer.cpp:271)
+    wave_add64(&b.stats[2], live ? 1u : 0u);
+}
+
+// Before bounce i: clear the next queue, the shadow queue and the traversal work counters;
+// count the extension rays about to be traced.
+__global__ void k_bounce_prep(PathBuffers b, uint32_t cur)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < 8) {
+        b.cnt[kCntWorkExt + t]    = 0;
+        b.cnt[kCntWorkShadow + t] = 0;
+    }
+    if (t == 0) {
+        b.cnt[cur ^ 1u]       = 0;
+        b.cnt[kCntShadow]     = 0;
+        atomicAdd(&b.stats[0], (unsigned long long)b.cnt[cur]);
+    }
+}
+
+__global__ void k_shadow_prep(PathBuffers b)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < 8) b.cnt[kCntWorkShadow + t] = 0;
+    if (t == 0) {
+        b.cnt[kCntShadow2] = 0;
+        atomicAdd(&b.stats[1], (unsigned long long)b.cnt[kCntShadow]);
+    }
+}
+
+struct HitRec {
+    f3       point, normal;
+    float    dist;
+    float    u, v;      // texture coordinates
+    uint32_t prim;
+};
+
+// model.cpp:5-49 + model.cpp:99-126 for the identity instance
+__device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, float t, uint32_t tri, float bu,
+                                             float bv)
+{
+    HitRec h;
+    const float4 a = sc.tri[3 * (size_t)tri + 0];
+    const float4 b = sc.tri[3 * (size_t)tri + 1];
+    const float4 c = sc.tri[3 * (size_t)tri + 2];
+    h.prim   = __float_as_uint(a.w);
+    const f3 d = normalize(d_in);
+    h.point  = o + d * t;
+    const f3 pd = h.point - o;
+    h.dist   = sqrtf(dot(pd, pd));
+    const f3 v0 = mk3(a.x, a.y, a.z), v1 = mk3(b.x, b.y, b.z), v2 = mk3(c.x, c.y, c.z);
+    h.normal = normalize(cross(v1 - v0, v2 - v0));
+    if (sc.uv) {
+        const float2 w0 = sc.uv[3 * (size_t)h.prim + 0], w1 = sc.uv[3 * (size_t)h.prim + 1],
+                     w2 = sc.uv[3 * (size_t)h.prim + 2];
+        const float b0 = 1.0f - bu - bv;
+        h.u = b0 * w0.x + bu * w1.x + bv * w2.x;
+        h.v = b0 * w0.y + bu * w1.y + bv * w2.y;
+    } else {
+        h.u = h.v = 0.0f;
+    }
+    return h;
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, RenderParams rp, uint32_t bounce,
+                                               uint32_t cur)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = b.cnt[cur];
+    const bool     live = k < n;
+    const RaySoA  &q = b.q[cur];
+    bool     push_next = false, push_shadow = false;
+    f3       no = mk3(0, 0, 0), nd = mk3(0, 0, 0), so = mk3(0, 0, 0), sd = mk3(0, 0, 0), scontrib = mk3(0, 0, 0);
+    uint32_t p = 0, seg = 0;
+    if (live) {
+        p = b.qpath[cur][k];
+        const f3       o   = mk3(q.ox[k], q.oy[k], q.oz[k]);
+        const f3       d   = mk3(q.dx[k], q.dy[k], q.dz[k]);
+        const uint32_t tri = b.hits.tri[k];
+        float4         thr4 = b.thr[p];
+        f3             thr  = mk3(thr4.x, thr4.y, thr4.z);
+        float4         rad4 = b.rad[p];
+        f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
+        const bool     can_continue = bounce + 1 < rp.max_bounces;
+        if (tri == kInvalidRef) {
+            const f3 s = sky(sc, d);
+            if (bounce == 0) aov_write(b, p, s, mk3(0, 0, 0), 0.0f);
+            rad = rad + thr * s;
+        } else {
+            const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k]);
+            const MaterialDev &m = sc.mats[sc.mat_idx[h.prim]];
+            const float4       col = mat_colour(sc, m, h.u, h.v);
+            seg = 1;
+            if (col.w == 0.0f) {
+                // alpha pass-through (renderer.cpp:294-301)
+                if (bounce == 0) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+                no = h.point + d * 0.1f;
+                nd = d;
+                push_next = can_continue;
+            } else {
+                const rnd4 r  = rng4(rp.seed, p, rp.sample, bounce, 0);
+                const f3   n  = h.normal;
+                const f3   cc = mk3(col.x, col.y, col.z);
+                f3   alb = cc, nee_n = n, f0 = mk3(0, 0, 0);
+                bool delta = false, absorbed = false, ggx = false;
+                float ar = 0.0f;
+                if (rp.mode == kModeCompat) {
+                    // renderer.cpp:45-99
+                    if (m.type == 2) {
+                        f3    out_n = n;
+                        const f3 reflected = reflect(d, n);
+                        float ni = 1.0f / m.ior;
+                        if (dot(d, n) > 0.0f) {
+                            out_n = -n;
+                            ni    = m.ior;
+                        }
+                        const f3    uvd  = normalize(d);
+                        const float dt   = dot(uvd, out_n);
+                        const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
+                        no = h.point + out_n * -0.0001f;
+                        nd = disc > 0.0f ? (uvd - out_n * dt) * ni - out_n * sqrtf(disc) : reflected;
+                    } else if (m.type == 0) {
+                        no  = h.point + n * 0.0001f;
+                        nd  = reflect(d, n);
+                        alb = alb * m.reflectiveness;
+                    } else {
+                        const f3 sp = sphere(r.a, r.b);
+                        no = h.point + n * 0.0001f;
+                        nd = normalize(n + sp);
+                    }
+                } else {
+                    const f3 nf = dot(d, n) > 0.0f ? -n : n;
+                    if (m.type == 2) {
+                        delta = true;
+                        f3    out_n = n;
+                        float ni    = 1.0f / m.ior;
+                        if (dot(d, n) > 0.0f) {
+                            out_n = -n;
+                            ni    = m.ior;
+                        }
+                        const f3    uvd  = normalize(d);
+                        const float dt   = dot(uvd, out_n);
+                        const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
+                        bool refl = true;
+                        if (disc > 0.0f) {
+                            const rnd4  r2  = rng4(rp.seed, p, rp.sample, bounce, 1);
+                            const float cf  = ni <= 1.0f ? -dt : sqrtf(disc);
+                            const float r0  = (1.0f - m.ior) / (1.0f + m.ior);
+                            const float f0s = r0 * r0;
+                            const float fr  = f0s + (1.0f - f0s) * powf(1.0f - cf, 5.0f);
+                            refl = r2.b < fr;
+                        }
+                        if (refl) {
+                            no = h.point + out_n * 0.0001f;
+                            nd = reflect(uvd, out_n);
+                        } else {
+                            no = h.point + out_n * -0.0001f;
+                            nd = (uvd - out_n * dt) * ni - out_n * sqrtf(disc);
+                        }
+                    } else if (m.type == 0) {
+                        const float a  = m.roughness;
+                        f0   = cc * m.reflectiveness;
+                        const f3 wo = -normalize(d);
+                        no    = h.point + nf * 0.0001f;
+                        nee_n = nf;
+                        if (a < 1e-3f) {
+                            delta = true;
+                            nd    = reflect(d, nf);
+                            alb   = schlick(dot(wo, nf), f0);
+                        } else {
+                            ggx = true;
+                            ar  = a;
+                            f3 tb, bb;
+                            onb(nf, tb, bb);
+                            const f3 lv = mk3(dot(wo, tb), dot(wo, bb), dot(wo, nf));
+                            // Heitz 2018, sampling the GGX distribution of visible normals
+                            const f3    vh    = normalize(mk3(a * lv.x, a * lv.y, lv.z));
+                            const float lensq = vh.x * vh.x + vh.y * vh.y;
+                            const f3    t1v   = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk3(1, 0, 0);
+                            const f3    t2v   = cross(vh, t1v);
+                            const float rr    = sqrtf(r.a);
+                            const float phi   = kTau * r.b;
+                            const float t1    = rr * cosf(phi);
+                            float       t2    = rr * sinf(phi);
+                            const float s     = 0.5f * (1.0f + vh.z);
+                            t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+                            const float nz2 = fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2);
+                            const f3 nh = t1v * t1 + t2v * t2 + vh * sqrtf(nz2);
+                            const f3 ne = normalize(mk3(a * nh.x, a * nh.y, fmaxf(nh.z, 0.0f)));
+                            const float voh = dot(lv, ne);
+                            const f3 ll = ne * (2.0f * voh) - lv;
+                            if (ll.z <= 0.0f || lv.z <= 0.0f) {
+                                absorbed = true;
+                                alb = mk3(0, 0, 0);
+                                nd  = nf;
+                            } else {
+                                nd = normalize(tb * ll.x + bb * ll.y + nf * ll.z);
+                                const float a2 = a * a;
+                                const float g1 = 2.0f * lv.z / (lv.z + sqrtf(a2 + (1.0f - a2) * lv.z * lv.z));
+                                const float g2 = ggx_vis(lv.z, ll.z, a) * 4.0f * lv.z * ll.z;
+                                alb = schlick(voh, f0) * (g2 / g1);
+                            }
+                        }
+                    } else {
+                        const f3 sp = sphere(r.a, r.b);
+                        no    = h.point + nf * 0.0001f;
+                        nd    = normalize(nf + sp);
+                        nee_n = nf;
+                    }
+                }
+                if (bounce == 0) aov_write(b, p, rp.mode == kModeCompat ? alb : cc, n, h.dist);
+                const f3 thr_prev = thr;
+                if (rp.mode == kModeCompat) {
+                    thr = thr * alb;
+                    rad = rad + thr * m.emission;
+                } else {
+                    rad = rad + (thr_prev * cc) * m.emission;
+                    thr = thr * alb;
+                }
+                // sun NEE (renderer.cpp:315-354)
+                if (sc.sun_enabled && !(rp.mode == kModePbr && delta)) {
+                    const SunDev &sun = sc.sun;
+                    so = h.point + nee_n * 0.001f;
+                    const f3 nd_sun = -sun.dir;
+                    f3 tb, bb;
+                    onb(nd_sun, tb, bb);
+                    const float ct  = 1.0f - r.d * (1.0f - cosf(sun.size));
+                    const float st  = sqrtf(1.0f - ct * ct);
+                    const float phi = kTau * r.c;
+                    const f3    loc = mk3(cosf(phi) * st, ct, sinf(phi) * st);
+                    sd = tb * loc.x + nd_sun * loc.y + bb * loc.z;
+                    const float pdf  = 1.0f / (kTau * (1.0f - cosf(sun.size)));
+                    const float cosv = fminf(fmaxf(dot(nee_n, sd), 0.0f), 1.0f);
+                    if (cosv > 0.0f) {
+                        const f3 L = sun_radiance(sun, sd);
+                        if (rp.mode == kModeCompat) {
+                            const f3 tc = (thr * cc) * cosv * L;
+                            scontrib = mk3(tc.x / pdf, tc.y / pdf, tc.z / pdf);
+                        } else if (ggx) {
+                            const f3    wo  = -normalize(d);
+                            const f3    hv  = normalize(wo + sd);
+                            const float nov = dot(nee_n, wo);
+                            if (nov > 0.0f) {
+                                const float dv = ggx_d(dot(nee_n, hv), ar) * ggx_vis(nov, cosv, ar);
+                                scontrib = ((thr_prev * schlick(dot(wo, hv), f0)) * L) * (dv * cosv / pdf);
+                            }
+                        } else {
+                            scontrib = (thr * L) * (cosv / (pdf * kPi));
+                        }
+                        push_shadow = scontrib.x != 0.0f || scontrib.y != 0.0f || scontrib.z != 0.0f ||
+                                      rp.mode == kModeCompat;
+                    }
+                }
+                if (absorbed) {
+                    seg = 0;
+                } else {
+                    push_next = can_continue;
+                }
+            }
+        }
+        // Russian roulette at the top of the next iteration
+        if (push_next && rp.rr_start && bounce + 1 >= rp.rr_start) {
+            const rnd4 r2 = rng4(rp.seed, p, rp.sample, bounce + 1, 1);
+            const float qv = fminf(maxc(thr), 0.95f);
+            if (r2.a >= qv) push_next = false;
+            else thr = thr * (1.0f / qv);
+        }
+        b.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+        b.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+    const uint32_t ns = wave_append(&b.cnt[kCntShadow], push_shadow);
+    if (push_shadow) {
+        put_ray(b.sq, ns, so, sd);
+        b.spath[ns]    = p;
+        b.scontrib[ns] = make_float4(scontrib.x, scontrib.y, scontrib.z, 0.0f);
+    }
+    const uint32_t nn = wave_append(&b.cnt[cur ^ 1u], push_next);
+    if (push_next) {
+        put_ray(b.q[cur ^ 1u], nn, no, nd);
+        b.qpath[cur ^ 1u][nn] = p;
+    }
+    wave_add64(&b.stats[2], seg);
+}
+
+__global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.cnt[kCntShadow]) return;
+    if (b.occ[k]) return;
+    const uint32_t p = b.spath[k];
+    const float4   c = b.scontrib[k];
+    float4         r = b.rad[p];
+    r.x += c.x;
+    r.y += c.y;
+    r.z += c.z;
+    b.rad[p] = r;
+}
+
+// Alpha-aware shadow step (renderer.cpp:330-345): after a closest-hit trace of the shadow
+// queue, escaped rays add their contribution, rays stopped by an alpha surface continue
+// from hit + 0.1*dir (compacted into the front of the same queue), others are blocked.
+__global__ __launch_bounds__(256) void k_shadow_alpha(SceneDev sc, PathBuffers b)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = b.cnt[kCntShadow];
+    bool     cont = false;
+    f3       o2 = mk3(0, 0, 0), d = mk3(0, 0, 0);
+    uint32_t p = 0;
+    float4   c = make_float4(0, 0, 0, 0);
+    if (k < n) {
+        p = b.spath[k];
+        c = b.scontrib[k];
+        const uint32_t tri = b.shits.tri[k];
+        const f3 o = mk3(b.sq.ox[k], b.sq.oy[k], b.sq.oz[k]);
+        d = mk3(b.sq.dx[k], b.sq.dy[k], b.sq.dz[k]);
+        if (tri == kInvalidRef) {
+            float4 r = b.rad[p];
+            r.x += c.x;
+            r.y += c.y;
+            r.z += c.z;
+            b.rad[p] = r;
+        } else {
+            const HitRec       h   = hit_record(sc, o, d, b.shits.t[k], tri, b.shits.u[k], b.shits.v[k]);
+            const MaterialDev &m   = sc.mats[sc.mat_idx[h.prim]];
+            const float4       col = mat_colour(sc, m, h.u, h.v);
+            if (col.w == 0.0f) {
+                cont = true;
+                o2   = h.point + d * 0.1f;
+            }
+        }
+    }
+    // continuation rays go to a staging count, then the host swaps them in
+    const uint32_t slot = wave_append(&b.cnt[kCntShadow2], cont);
+    if (cont) {
+        // staged after the live entries; k_shadow_compact moves them to the front
+        const uint32_t dst = n + slot;
+        if (dst < b.P * 2u) {
+            put_ray(b.sq, dst, o2, d);
+            b.spath[dst]    = p;
+            b.scontrib[dst] = c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shadow_compact(PathBuffers b)
+{
+    const uint32_t n = b.cnt[kCntShadow];
+    const uint32_t m = b.cnt[kCntShadow2];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) {
+        const uint32_t s = n + k;
+        const RaySoA  &q = b.sq;
+        const float ox = q.ox[s], oy = q.oy[s], oz = q.oz[s], dx = q.dx[s], dy = q.dy[s], dz = q.dz[s];
+        const uint32_t pp = b.spath[s];
+        const float4   cc = b.scontrib[s];
+        // source indices >= n and destinations < m <= n never overlap
+        q.ox[k] = ox; q.oy[k] = oy; q.oz[k] = oz; q.dx[k] = dx; q.dy[k] = dy; q.dz[k] = dz;
+        q.tn[k] = 1e-5f;
+        q.tf[k] = kInf;
+        b.spath[k]    = pp;
+        b.scontrib[k] = cc;
+    }
+}
+
+__global__ void k_shadow_swap(PathBuffers b)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t m = b.cnt[kCntShadow2];
+        b.cnt[kCntShadow]  = m;
+        b.cnt[kCntShadow2] = 0;
+        atomicAdd(&b.stats[1], (unsigned long long)m);
+    }
+    if (threadIdx.x < 8) b.cnt[kCntWorkShadow + threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(PathBuffers b)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.P) return;
+    const uint32_t x = p % b.W, y = p / b.W;
+    const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
+    const float4   r = b.rad[p];
+    b.raw[3 * (size_t)q + 0] += r.x;
+    b.raw[3 * (size_t)q + 1] += r.y;
+    b.raw[3 * (size_t)q + 2] += r.z;
+}
+
+// renderer.cpp:371-383: pow(clamp(sum / n, 0, 1), 1/2.2); FLT_MAX before the first pass
+// (image.h:30-37)
+__global__ __launch_bounds__(256) void k_display(const float *raw, uint64_t n, uint32_t npix, float4 *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    if (n == 0) {
+        out[i] = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX);
+        return;
+    }
+    const float inv = (float)n;
+    float c[3];
+    for (int k = 0; k < 3; ++k) c[k] = powf(fminf(fmaxf(raw[3 * (size_t)i + k] / inv, 0.0f), 1.0f), 1.0f / 2.2f);
+    out[i] = make_float4(c[0], c[1], c[2], 1.0f);
+}
+
+__global__ void k_fill(float4 *p, uint64_t n, float v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(v, v, v, v);
+}
+
+__global__ void k_hits_to_api(SceneDev sc, HitSoA h, uint32_t n, uint32_t *prim, uint32_t *geom, float *ngx,
+                              float *ngy, float *ngz)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tri = h.tri[i];
+    uint32_t pid = kInvalidRef, gid = kInvalidRef;
+    f3 ng = mk3(0, 0, 0);
+    if (tri != kInvalidRef) {
+        const float4 a = sc.tri[3 * (size_t)tri + 0];
+        const float4 b = sc.tri[3 * (size_t)tri + 1];
+        const float4 c = sc.tri[3 * (size_t)tri + 2];
+        pid = __float_as_uint(a.w);
+        const f3 v0 = mk3(a.x, a.y, a.z);
+        ng = cross(mk3(b.x, b.y, b.z) - v0, mk3(c.x, c.y, c.z) - v0);
+        gid = 0;
+        for (uint32_t m = 1; m < sc.nmodels; ++m)
+            if (pid >= sc.model_first[m]) gid = m;
+        pid -= sc.model_first[gid];
+    }
+    prim[i] = pid;
+    if (geom) geom[i] = gid;
+    if (ngx) ngx[i] = ng.x;
+    if (ngy) ngy[i] = ng.y;
+    if (ngz) ngz[i] = ng.z;
+}
+
+}  // namespace
+
+static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s)
+{
+    k_generate<<<blocks(b.P), 256, 0, s>>>(sc, b, rp);
+    return hipGetLastError();
+}
+hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)
+{
+    k_bounce_prep<<<1, 64, 0, s>>>(b, cur);
+    return hipGetLastError();
+}
+hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
+                        uint32_t cur, hipStream_t s)
+{
+    k_shade<<<blocks(b.P), 256, 0, s>>>(sc, b, rp, bounce, cur);
+    return hipGetLastError();
+}
+hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s)
+{
+    k_shadow_prep<<<1, 64, 0, s>>>(b);
+    return hipGetLastError();
+}
+hipError_t launch_shadow_resolve(const PathBuffers &b, hipStream_t s)
+{
+    k_shadow_resolve<<<blocks(b.P), 256, 0, s>>>(b);
+    return hipGetLastError();
+}
+hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hipStream_t s)
+{
+    k_shadow_alpha<<<blocks(b.P), 256, 0, s>>>(sc, b);
+    k_shadow_compact<<<blocks(b.P), 256, 0, s>>>(b);
+    k_shadow_swap<<<1, 64, 0, s>>>(b);
+    return hipGetLastError();
+}
+hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s)
+{
+    k_accumulate<<<blocks(b.P), 256, 0, s>>>(b);
+    return hipGetLastError();
+}
+hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s)
+{
+    k_display<<<blocks(npix), 256, 0, s>>>(raw, n, npix, out);
+    return hipGetLastError();
+}
+hipError_t launch_fill(float4 *p, uint64_t n, float v, hipStream_t s)
+{
+    const uint32_t g = (uint32_t)std::min<uint64_t>(blocks(n), 4096);
+    if (n) k_fill<<<g, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
+                              float *ngx, float *ngy, float *ngz, hipStream_t s)
+{
+    if (n) k_hits_to_api<<<blocks(n), 256, 0, s>>>(sc, h, n, prim, geom, ngx, ngy, ngz);
+    return hipGetLastError();
+}
+
+}  // namespace crt

@@ -1,0 +1,397 @@
+// traverse.hip -- wide-BVH ray traversal for the crt path tracer (gfx950).
+//
+// Replaces rtcIntersect1 (CRender src/objects/model.cpp:27): closest hit in (tnear, tfar]
+// with equal t resolved to the smaller primID; plus an any-hit variant for shadow rays.
+//
+// Execution model (wave64):
+//   * persistent grid: a fixed number of 256-thread blocks per CU; every wave loops
+//     fetching rays until the queue is drained, so the launch shape never depends on the
+//     ray count (which lives in device memory and is produced by the previous kernel).
+//   * XCD-aware fetch: the ray range is cut into 8 contiguous slices, one per XCD
+//     (blocks are dispatched round-robin over XCDs), so neighbouring rays share an L2;
+//     an XCD that drains its slice steals from the others.
+//   * refill: when 16 or more lanes of a wave are idle, one lane reserves a run of rays
+//     for all idle lanes with a single atomic; lanes take their ray by ballot rank.
+//   * per-lane short stack in LDS (entry-major, conflict-free), spilled 16 entries at a
+//     time to a per-lane global region when full.  Entries carry the child's entry
+//     distance so popped nodes beyond the current hit are skipped.
+//   * children are ordered by entry distance with a sorting network; the nearest is
+//     visited next and the others pushed far-to-near.
+#include <hip/hip_runtime.h>
+#include "crt_internal.h"
+
+namespace crt {
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr uint32_t kEmpty = kInvalidRef;
+constexpr uint32_t kMissKey = 0xFFFFFFFFu;
+constexpr uint32_t kRefillIdle = 16;
+// Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
+// 1 + 2*gamma(3) keeps FP32 box tests conservative.
+constexpr float kFarScale = 1.0000003576278687f;
+
+struct RayState {
+    f3     o, d, inv;
+    float  tn, tmax;
+    wt_ray wt;
+    uint32_t best_tri, best_prim;
+    float  bu, bv;
+};
+
+// compare-exchange of (key, ref) pairs; refs travel with their keys so that every array
+// index stays static and the arrays live in registers
+__device__ __forceinline__ void cswap(uint32_t &a, uint32_t &b, uint32_t &ra, uint32_t &rb)
+{
+    const bool     sw = b < a;
+    const uint32_t ka = sw ? b : a, kb = sw ? a : b;
+    const uint32_t xa = sw ? rb : ra, xb = sw ? ra : rb;
+    a  = ka;
+    b  = kb;
+    ra = xa;
+    rb = xb;
+}
+
+template <int W>
+__device__ __forceinline__ void sort_pairs(uint32_t (&k)[W], uint32_t (&r)[W])
+{
+#define CS(i, j) cswap(k[i], k[j], r[i], r[j])
+    if constexpr (W == 4) {
+        CS(0, 1); CS(2, 3);
+        CS(0, 2); CS(1, 3);
+        CS(1, 2);
+    } else {
+        // optimal 19-comparator network for 8 inputs
+        CS(0, 2); CS(1, 3); CS(4, 6); CS(5, 7);
+        CS(0, 4); CS(1, 5); CS(2, 6); CS(3, 7);
+        CS(0, 1); CS(2, 3); CS(4, 5); CS(6, 7);
+        CS(2, 4); CS(3, 5);
+        CS(1, 4); CS(3, 6);
+        CS(1, 2); CS(3, 4); CS(5, 6);
+    }
+#undef CS
+}
+
+// key = entry distance (non-negative float bits, low 3 bits cleared) | child slot
+__device__ __forceinline__ uint32_t make_key(float t, int j)
+{
+    return (__float_as_uint(t) & ~7u) | (uint32_t)j;
+}
+
+// Finite reciprocal: a zero direction component gives +-1e30 instead of inf, so an
+// axis-parallel ray whose origin lies on a slab plane yields [0, big] rather than the
+// NaN of 0 * inf, and the box test stays conservative.
+__device__ __forceinline__ float safe_rcp(float d)
+{
+    return fabsf(d) < 1e-30f ? copysignf(1e30f, d) : 1.0f / d;
+}
+
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const RayState &r,
+                                     float &tentry)
+{
+    float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+    float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+    float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+    const float tnx = fminf(t0x, t1x), tfx = fmaxf(t0x, t1x);
+    const float tny = fminf(t0y, t1y), tfy = fmaxf(t0y, t1y);
+    const float tnz = fminf(t0z, t1z), tfz = fmaxf(t0z, t1z);
+    const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, fmaxf(r.tn, 0.0f)));
+    const float tmax = fminf(fminf(tfx, tfy), tfz) * kFarScale;
+    tentry = tmin;
+    return tmin <= fminf(tmax, r.tmax);
+}
+
+template <int W>
+struct NodeOps;
+
+template <>
+struct NodeOps<4> {
+    static __device__ __forceinline__ int expand(const void *nodes, uint32_t idx, const RayState &r, uint32_t (&key)[4],
+                                                 uint32_t (&ref)[4])
+    {
+        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const Node4 *>(nodes) + idx);
+        const float4  lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+        const uint4   ch = reinterpret_cast<const uint4 *>(p)[6];
+        const uint32_t nc = reinterpret_cast<const uint4 *>(p)[7].x;
+        ref[0] = ch.x;
+        ref[1] = ch.y;
+        ref[2] = ch.z;
+        ref[3] = ch.w;
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        int nh = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float t;
+            const bool h = (uint32_t)j < nc && slab(LX[j], HX[j], LY[j], HY[j], LZ[j], HZ[j], r, t);
+            key[j] = h ? make_key(t, j) : kMissKey;
+            nh += h ? 1 : 0;
+        }
+        return nh;
+    }
+};
+
+template <>
+struct NodeOps<8> {
+    static __device__ __forceinline__ int expand(const void *nodes, uint32_t idx, const RayState &r, uint32_t (&key)[8],
+                                                 uint32_t (&ref)[8])
+    {
+        const uint4 *p  = reinterpret_cast<const uint4 *>(static_cast<const Node8Q *>(nodes) + idx);
+        const uint4  h  = p[0];
+        const uint4  c0 = p[1], c1 = p[2];
+        const uint4  qx = p[3], qy = p[4], qz = p[5];
+        ref[0] = c0.x; ref[1] = c0.y; ref[2] = c0.z; ref[3] = c0.w;
+        ref[4] = c1.x; ref[5] = c1.y; ref[6] = c1.z; ref[7] = c1.w;
+        const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
+        const float sx = __uint_as_float((h.w & 0xFFu) << 23);
+        const float sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23);
+        const uint32_t nc = h.w >> 24;
+        int nh = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int sh = 8 * (j & 3);
+            const uint32_t wlo_x = j < 4 ? qx.x : qx.y, whi_x = j < 4 ? qx.z : qx.w;
+            const uint32_t wlo_y = j < 4 ? qy.x : qy.y, whi_y = j < 4 ? qy.z : qy.w;
+            const uint32_t wlo_z = j < 4 ? qz.x : qz.y, whi_z = j < 4 ? qz.z : qz.w;
+            // lo = fl(origin + q*2^e): q*2^e is exact, one rounding, monotone -> conservative
+            const float lx = fmaf((float)((wlo_x >> sh) & 0xFFu), sx, ox);
+            const float hx = fmaf((float)((whi_x >> sh) & 0xFFu), sx, ox);
+            const float ly = fmaf((float)((wlo_y >> sh) & 0xFFu), sy, oy);
+            const float hy = fmaf((float)((whi_y >> sh) & 0xFFu), sy, oy);
+            const float lz = fmaf((float)((wlo_z >> sh) & 0xFFu), sz, oz);
+            const float hz = fmaf((float)((whi_z >> sh) & 0xFFu), sz, oz);
+            float t;
+            const bool hit = (uint32_t)j < nc && slab(lx, hx, ly, hy, lz, hz, r, t);
+            key[j] = hit ? make_key(t, j) : kMissKey;
+            nh += hit ? 1 : 0;
+        }
+        return nh;
+    }
+};
+
+struct Stack {
+    uint32_t *s_ref;    // LDS [kStackLds][kTraceBlock]
+    float    *s_dist;
+    uint32_t *g;        // global spill, entry-major over all lanes of the grid: [k][2][nlanes]
+    uint32_t  nlanes, gl, tid;
+    uint32_t  sp, gsp;
+
+    uint32_t *err;
+
+    __device__ __forceinline__ void push(uint32_t ref, float dist)
+    {
+        if (sp == kStackLds) {
+            if (gsp + kStackLds > kSpillDepth) {
+                // cannot happen for trees admitted at commit; never write out of bounds
+                *err = 1u;
+                return;
+            }
+            for (uint32_t k = 0; k < kStackLds; ++k) {
+                const size_t e = (size_t)(gsp + k);
+                g[(e * 2 + 0) * nlanes + gl] = s_ref[k * kTraceBlock + tid];
+                g[(e * 2 + 1) * nlanes + gl] = __float_as_uint(s_dist[k * kTraceBlock + tid]);
+            }
+            gsp += kStackLds;
+            sp = 0;
+        }
+        s_ref[sp * kTraceBlock + tid]  = ref;
+        s_dist[sp * kTraceBlock + tid] = dist;
+        ++sp;
+    }
+    // pops the next entry closer than tmax; kEmpty when the stack is exhausted
+    __device__ __forceinline__ uint32_t pop(float tmax)
+    {
+        for (;;) {
+            if (sp == 0) {
+                if (gsp == 0) return kEmpty;
+                gsp -= kStackLds;
+                for (uint32_t k = 0; k < kStackLds; ++k) {
+                    const size_t e = (size_t)(gsp + k);
+                    s_ref[k * kTraceBlock + tid]  = g[(e * 2 + 0) * nlanes + gl];
+                    s_dist[k * kTraceBlock + tid] = __uint_as_float(g[(e * 2 + 1) * nlanes + gl]);
+                }
+                sp = kStackLds;
+            }
+            --sp;
+            const float d = s_dist[sp * kTraceBlock + tid];
+            if (d <= tmax) return s_ref[sp * kTraceBlock + tid];
+        }
+    }
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+template <int W, bool kAny>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ, uint32_t *spill)
+{
+    __shared__ uint32_t s_ref[kStackLds * kTraceBlock];
+    __shared__ float    s_dist[kStackLds * kTraceBlock];
+    const uint32_t n = *q.count;
+    if (n == 0) return;
+    const uint32_t lane  = lane_id();
+    const uint32_t xcd   = blockIdx.x % kXcds;
+    const uint32_t slice = (n + kXcds - 1) / kXcds;
+
+    Stack st;
+    st.s_ref  = s_ref;
+    st.s_dist = s_dist;
+    st.g      = spill;
+    st.nlanes = gridDim.x * blockDim.x;
+    st.gl     = blockIdx.x * blockDim.x + threadIdx.x;
+    st.tid    = threadIdx.x;
+    st.sp = st.gsp = 0;
+    st.err = q.error;
+
+    RayState r;
+    uint32_t rid = 0, cur = kEmpty;
+    bool     active = false, exhausted = false;
+    uint32_t next_slice = 0;   // slices tried by this wave (wave-uniform)
+
+    for (;;) {
+        // ---- refill idle lanes ----
+        uint64_t idle = __ballot(!active);
+        uint32_t nidle = __popcll(idle);
+        while (!exhausted && nidle >= kRefillIdle) {
+            uint32_t base = 0, got = 0;
+            if (lane == 0) {
+                while (next_slice < kXcds) {
+                    const uint32_t x   = (xcd + next_slice) % kXcds;
+                    const uint32_t beg = x * slice;
+                    const uint32_t len = beg < n ? min(slice, n - beg) : 0u;
+                    const uint32_t b   = len ? atomicAdd(q.counters + x, nidle) : len;
+                    if (b < len) {
+                        base = beg + b;
+                        got  = min(nidle, len - b);
+                        break;
+                    }
+                    ++next_slice;
+                }
+            }
+            base       = __shfl(base, 0);
+            got        = __shfl(got, 0);
+            next_slice = __shfl(next_slice, 0);
+            if (got == 0) {
+                exhausted = true;
+                break;
+            }
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+            if (!active && rank < got) {
+                rid = base + rank;
+                r.o   = mk3(q.rays.ox[rid], q.rays.oy[rid], q.rays.oz[rid]);
+                r.d   = mk3(q.rays.dx[rid], q.rays.dy[rid], q.rays.dz[rid]);
+                r.tn  = q.rays.tn[rid];
+                r.tmax = q.rays.tf[rid];
+                r.inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+                r.wt  = wt_setup(r.d);
+                r.best_tri = kInvalidRef;
+                r.best_prim = kInvalidRef;
+                r.bu = r.bv = 0.0f;
+                st.sp = st.gsp = 0;
+                const bool valid = sc.ntri > 0 && r.tn <= r.tmax && r.tmax == r.tmax && r.tn == r.tn &&
+                                   (r.d.x != 0.0f || r.d.y != 0.0f || r.d.z != 0.0f);
+                cur    = valid ? sc.root : kEmpty;
+                active = true;
+            }
+            idle  = __ballot(!active);
+            nidle = __popcll(idle);
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // ---- one traversal step per active lane ----
+        if (active && cur != kEmpty) {
+            if (!ref_is_leaf(cur)) {
+                uint32_t key[W], ref[W];
+                const int nh = NodeOps<W>::expand(sc.nodes, cur, r, key, ref);
+                if (nh == 0) {
+                    cur = st.pop(r.tmax);
+                } else {
+                    sort_pairs<W>(key, ref);
+                    cur = ref[0];
+#pragma unroll
+                    for (int k = W - 1; k >= 1; --k) {
+                        if (key[k] != kMissKey) st.push(ref[k], __uint_as_float(key[k] & ~7u));
+                    }
+                }
+            } else {
+                const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+                bool any = false;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const size_t  ti = (size_t)(first + k);
+                    const float4  a  = sc.tri[3 * ti + 0];
+                    const float4  b  = sc.tri[3 * ti + 1];
+                    const float4  c  = sc.tri[3 * ti + 2];
+                    float t, u, v;
+                    if (wt_test(r.wt, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r.o, r.tn, r.tmax,
+                                t, u, v)) {
+                        const uint32_t prim = __float_as_uint(a.w);
+                        if (t < r.tmax || prim < r.best_prim || r.best_tri == kInvalidRef) {
+                            r.tmax      = t;
+                            r.best_tri  = (uint32_t)ti;
+                            r.best_prim = prim;
+                            r.bu        = u;
+                            r.bv        = v;
+                        }
+                        if (kAny) {
+                            any = true;
+                            break;
+                        }
+                    }
+                }
+                cur = (kAny && any) ? kEmpty : st.pop(r.tmax);
+            }
+        }
+        if (active && cur == kEmpty) {
+            if (kAny) {
+                occ[rid] = r.best_tri != kInvalidRef ? 1 : 0;
+            } else {
+                const bool h = r.best_tri != kInvalidRef;
+                q.hits.t[rid]   = h ? r.tmax : kInf;
+                q.hits.tri[rid] = r.best_tri;
+                if (q.hits.u) q.hits.u[rid] = r.bu;
+                if (q.hits.v) q.hits.v[rid] = r.bv;
+            }
+            active = false;
+        }
+    }
+}
+
+}  // namespace
+
+uint32_t trace_grid(int device, uint32_t blocks_per_cu)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return 1024;
+    if (blocks_per_cu == 0) blocks_per_cu = 4;
+    uint32_t g = (uint32_t)p.multiProcessorCount * blocks_per_cu;
+    // whole multiple of the XCD count so every XCD gets the same number of blocks
+    g = (g + kXcds - 1) / kXcds * kXcds;
+    return g;
+}
+
+uint64_t trace_spill_bytes(uint32_t grid)
+{
+    return (uint64_t)grid * kTraceBlock * kSpillDepth * 2u * sizeof(uint32_t);
+}
+
+hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t, uint32_t *spill, uint32_t grid,
+                                hipStream_t s)
+{
+    if (sc.width == 4) k_trace<4, false><<<grid, kTraceBlock, 0, s>>>(sc, q, nullptr, spill);
+    else k_trace<8, false><<<grid, kTraceBlock, 0, s>>>(sc, q, nullptr, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint32_t, uint8_t *occ, uint32_t *spill,
+                            uint32_t grid, hipStream_t s)
+{
+    if (sc.width == 4) k_trace<4, true><<<grid, kTraceBlock, 0, s>>>(sc, q, occ, spill);
+    else k_trace<8, true><<<grid, kTraceBlock, 0, s>>>(sc, q, occ, spill);
+    return hipGetLastError();
+}
+
+}  // namespace crt

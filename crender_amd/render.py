@@ -1,0 +1,365 @@
+"""Python mirror of CRender's scene / renderer interface over the crt C ABI.
+
+Method names, argument meaning and error behaviour follow the reference:
+  * ``Scene`` mirrors ``cr::scene`` (src/render/scene.h:19-53): add_model, set_skybox,
+    set_skybox_rotation, set_sun_enabled, cast_ray (batched here: ``cast_rays``), plus
+    set_sun / set_camera for the registry's sun and camera entities.
+  * ``Renderer`` mirrors ``cr::renderer`` (src/render/renderer.h:27-68): start, pause,
+    update, set_resolution, set_max_bounces, set_target_spp, current_sample_count,
+    current_resolution, current_progress / normals / albedos / depths, current_stats.
+Failures raise :class:`CrtError` with the library's message; nothing falls back to a
+CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable
+
+import numpy as np
+
+from . import _abi as A
+from ._abi import CrtError, check, fptr, uptr, vptr
+
+
+class Context:
+    def __init__(self, device: int = 0, bvh_width: int = 0, max_leaf: int = 0, phase_timing: bool = False,
+                 trace_waves_per_cu: int = 0):
+        L = A.lib()
+        o = A.Options()
+        L.crt_options_default(C.byref(o))
+        o.device = device
+        o.bvh_width = bvh_width
+        o.max_leaf = max_leaf
+        o.trace_waves_per_cu = trace_waves_per_cu
+        o.flags = A.CRT_FLAG_PHASE_TIMING if phase_timing else 0
+        h = C.c_void_p()
+        check(L.crt_context_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._L = L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        check(self._L.crt_context_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def close(self):
+        if self._h:
+            self._L.crt_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _material(m) -> A.Material:
+    x = A.Material()
+    A.lib().crt_material_default(C.byref(x))
+    x.type = int(m.type)
+    x.ior = float(m.ior)
+    x.roughness = float(m.roughness)
+    x.reflectiveness = float(m.reflectiveness)
+    x.emission = float(m.emission)
+    for k in range(4):
+        x.colour[k] = float(m.colour[k])
+    x.tex = int(m.tex)
+    return x
+
+
+class Scene:
+    """cr::scene over crt_scene_*; geometry is committed (device BVH build) by commit()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._L = A.lib()
+        h = C.c_void_p()
+        check(self._L.crt_scene_create(ctx.handle, C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    @property
+    def handle(self):
+        return self._h
+
+    # scene::add_model (scene.cpp:31-34)
+    def add_model(self, spec) -> int:
+        verts = np.ascontiguousarray(spec.vertices, dtype=np.float32)
+        vidx = np.ascontiguousarray(spec.vertex_indices, dtype=np.uint32)
+        midx = np.ascontiguousarray(spec.material_indices, dtype=np.uint32)
+        md = A.ModelData()
+        md.name = spec.name.encode()
+        md.vertices = fptr(verts)
+        md.num_vertices = verts.shape[0]
+        mats = (A.Material * max(1, len(spec.materials)))(*[_material(m) for m in spec.materials])
+        md.materials = mats
+        md.num_materials = len(spec.materials)
+        keep = [verts, vidx, midx, mats]
+        if spec.texture_coords is not None and spec.texture_indices is not None:
+            tc = np.ascontiguousarray(spec.texture_coords, dtype=np.float32)
+            ti = np.ascontiguousarray(spec.texture_indices, dtype=np.uint32)
+            md.texture_coords = fptr(tc)
+            md.num_texture_coords = tc.shape[0]
+            md.texture_indices = uptr(ti)
+            keep += [tc, ti]
+        texs = [np.ascontiguousarray(t, dtype=np.float32) for t in spec.textures]
+        if texs:
+            arr = (A.Image * len(texs))()
+            for i, t in enumerate(texs):
+                arr[i].rgba = fptr(t)
+                arr[i].height, arr[i].width = t.shape[0], t.shape[1]
+            md.textures = arr
+            md.num_textures = len(texs)
+            keep += texs + [arr]
+        md.vertex_indices = uptr(vidx)
+        md.material_indices = uptr(midx)
+        md.num_triangles = midx.shape[0]
+        mid = C.c_uint32()
+        check(self._L.crt_scene_add_model(self._h, C.byref(md), C.byref(mid)))
+        return mid.value
+
+    def set_materials(self, model_id: int, materials) -> None:
+        mats = (A.Material * max(1, len(materials)))(*[_material(m) for m in materials])
+        check(self._L.crt_scene_set_materials(self._h, model_id, mats, len(materials)))
+
+    def set_instances(self, model_id: int, transforms) -> None:
+        t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float32).reshape(-1, 16))
+        check(self._L.crt_scene_set_instances(self._h, model_id, fptr(t), t.shape[0]))
+
+    # scene::set_skybox (scene.cpp:36-60)
+    def set_skybox(self, rgba) -> None:
+        if rgba is None:
+            check(self._L.crt_scene_set_skybox(self._h, None))
+            return
+        img = np.ascontiguousarray(rgba, dtype=np.float32)
+        im = A.Image(fptr(img), img.shape[1], img.shape[0])
+        check(self._L.crt_scene_set_skybox(self._h, C.byref(im)))
+
+    def set_skybox_rotation(self, x: float, y: float) -> None:
+        check(self._L.crt_scene_set_skybox_rotation(self._h, x, y))
+
+    def set_sun_enabled(self, value: bool) -> None:
+        check(self._L.crt_scene_set_sun_enabled(self._h, 1 if value else 0))
+
+    def set_sun(self, size=None, intensity=None, direction=None, colour=None) -> None:
+        s = A.Sun()
+        self._L.crt_sun_default(C.byref(s))
+        if size is not None:
+            s.size = size
+        if intensity is not None:
+            s.intensity = intensity
+        if direction is not None:
+            d = np.asarray(direction, dtype=np.float64)
+            d = d / np.linalg.norm(d)
+            for k in range(3):
+                s.direction[k] = d[k]
+        if colour is not None:
+            for k in range(3):
+                s.colour[k] = colour[k]
+        check(self._L.crt_scene_set_sun(self._h, C.byref(s)))
+
+    def set_camera(self, cam: dict) -> None:
+        c = A.Camera()
+        self._L.crt_camera_default(C.byref(c))
+        for k in range(3):
+            c.position[k] = cam["position"][k]
+            c.rotation[k] = cam.get("rotation", (0, 0, 0))[k]
+        c.fov = cam.get("fov", 75.0)
+        c.mode = cam.get("mode", 0)
+        c.scale = cam.get("scale", 1.0)
+        m = np.asarray(cam["matrix"], dtype=np.float32).reshape(16)
+        for k in range(16):
+            c.matrix[k] = m[k]
+        check(self._L.crt_scene_set_camera(self._h, C.byref(c)))
+
+    def commit(self) -> None:
+        check(self._L.crt_scene_commit(self._h))
+
+    def bvh_info(self) -> dict:
+        bi = A.BvhInfo()
+        check(self._L.crt_scene_bvh_info(self._h, C.byref(bi)))
+        return {f: getattr(bi, f) for f, _ in A.BvhInfo._fields_}
+
+    def bvh_export(self):
+        nb = C.c_uint64(0)
+        npr = C.c_uint32(0)
+        check(self._L.crt_scene_bvh_export(self._h, None, C.byref(nb), None, C.byref(npr)))
+        nodes = np.zeros(nb.value, dtype=np.uint8)
+        prims = np.zeros(npr.value, dtype=np.uint32)
+        check(self._L.crt_scene_bvh_export(self._h, vptr(nodes), C.byref(nb), vptr(prims), C.byref(npr)))
+        return nodes, prims
+
+    # scene::cast_ray (scene.cpp:79-98), batched
+    def cast_rays(self, org, dirs, tnear=1e-5, tfar=np.inf, normals=False) -> dict:
+        org = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = org.shape[0]
+        comps = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k]) for k in range(3)]
+        tn = np.ascontiguousarray(np.broadcast_to(np.float32(tnear), (n,)) if np.isscalar(tnear) else tnear, dtype=np.float32)
+        tf = np.ascontiguousarray(np.broadcast_to(np.float32(tfar), (n,)) if np.isscalar(tfar) else tfar, dtype=np.float32)
+        rays = A.Rays(*[vptr(a) for a in comps + [tn, tf]])
+        out = {"t": np.empty(n, np.float32), "u": np.empty(n, np.float32), "v": np.empty(n, np.float32),
+               "prim_id": np.empty(n, np.uint32), "geom_id": np.empty(n, np.uint32)}
+        if normals:
+            for k in ("ng_x", "ng_y", "ng_z"):
+                out[k] = np.empty(n, np.float32)
+        hits = A.Hits(*[vptr(out[k]) if k in out else None for k in
+                        ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z")])
+        check(self._L.crt_intersect(self._h, C.byref(rays), n, C.byref(hits), A.CRT_MEM_HOST))
+        return out
+
+    def occluded(self, org, dirs, tnear=1e-5, tfar=np.inf) -> np.ndarray:
+        org = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = org.shape[0]
+        comps = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k]) for k in range(3)]
+        tn = np.ascontiguousarray(np.broadcast_to(np.float32(tnear), (n,)) if np.isscalar(tnear) else tnear, dtype=np.float32)
+        tf = np.ascontiguousarray(np.broadcast_to(np.float32(tfar), (n,)) if np.isscalar(tfar) else tfar, dtype=np.float32)
+        rays = A.Rays(*[vptr(a) for a in comps + [tn, tf]])
+        occ = np.zeros(n, np.uint8)
+        check(self._L.crt_occluded(self._h, C.byref(rays), n, vptr(occ), A.CRT_MEM_HOST))
+        return occ.astype(bool)
+
+    def close(self):
+        if self._h:
+            check(self._L.crt_scene_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_from_spec(ctx: Context, spec, commit: bool = True) -> Scene:
+    """Loads a SceneSpec into a Scene: model, sky, sun, camera; then commits."""
+    sc = Scene(ctx)
+    sc.add_model(spec)
+    sc.set_skybox(spec.sky)
+    sc.set_sun_enabled(spec.sun_enabled)
+    if spec.sun:
+        sc.set_sun(**spec.sun)
+    sc.set_camera(spec.camera)
+    if commit:
+        sc.commit()
+    return sc
+
+
+class Renderer:
+    """cr::renderer (renderer.h:27-68) over crt_renderer_*."""
+
+    def __init__(self, ctx: Context, scene: Scene, res_x: int, res_y: int, bounces: int):
+        self.ctx = ctx
+        self.scene = scene
+        self._L = A.lib()
+        h = C.c_void_p()
+        check(self._L.crt_renderer_create(ctx.handle, scene.handle, res_x, res_y, bounces, C.byref(h)))
+        self._h = h
+
+    def start(self) -> None:
+        check(self._L.crt_renderer_start(self._h))
+
+    def pause(self) -> None:
+        check(self._L.crt_renderer_pause(self._h))
+
+    def update(self, fn: Callable[[], None]) -> None:
+        """renderer.cpp:185-192: pause, mutate, start (which clears the accumulation)."""
+        self.pause()
+        fn()
+        self.start()
+
+    def wait(self) -> None:
+        check(self._L.crt_renderer_wait(self._h))
+
+    def cancel(self) -> None:
+        check(self._L.crt_renderer_cancel(self._h))
+
+    def is_running(self) -> bool:
+        v = C.c_int()
+        check(self._L.crt_renderer_is_running(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def set_resolution(self, x: int, y: int) -> None:
+        check(self._L.crt_renderer_set_resolution(self._h, x, y))
+
+    def set_max_bounces(self, n: int) -> None:
+        check(self._L.crt_renderer_set_max_bounces(self._h, n))
+
+    def set_target_spp(self, n: int) -> None:
+        check(self._L.crt_renderer_set_target_spp(self._h, n))
+
+    def set_options(self, mode: int = 0, rr_start: int = 0, seed: int = 0, sample_offset: int = 0) -> None:
+        o = A.RenderOptions(mode, rr_start, seed, sample_offset)
+        check(self._L.crt_renderer_set_options(self._h, C.byref(o)))
+
+    def render(self, passes: int) -> None:
+        check(self._L.crt_renderer_render(self._h, passes))
+
+    def reset(self) -> None:
+        check(self._L.crt_renderer_reset(self._h))
+
+    def current_sample_count(self) -> int:
+        v = C.c_uint64()
+        check(self._L.crt_renderer_current_sample_count(self._h, C.byref(v)))
+        return v.value
+
+    def current_resolution(self) -> tuple:
+        x, y = C.c_uint32(), C.c_uint32()
+        check(self._L.crt_renderer_current_resolution(self._h, C.byref(x), C.byref(y)))
+        return x.value, y.value
+
+    def _read(self, which: str) -> np.ndarray:
+        W, H = self.current_resolution()
+        ch = 3 if which == "raw" else 4
+        out = np.empty((H, W, ch), np.float32)
+        check(self._L.crt_renderer_read(self._h, A.BUFFER[which], vptr(out)))
+        return out
+
+    def current_progress(self) -> np.ndarray:
+        return self._read("progress")
+
+    def current_normals(self) -> np.ndarray:
+        return self._read("normals")
+
+    def current_albedos(self) -> np.ndarray:
+        return self._read("albedos")
+
+    def current_depths(self) -> np.ndarray:
+        return self._read("depths")
+
+    def raw(self) -> np.ndarray:
+        """_raw_buffer: (H, W, 3) running sums, flipped in x and y."""
+        return self._read("raw")
+
+    def current_stats(self) -> dict:
+        s = A.Stats()
+        check(self._L.crt_renderer_current_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in A.Stats._fields_}
+
+    def checkpoint(self) -> bytes:
+        n = C.c_uint64()
+        check(self._L.crt_renderer_checkpoint_size(self._h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self._L.crt_renderer_checkpoint_save(self._h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data: bytes) -> None:
+        buf = C.create_string_buffer(data, len(data))
+        check(self._L.crt_renderer_checkpoint_load(self._h, buf, len(data)))
+
+    def close(self):
+        if self._h:
+            self._L.crt_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

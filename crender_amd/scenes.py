@@ -1,0 +1,266 @@
+"""Seeded synthetic scenes for the path tracer's workloads.
+
+Each generator returns a :class:`SceneSpec`: indexed geometry in the shape of CRender's
+``asset_loader::model_data`` (asset_loader.h:16-30) plus materials, sky, sun and camera.
+All randomness comes from :mod:`crender_amd.philox`, so a scene is a pure function of
+its seed.
+
+Workloads:
+  * ``cornell``  -- 36-triangle Cornell box: 5 walls, two boxes rotated by seeded angles
+    in [0, 30] degrees, one emissive quad.  Rendered at 256x256, 16 spp, depth 8,
+    Russian roulette after bounce 3.
+  * ``soup``     -- random triangle soup: centroids uniform in the unit cube, vertices
+    offset from the centroid by N(0, 0.005^2 I).  1M triangles at 1920x1080.
+  * ``terrain``  -- fBm height field plus icospheres with diffuse / GGX / dielectric
+    materials under a sun and constant sky.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .philox import Stream
+
+SMOOTH, METAL, GLASS = 1, 0, 2
+
+
+@dataclass
+class MaterialSpec:
+    type: int = SMOOTH
+    ior: float = 1.5
+    roughness: float = 0.5
+    reflectiveness: float = 1.0
+    emission: float = 0.0
+    colour: tuple = (1.0, 1.0, 1.0, 1.0)
+    tex: int = -1
+
+
+@dataclass
+class SceneSpec:
+    name: str
+    vertices: np.ndarray                 # (V, 3) float32
+    vertex_indices: np.ndarray           # (3T,) uint32
+    material_indices: np.ndarray         # (T,) uint32
+    materials: list
+    texture_coords: np.ndarray | None = None   # (U, 2) float32
+    texture_indices: np.ndarray | None = None  # (3T,) uint32
+    textures: list = field(default_factory=list)  # list of (h, w, 4) float32
+    sky: np.ndarray | None = None        # (h, w, 4) float32 or None (black)
+    sun_enabled: bool = False
+    sun: dict | None = None
+    camera: dict = field(default_factory=dict)
+    res: tuple = (256, 256)
+    spp: int = 1
+    max_bounces: int = 8
+    rr_start: int = 0
+    mode: int = 0                        # 0 compat, 1 pbr
+
+    @property
+    def ntri(self) -> int:
+        return int(self.material_indices.shape[0])
+
+    def corners(self) -> np.ndarray:
+        """Expanded (T, 9) corners in primID order (registry.cpp:13-24)."""
+        return self.vertices[self.vertex_indices].reshape(-1, 9).astype(np.float32)
+
+    def corner_uv(self) -> np.ndarray | None:
+        if self.texture_coords is None:
+            return None
+        return self.texture_coords[self.texture_indices].reshape(-1, 6).astype(np.float32)
+
+
+def _quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); winding gives normal (p1-p0)x(p2-p0)."""
+    return [(p0, p1, p2), (p0, p2, p3)]
+
+
+def _box(size, angle_deg, pos):
+    """12 outward-facing triangles of an axis-aligned box rotated about +y."""
+    sx, sy, sz = size
+    a = math.radians(angle_deg)
+    c, s = math.cos(a), math.sin(a)
+
+    def xf(x, y, z):
+        return (pos[0] + c * x + s * z, pos[1] + y, pos[2] - s * x + c * z)
+
+    hx, hz = sx / 2, sz / 2
+    v = {
+        "lbf": xf(-hx, 0, -hz), "rbf": xf(hx, 0, -hz), "rbb": xf(hx, 0, hz), "lbb": xf(-hx, 0, hz),
+        "ltf": xf(-hx, sy, -hz), "rtf": xf(hx, sy, -hz), "rtb": xf(hx, sy, hz), "ltb": xf(-hx, sy, hz),
+    }
+    tris = []
+    tris += _quad(v["lbf"], v["rbf"], v["rbb"], v["lbb"])  # bottom (normal -y)
+    tris += _quad(v["ltf"], v["ltb"], v["rtb"], v["rtf"])  # top (+y)
+    tris += _quad(v["lbf"], v["ltf"], v["rtf"], v["rbf"])  # front (-z)
+    tris += _quad(v["lbb"], v["rbb"], v["rtb"], v["ltb"])  # back (+z)
+    tris += _quad(v["lbf"], v["lbb"], v["ltb"], v["ltf"])  # left (-x)
+    tris += _quad(v["rbf"], v["rtf"], v["rtb"], v["rbb"])  # right (+x)
+    return tris
+
+
+def _from_tris(tris, mats_per_tri):
+    verts = np.asarray([p for t in tris for p in t], dtype=np.float32)
+    idx = np.arange(len(verts), dtype=np.uint32)
+    return verts, idx, np.asarray(mats_per_tri, dtype=np.uint32)
+
+
+def camera_dict(position, rotation=(0.0, 0.0, 0.0), fov=75.0, mode=0, scale=1.0, matrix=None):
+    """Camera with the _cached_matrix of camera.cpp:54-68 (identity rotation here unless given)."""
+    if matrix is None:
+        matrix = camera_matrix(position, rotation)
+    return {"position": tuple(map(float, position)), "rotation": tuple(map(float, rotation)),
+            "fov": float(fov), "mode": int(mode), "scale": float(scale),
+            "matrix": np.asarray(matrix, dtype=np.float32).reshape(16)}
+
+
+def _rot(axis, deg):
+    a = math.radians(deg)
+    c, s = math.cos(a), math.sin(a)
+    x, y, z = axis
+    t = 1 - c
+    return np.array([[c + t * x * x, t * x * y - s * z, t * x * z + s * y, 0],
+                     [t * x * y + s * z, c + t * y * y, t * y * z - s * x, 0],
+                     [t * x * z - s * y, t * y * z + s * x, c + t * z * z, 0],
+                     [0, 0, 0, 1]], dtype=np.float64)
+
+
+def camera_matrix(position, rotation):
+    """T(position) * Ry(rot.x) * Rx(rot.y) * Rz(rot.z), column-major flat (camera.cpp:54-68)."""
+    m = np.eye(4)
+    m[:3, 3] = position
+    m = m @ _rot((0, 1, 0), rotation[0]) @ _rot((1, 0, 0), rotation[1]) @ _rot((0, 0, 1), rotation[2])
+    return m.T.reshape(16).astype(np.float32)   # column-major
+
+
+def constant_sky(value=1.0):
+    return np.full((1, 1, 4), value, dtype=np.float32)
+
+
+def cornell(seed: int = 1) -> SceneSpec:
+    """Cornell box; normals of walls and light face the interior, boxes face outward."""
+    st = Stream(seed, stream=1)
+    ang = st.uniform(2, dims=1)[:, 0] * 30.0
+    red, green, white, light = 0, 1, 2, 3
+    tris, mats = [], []
+
+    def add(ts, m):
+        tris.extend(ts)
+        mats.extend([m] * len(ts))
+
+    add(_quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)), white)     # floor, +y
+    add(_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)), white)     # ceiling, -y
+    add(_quad((-1, 0, 1), (-1, 2, 1), (1, 2, 1), (1, 0, 1)), white)       # back, -z
+    add(_quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)), red)     # left, +x
+    add(_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)), green)       # right, -x
+    add(_box((0.6, 1.2, 0.6), float(ang[0]), (-0.35, 0.0, 0.3)), white)
+    add(_box((0.6, 0.6, 0.6), float(ang[1]), (0.35, 0.0, -0.3)), white)
+    add(_quad((-0.25, 1.98, -0.25), (0.25, 1.98, -0.25), (0.25, 1.98, 0.25), (-0.25, 1.98, 0.25)), light)  # -y
+    verts, idx, mi = _from_tris(tris, mats)
+    materials = [
+        MaterialSpec(colour=(0.63, 0.065, 0.05, 1.0)),
+        MaterialSpec(colour=(0.14, 0.45, 0.091, 1.0)),
+        MaterialSpec(colour=(0.725, 0.71, 0.68, 1.0)),
+        MaterialSpec(colour=(1.0, 1.0, 1.0, 1.0), emission=15.0),
+    ]
+    uv = np.zeros((len(verts), 2), dtype=np.float32)
+    return SceneSpec(name=f"cornell-{seed}", vertices=verts, vertex_indices=idx, material_indices=mi,
+                     materials=materials, texture_coords=uv, texture_indices=idx.copy(),
+                     sky=None, sun_enabled=False, camera=camera_dict((0.0, 1.0, -3.4), fov=40.0),
+                     res=(256, 256), spp=16, max_bounces=8, rr_start=3, mode=0)
+
+
+def soup(n: int = 1_000_000, seed: int = 2, res=(1920, 1080), sigma: float = 0.005) -> SceneSpec:
+    """Random triangle soup; sky constant 1 so the image is not black (the sun is off)."""
+    st = Stream(seed, stream=2)
+    cen = st.uniform(n, dims=3).astype(np.float64)
+    off = Stream(seed, stream=3).normal(3 * n, dims=3) * sigma
+    verts = (np.repeat(cen, 3, axis=0) + off).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.uint32)
+    mi = np.zeros(n, dtype=np.uint32)
+    return SceneSpec(name=f"soup-{n}-{seed}", vertices=verts, vertex_indices=idx, material_indices=mi,
+                     materials=[MaterialSpec(colour=(0.8, 0.8, 0.8, 1.0))], sky=constant_sky(1.0),
+                     sun_enabled=False, camera=camera_dict((0.5, 0.5, -1.0), fov=60.0),
+                     res=tuple(res), spp=1, max_bounces=4, rr_start=0, mode=0)
+
+
+def furnace(albedo: float = 0.5, emission: float = 1.0) -> SceneSpec:
+    """Closed cube, every face emissive with the same albedo: L = E*a / (1 - a) inside
+    (compat: emission is added after the albedo multiply, renderer.cpp:310-311)."""
+    tris = []
+    p = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+    faces = [(0, 1, 2, 3), (4, 7, 6, 5), (0, 3, 7, 4), (1, 5, 6, 2), (0, 4, 5, 1), (3, 2, 6, 7)]
+    for f in faces:   # inward normals
+        tris += _quad(p[f[0]], p[f[1]], p[f[2]], p[f[3]])
+    verts, idx, mi = _from_tris(tris, [0] * len(tris))
+    return SceneSpec(name="furnace", vertices=verts, vertex_indices=idx, material_indices=mi,
+                     materials=[MaterialSpec(colour=(albedo, albedo, albedo, 1.0), emission=emission)],
+                     sky=None, sun_enabled=False, camera=camera_dict((0.0, 0.0, 0.0), fov=60.0),
+                     res=(32, 32), spp=1, max_bounces=64, rr_start=0, mode=0)
+
+
+def rays_from_camera(spec: SceneSpec, res, seed: int = 7, jitter: bool = True):
+    """Primary rays (SoA float32) of a pinhole camera, for traversal parity tests."""
+    W, H = res
+    xs, ys = np.meshgrid(np.arange(W, dtype=np.float32), np.arange(H, dtype=np.float32))
+    xs = xs.reshape(-1)
+    ys = ys.reshape(-1)
+    if jitter:
+        u = Stream(seed, stream=9).uniform(W * H, dims=2)
+        xs = xs + u[:, 0]
+        ys = ys + u[:, 1]
+    else:
+        xs = xs + 0.5
+        ys = ys + 0.5
+    cam = spec.camera
+    m = np.asarray(cam["matrix"], dtype=np.float64).reshape(4, 4).T
+    aspect = W / H
+    u = (2.0 * xs / W - 1.0) * aspect
+    v = 2.0 * ys / H - 1.0
+    w = 1.0 / math.tan(0.5 * math.radians(cam["fov"]))
+    d = (m[:3, :3] @ np.stack([u, v, np.full_like(u, w)]).astype(np.float64)).T
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.broadcast_to(np.asarray(cam["position"], dtype=np.float32), d.shape)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def diffuse_rays(n: int, seed: int, lo=0.0, hi=1.0):
+    """Rays with origins uniform in a box and uniformly random directions."""
+    u = Stream(seed, stream=10).uniform(n, dims=6).astype(np.float64)
+    o = lo + (hi - lo) * u[:, :3]
+    z = 2.0 * u[:, 3] - 1.0
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    ph = 2.0 * np.pi * u[:, 4]
+    d = np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=1)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def sunlit(seed: int = 4, mode: int = 0) -> SceneSpec:
+    """Ground quad and four boxes (diffuse, GGX metal, mirror, glass) under the default sun
+    and a constant sky: exercises every material branch and the sun NEE."""
+    st = Stream(seed, stream=4)
+    ang = st.uniform(4, dims=1)[:, 0] * 30.0
+    tris, mats = [], []
+
+    def add(ts, m):
+        tris.extend(ts)
+        mats.extend([m] * len(ts))
+
+    add(_quad((-4, 0, -4), (-4, 0, 4), (4, 0, 4), (4, 0, -4)), 0)
+    add(_box((0.8, 0.8, 0.8), float(ang[0]), (-1.2, 0.0, 0.5)), 1)
+    add(_box((0.8, 1.2, 0.8), float(ang[1]), (0.0, 0.0, 1.0)), 2)
+    add(_box((0.6, 0.6, 0.6), float(ang[2]), (1.2, 0.0, 0.2)), 3)
+    add(_box((0.5, 0.9, 0.5), float(ang[3]), (0.4, 0.0, -0.6)), 4)
+    verts, idx, mi = _from_tris(tris, mats)
+    materials = [
+        MaterialSpec(colour=(0.6, 0.6, 0.6, 1.0)),
+        MaterialSpec(colour=(0.8, 0.3, 0.2, 1.0)),
+        MaterialSpec(type=METAL, roughness=0.3, colour=(0.9, 0.8, 0.5, 1.0)),
+        MaterialSpec(type=METAL, roughness=0.0, colour=(0.95, 0.95, 0.95, 1.0), reflectiveness=0.9),
+        MaterialSpec(type=GLASS, ior=1.5, colour=(0.95, 0.95, 1.0, 1.0)),
+    ]
+    cam = camera_dict((0.0, 1.5, -4.0), rotation=(0.0, -12.0, 0.0), fov=50.0)
+    return SceneSpec(name=f"sunlit-{seed}", vertices=verts, vertex_indices=idx, material_indices=mi,
+                     materials=materials, sky=constant_sky(0.4), sun_enabled=True, sun=None, camera=cam,
+                     res=(96, 64), spp=4, max_bounces=6, rr_start=3, mode=mode)

@@ -1,0 +1,263 @@
+/*
+ * crt.h -- C ABI of the crt MI355X path tracer.
+ *
+ * This is the drop-in boundary for CRender's render hot path (the src/render bounce loop
+ * that calls Embree).  Every entry point below replaces one reference interface; the
+ * reference file:line is given beside it.  All functions return CRT_OK (0) or a negative
+ * CRT_ERR_* code and never exit the process; crt_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Threading: a scene is immutable after crt_scene_commit.  Calls on distinct renderers
+ * are thread-safe.  Each context owns one HIP stream; every renderer and query of a
+ * context is ordered on it.  Mutating a scene or renderer requires the renderer to be
+ * paused, mirroring cr::renderer::update (renderer.cpp:185-192).
+ *
+ * Memory: buffers passed to crt_scene_* are copied at commit; the caller may free them
+ * after the call returns (the reference shares them with Embree instead,
+ * model.cpp:60-90).  crt_intersect/crt_occluded accept host or device pointers
+ * (crt_memory).
+ */
+#ifndef CRT_H
+#define CRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRT_ABI_VERSION 1
+
+enum crt_status {
+    CRT_OK               = 0,
+    CRT_ERR_INVALID_ARG  = -1,
+    CRT_ERR_INVALID_MESH = -2,
+    CRT_ERR_HIP          = -3,
+    CRT_ERR_OUT_OF_MEMORY = -4,
+    CRT_ERR_STATE        = -5, /* e.g. mutation while rendering, query before commit */
+    CRT_ERR_UNSUPPORTED  = -6,
+    CRT_ERR_CANCELLED    = -7,
+    CRT_ERR_CHECKPOINT   = -8
+};
+
+enum crt_memory { CRT_MEM_HOST = 0, CRT_MEM_DEVICE = 1 };
+
+/* material.h:14-19 */
+enum crt_material_type { CRT_METAL = 0, CRT_SMOOTH = 1, CRT_GLASS = 2 };
+
+/* estimator: compat reproduces renderer.cpp:21-102,258-384 (with Philox); pbr fixes the
+ * estimator (Lambert NEE with 1/pi, GGX metal, Fresnel glass, normals facing the ray). */
+enum crt_mode { CRT_MODE_COMPAT = 0, CRT_MODE_PBR = 1 };
+
+/* buffers of cr::renderer (renderer.h:62-68, renderer.cpp:85-91) */
+enum crt_buffer {
+    CRT_BUFFER_PROGRESS = 0, /* current_progress(): RGBA, pow(clamp(sum/n,0,1),1/2.2) */
+    CRT_BUFFER_NORMALS  = 1, /* current_normals():  RGBA, n*0.5+0.5 of the last pass */
+    CRT_BUFFER_ALBEDOS  = 2, /* current_albedos():  RGBA */
+    CRT_BUFFER_DEPTHS   = 3, /* current_depths():   RGBA, min(d,200)/200 */
+    CRT_BUFFER_RAW      = 4  /* _raw_buffer: RGB FP32 running sum (W*H*3) */
+};
+
+/* material::information (material.h:31-41) without the UI name */
+typedef struct crt_material {
+    int32_t type;            /* crt_material_type */
+    float   ior;             /* default 1.5 */
+    float   roughness;       /* default 0.5 (GGX alpha in pbr mode) */
+    float   reflectiveness;  /* default 1 */
+    float   emission;        /* default 0 */
+    float   colour[4];       /* default (1,1,1,1); colour[3] == 0 is alpha pass-through */
+    int32_t tex;             /* -1: none; else index into the model's textures */
+} crt_material;
+
+/* cr::image (image.h:13-151): RGBA FP32, (x + y*w)*4 */
+typedef struct crt_image {
+    const float *rgba;
+    uint32_t     width, height;
+} crt_image;
+
+/* asset_loader::model_data (asset_loader.h:16-30).  Triangles only: vertex_indices,
+ * texture_indices hold 3 entries per triangle, material_indices 1.  The model is
+ * de-indexed ("expanded") at add time as registry.cpp:13-24,62-66 does, so
+ * primID = triangle index in this order. */
+typedef struct crt_model_data {
+    const char         *name;
+    const float        *vertices;         /* vec3 */
+    uint32_t            num_vertices;
+    const float        *texture_coords;   /* vec2, may be NULL */
+    uint32_t            num_texture_coords;
+    const crt_material *materials;
+    uint32_t            num_materials;
+    const crt_image    *textures;
+    uint32_t            num_textures;
+    const uint32_t     *vertex_indices;   /* 3 per triangle */
+    const uint32_t     *texture_indices;  /* 3 per triangle, may be NULL */
+    const uint32_t     *material_indices; /* 1 per triangle */
+    uint32_t            num_triangles;
+} crt_model_data;
+
+/* entity::sun (components.h:23-29) */
+typedef struct crt_sun {
+    float size;       /* cone half-angle, default pi/48 */
+    float intensity;  /* default 100 */
+    float direction[3];
+    float colour[3];
+} crt_sun;
+
+/* cr::camera (camera.h:18-49).  matrix is _cached_matrix, column-major; crt_camera_update
+ * recomputes it from position and rotation as camera.cpp:54-68 does. */
+typedef struct crt_camera {
+    float   position[3];
+    float   rotation[3];  /* degrees: yaw about +y, pitch about +x, roll about +z */
+    float   fov;          /* degrees, default 75 */
+    int32_t mode;         /* 0 perspective, 1 orthographic */
+    float   scale;        /* orthographic half-extent */
+    float   matrix[16];
+} crt_camera;
+
+typedef struct crt_options {
+    int32_t  device;        /* HIP device ordinal */
+    uint32_t bvh_width;     /* 0 = autotune, 2/4/8 */
+    uint32_t max_leaf;      /* 0 = default (4) */
+    uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy */
+    uint32_t flags;         /* CRT_FLAG_* */
+} crt_options;
+
+#define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
+
+typedef struct crt_render_options {
+    uint32_t mode;          /* crt_mode */
+    uint32_t rr_start;      /* 0 = no Russian roulette, else RR before bounce >= rr_start */
+    uint32_t seed;          /* Philox key word 0 */
+    uint32_t sample_offset; /* absolute index of the next pass (resume) */
+} crt_render_options;
+
+/* renderer::renderer_stats (renderer.h:48-54) plus per-kind ray counts and phase times. */
+typedef struct crt_stats {
+    uint64_t rays_per_second;    /* reference counting: total_rays / running_time */
+    uint64_t samples_per_second;
+    uint64_t total_rays;         /* reference counting: sum of total_bounces (renderer.cpp:271-272,356) */
+    double   running_time;       /* seconds since start() */
+    uint64_t samples;            /* completed passes */
+    uint64_t extension_rays;     /* closest-hit rays traced */
+    uint64_t shadow_rays;        /* sun shadow rays traced */
+    double   ms_generate, ms_extend, ms_shade, ms_shadow, ms_accumulate; /* with CRT_FLAG_PHASE_TIMING */
+    double   ms_render;          /* GPU time of all passes (HIP events) */
+    uint64_t extend_launches;    /* closest-hit launches covered by ms_extend */
+} crt_stats;
+
+/* BVH summary returned by crt_scene_bvh_info */
+typedef struct crt_bvh_info {
+    uint32_t width;
+    uint32_t node_count;       /* wide nodes */
+    uint32_t bvh2_node_count;  /* binary nodes before collapse */
+    uint32_t depth;
+    uint32_t max_stack;        /* worst-case traversal stack entries */
+    double   sah_cost;
+    double   build_ms;         /* device build (binned SAH + collapse), HIP events */
+    uint64_t bytes;            /* nodes + triangles in HBM */
+} crt_bvh_info;
+
+typedef struct crt_rays {     /* SoA, Embree RTCRayNp-like */
+    const float *org_x, *org_y, *org_z;
+    const float *dir_x, *dir_y, *dir_z;
+    const float *tnear, *tfar;
+} crt_rays;
+
+typedef struct crt_hits {     /* SoA; any pointer except t/prim_id may be NULL */
+    float    *t;              /* +inf on miss */
+    float    *u, *v;          /* barycentrics of v1 and v2 (Embree convention) */
+    uint32_t *prim_id;        /* 0xFFFFFFFF on miss (RTC_INVALID_GEOMETRY_ID) */
+    uint32_t *geom_id;        /* model index, 0xFFFFFFFF on miss */
+    float    *ng_x, *ng_y, *ng_z; /* unnormalised (v1-v0)x(v2-v0) */
+} crt_hits;
+
+typedef struct crt_context  crt_context;
+typedef struct crt_scene    crt_scene;
+typedef struct crt_renderer crt_renderer;
+
+/* ---- errors / version -------------------------------------------------------------- */
+const char *crt_last_error(void);
+int         crt_abi_version(void);
+
+/* ---- lifecycle: rtcNewDevice (components.h:67) + the renderer's thread pool ----------- */
+void crt_options_default(crt_options *opt);
+int  crt_context_create(const crt_options *opt, crt_context **out);
+int  crt_context_destroy(crt_context *ctx);
+int  crt_context_device_name(crt_context *ctx, char *buf, size_t len);
+
+/* ---- scene-load: cr::scene (scene.h:19-53) ------------------------------------------ */
+int crt_scene_create(crt_context *ctx, crt_scene **out);                      /* scene() */
+int crt_scene_destroy(crt_scene *scene);
+/* scene::add_model (scene.cpp:31-34) -> registry::register_model (registry.cpp:51-97) */
+int crt_scene_add_model(crt_scene *scene, const crt_model_data *model, uint32_t *model_id);
+/* entity::instances (components.h:18-21); transforms are column-major 4x4.  Identity by
+ * default (registry.cpp:71-72).  Non-identity instances are traced as model.cpp:99-126. */
+int crt_scene_set_instances(crt_scene *scene, uint32_t model_id, const float *transforms, uint32_t count);
+/* replace a model's material table (ui.h:924-929 edits) */
+int crt_scene_set_materials(crt_scene *scene, uint32_t model_id, const crt_material *mats, uint32_t count);
+int crt_scene_set_skybox(crt_scene *scene, const crt_image *sky);             /* scene.cpp:36-60 */
+int crt_scene_set_skybox_rotation(crt_scene *scene, float x, float y);        /* scene.cpp:62-65 */
+int crt_scene_set_sun_enabled(crt_scene *scene, int enabled);                 /* scene.cpp:115-118 */
+int crt_scene_set_sun(crt_scene *scene, const crt_sun *sun);                  /* registry.cpp:248-256 */
+int crt_scene_set_camera(crt_scene *scene, const crt_camera *camera);         /* registry camera entity */
+/* Uploads geometry and builds the BVH on the device: rtcCommitGeometry / rtcCommitScene
+ * (model.cpp:92-94). */
+int crt_scene_commit(crt_scene *scene);
+int crt_scene_bvh_info(crt_scene *scene, crt_bvh_info *info);
+/* Copies the wide BVH out for structural checks: nodes as stored (128 B each) and the
+ * leaf-order primID table.  Pass NULL to query sizes. */
+int crt_scene_bvh_export(crt_scene *scene, void *nodes, uint64_t *node_bytes, uint32_t *prims, uint32_t *nprims);
+
+/* ---- ray query: scene::cast_ray (scene.cpp:79-98) -> rtcIntersect1 (model.cpp:27) ------ */
+int crt_intersect(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, int memory);
+/* any hit in (tnear, tfar]; occluded[i] = 1/0 */
+int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory);
+
+/* ---- render / accumulate: cr::renderer (renderer.h:27-68) ---------------------------- */
+/* renderer(res_x, res_y, bounces, pool, scene) (renderer.cpp:106-145).  Created paused;
+ * crt_renderer_start begins background passes as the management thread does. */
+int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint32_t res_y,
+                        uint32_t bounces, crt_renderer **out);
+int crt_renderer_destroy(crt_renderer *r);                                    /* ~renderer */
+int crt_renderer_start(crt_renderer *r);      /* renderer.cpp:154-170 (clears accumulation) */
+int crt_renderer_pause(crt_renderer *r);      /* renderer.cpp:172-183 (waits for the pass) */
+int crt_renderer_is_running(crt_renderer *r, int *running);
+int crt_renderer_wait(crt_renderer *r);       /* block until the target spp is reached */
+int crt_renderer_set_resolution(crt_renderer *r, uint32_t x, uint32_t y);    /* renderer.cpp:194-208 */
+int crt_renderer_set_max_bounces(crt_renderer *r, uint32_t bounces);         /* renderer.cpp:210-213 */
+int crt_renderer_set_target_spp(crt_renderer *r, uint64_t target);           /* renderer.cpp:215-218 */
+int crt_renderer_set_options(crt_renderer *r, const crt_render_options *opt);
+int crt_renderer_get_options(crt_renderer *r, crt_render_options *opt);
+/* Synchronous passes on the caller's thread (the renderer must be paused). */
+int crt_renderer_render(crt_renderer *r, uint32_t passes);
+/* Clears accumulation, sample count and stats without starting (start() minus resume). */
+int crt_renderer_reset(crt_renderer *r);
+int crt_renderer_current_sample_count(crt_renderer *r, uint64_t *count);     /* renderer.cpp:391-394 */
+int crt_renderer_current_resolution(crt_renderer *r, uint32_t *x, uint32_t *y); /* renderer.cpp:386-389 */
+/* current_progress/normals/albedos/depths (renderer.cpp:220-238) and _raw_buffer:
+ * copies into host memory (W*H*4 floats, or W*H*3 for CRT_BUFFER_RAW). */
+int crt_renderer_read(crt_renderer *r, int buffer, float *out);
+/* Device pointer of a buffer (valid until resize/destroy); PROGRESS is refreshed first. */
+int crt_renderer_device_buffer(crt_renderer *r, int buffer, void **dev_ptr);
+int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats);           /* renderer.cpp:396-404 */
+/* Stop the background passes at the next bounce boundary (pause() without waiting). */
+int crt_renderer_cancel(crt_renderer *r);
+
+/* Checkpoint = {raw sum, sample count, render options, resolution, bounces, scene hash}.
+ * Resuming continues at the stored sample index; with the counter-based RNG the result
+ * equals an uninterrupted run bit for bit. */
+int crt_renderer_checkpoint_size(crt_renderer *r, uint64_t *bytes);
+int crt_renderer_checkpoint_save(crt_renderer *r, void *buf, uint64_t bytes);
+int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t bytes);
+
+/* camera.cpp:54-68: matrix = T(position) * Ry(rot.x) * Rx(rot.y) * Rz(rot.z) */
+void crt_camera_default(crt_camera *cam);   /* camera.h:21: position (5,5,0), fov 75, identity */
+void crt_camera_update(crt_camera *cam);
+void crt_sun_default(crt_sun *sun);         /* components.h:23-29 */
+void crt_material_default(crt_material *m); /* material.h:31-41 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

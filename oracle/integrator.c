@@ -1,0 +1,173 @@
+/*
+ * integrator.c -- oracle render driver (TEST INFRASTRUCTURE ONLY).
+ *
+ * Instantiates integrator_impl.h in FP32 and FP64 and schedules it as CRender does:
+ * one task per image row, all rows of one sample pass finished before the next pass
+ * starts (src/render/renderer.cpp:116-144,240-256; src/objects/thread_pool.cpp:40-60).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "oracle.h"
+
+
+void orc_rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t block, float u[4])
+{
+    const uint32_t c[4] = {pixel, sample, bounce, block};
+    const uint32_t k[2] = {seed, ORC_KEY_HI};
+    uint32_t       r[4];
+    orc_philox4x32_10(c, k, r);
+    for (int i = 0; i < 4; ++i) u[i] = orc_u01(r[i]);
+}
+
+/* image.h:104-109: static_cast<uint64_t>(u * width) % width.  The cast truncates toward
+ * zero; negative products (undefined in the original) wrap modulo the size here. */
+void orc_get_uv(const orc_image *img, float u, float v, float out[4])
+{
+    const int64_t w = img->w, h = img->h;
+    int64_t       x = (int64_t)(u * (float)img->w);
+    int64_t       y = (int64_t)(v * (float)img->h);
+    x = ((x % w) + w) % w;
+    y = ((y % h) + h) % h;
+    const float *p = img->rgba + (size_t)(x + y * w) * 4;
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
+    out[3] = p[3];
+}
+
+#define R float
+#define SFX f32
+#define SQRT_ sqrtf
+#define SIN_ sinf
+#define COS_ cosf
+#define TAN_ tanf
+#define ATAN2_ atan2f
+#define ASIN_ asinf
+#define ACOS_ acosf
+#define POW_ powf
+#define FABS_ fabsf
+#include "integrator_impl.h"
+#undef R
+#undef SFX
+#undef SQRT_
+#undef SIN_
+#undef COS_
+#undef TAN_
+#undef ATAN2_
+#undef ASIN_
+#undef ACOS_
+#undef POW_
+#undef FABS_
+#undef PI_R
+#undef TAU_R
+
+#define R double
+#define SFX f64
+#define SQRT_ sqrt
+#define SIN_ sin
+#define COS_ cos
+#define TAN_ tan
+#define ATAN2_ atan2
+#define ASIN_ asin
+#define ACOS_ acos
+#define POW_ pow
+#define FABS_ fabs
+#include "integrator_impl.h"
+
+typedef struct {
+    const orc_scene    *sc;
+    const orc_settings *st;
+    void               *raw;
+    int                 is64;
+    float              *alb, *nrm, *dep;
+    uint32_t            sample;
+    uint32_t            row_end;
+    volatile uint32_t   next_row;
+    uint64_t            ext, shadow, segs;
+    pthread_mutex_t     mu;
+} pass_job;
+
+static void *pass_worker(void *arg)
+{
+    pass_job *job = (pass_job *)arg;
+    cnt_f32   c32 = {0, 0, 0};
+    cnt_f64   c64 = {0, 0, 0};
+    for (;;) {
+        const uint32_t y = __atomic_fetch_add(&job->next_row, 1u, __ATOMIC_RELAXED);
+        if (y >= job->row_end) break;
+        for (uint32_t x = 0; x < job->st->res_x; ++x) {
+            if (job->is64)
+                sample_pixel_f64(job->sc, job->st, x, y, job->sample, (double *)job->raw, job->alb, job->nrm, job->dep, &c64);
+            else
+                sample_pixel_f32(job->sc, job->st, x, y, job->sample, (float *)job->raw, job->alb, job->nrm, job->dep, &c32);
+        }
+    }
+    pthread_mutex_lock(&job->mu);
+    job->ext += c32.ext + c64.ext;
+    job->shadow += c32.shadow + c64.shadow;
+    job->segs += c32.segs + c64.segs;
+    pthread_mutex_unlock(&job->mu);
+    return NULL;
+}
+
+static int render_common(const orc_scene *sc, const orc_settings *st, void *raw, int is64, float *alb,
+                         float *nrm, float *dep, orc_stats *stats)
+{
+    if (!sc || !st || !raw || st->res_x == 0 || st->res_y == 0) return -1;
+    if (sc->ntri && (!sc->tri || !sc->mat_idx || !sc->mats)) return -2;
+    for (uint32_t i = 0; i < sc->ntri; ++i)
+        if (sc->mat_idx[i] >= sc->nmat) return -3;
+    if (sc->trav == ORC_TRAV_BVH32 && !sc->bvh) return -4;
+    int nthreads = st->nthreads > 0 ? st->nthreads : orc_default_threads();
+    if (nthreads > 256) nthreads = 256;
+    const uint32_t  row_begin = st->row_begin;
+    const uint32_t  row_end   = st->row_end ? (st->row_end < st->res_y ? st->row_end : st->res_y) : st->res_y;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    uint64_t ext = 0, shadow = 0, segs = 0;
+    for (uint32_t s = 0; s < st->spp; ++s) {
+        pass_job job;
+        memset(&job, 0, sizeof job);
+        job.sc       = sc;
+        job.st       = st;
+        job.raw      = raw;
+        job.is64     = is64;
+        job.alb      = alb;
+        job.nrm      = nrm;
+        job.dep      = dep;
+        job.sample   = st->sample_offset + s;
+        job.row_end  = row_end;
+        job.next_row = row_begin;
+        pthread_mutex_init(&job.mu, NULL);
+        pthread_t th[256];
+        for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, pass_worker, &job);
+        for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL); /* barrier per pass */
+        pthread_mutex_destroy(&job.mu);
+        ext += job.ext;
+        shadow += job.shadow;
+        segs += job.segs;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (stats) {
+        stats->ext_rays     = ext;
+        stats->shadow_rays  = shadow;
+        stats->ref_segments = segs;
+        stats->seconds      = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    }
+    return 0;
+}
+
+int orc_render_f32(const orc_scene *scene, const orc_settings *s, float *raw, float *albedo,
+                   float *normal, float *depth, orc_stats *stats)
+{
+    return render_common(scene, s, raw, 0, albedo, normal, depth, stats);
+}
+
+int orc_render_f64(const orc_scene *scene, const orc_settings *s, double *raw, float *albedo,
+                   float *normal, float *depth, orc_stats *stats)
+{
+    return render_common(scene, s, raw, 1, albedo, normal, depth, stats);
+}

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every function include/crt.h declares (CPU only)."""
+import ctypes
+
+import pytest
+
+from crender_amd import _abi
+
+
+def test_header_functions_are_bound():
+    names = _abi.header_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if n not in _abi.SIGNATURES]
+    assert not missing, f"declared in crt.h but not bound in _abi.SIGNATURES: {missing}"
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_abi.LIBPATH))
+    missing = [n for n in _abi.header_functions() if not hasattr(lib, n)]
+    assert not missing, f"libcrt.so lacks {missing}"
+
+
+def test_cpu_only_entry_points():
+    L = _abi.lib()
+    assert L.crt_abi_version() == 1
+    cam = _abi.Camera()
+    L.crt_camera_default(ctypes.byref(cam))
+    assert tuple(cam.position) == (5.0, 5.0, 0.0) and cam.fov == 75.0   # camera.h:21
+    assert list(cam.matrix) == [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
+    sun = _abi.Sun()
+    L.crt_sun_default(ctypes.byref(sun))
+    assert sun.intensity == 100.0 and abs(sun.size - 3.14159265359 / 48) < 1e-7   # components.h:23-29
+    m = _abi.Material()
+    L.crt_material_default(ctypes.byref(m))
+    assert (m.type, m.ior, m.roughness, m.reflectiveness, m.emission, m.tex) == (1, 1.5, 0.5, 1.0, 0.0, -1)
+
+
+def test_camera_update_matches_glm_order():
+    """camera.cpp:54-68: T(p) * Ry(x) * Rx(y) * Rz(z), checked against numpy."""
+    import numpy as np
+    from crender_amd import scenes
+
+    L = _abi.lib()
+    cam = _abi.Camera()
+    L.crt_camera_default(ctypes.byref(cam))
+    pos, rot = (1.0, 2.0, -3.0), (30.0, -20.0, 10.0)
+    for k in range(3):
+        cam.position[k] = pos[k]
+        cam.rotation[k] = rot[k]
+    L.crt_camera_update(ctypes.byref(cam))
+    ref = scenes.camera_matrix(pos, rot)
+    np.testing.assert_allclose(np.array(cam.matrix), ref, atol=1e-6)
+    # a pure yaw of 90 degrees turns +z (the camera's forward) into +x
+    m = scenes.camera_matrix((0, 0, 0), (90, 0, 0)).reshape(4, 4).T
+    np.testing.assert_allclose(m[:3, :3] @ [0, 0, 1], [1, 0, 0], atol=1e-6)
+
+
+def test_errors_without_gpu_are_reported_not_fatal():
+    """Invalid arguments return codes with a message; the process keeps running."""
+    L = _abi.lib()
+    rc = L.crt_context_create(None, None)
+    assert rc == -1
+    assert b"null" in L.crt_last_error()
