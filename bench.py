@@ -71,9 +71,24 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=0, help="BVH width (0 = library default)")
+    ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent traversal occupancy (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) 5 times, for scaling "
+                         "profiler byte counters; prints the bytes moved per launch")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)   # one elementwise kernel: reads 4n bytes, writes 4n bytes
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
               file=sys.stderr)
@@ -83,7 +98,7 @@ def main(argv=None) -> int:
 
     res, bounces = (1920, 1080), 4
     spec = scenes.soup(args.tris, seed=2, res=res)
-    ctx = Context(0, bvh_width=args.width, phase_timing=True)
+    ctx = Context(0, bvh_width=args.width, phase_timing=True, trace_waves_per_cu=args.waves_per_cu)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
@@ -126,6 +141,7 @@ def main(argv=None) -> int:
             "workload": f"triangle soup {args.tris} tris, 1920x1080, 1 spp, primary + 3 diffuse bounces "
                         f"(depth 4), compat estimator, sun off, constant sky",
             "bvh_width": info["width"],
+            "waves_per_cu": args.waves_per_cu,
             "device": dev,
         },
         "roofline": {

@@ -171,3 +171,18 @@ int orc_render_f64(const orc_scene *scene, const orc_settings *s, double *raw, f
 {
     return render_common(scene, s, raw, 1, albedo, normal, depth, stats);
 }
+
+/* GGX pieces in FP64 for the sampling / quadrature checks (sampling.h:93-140). */
+double orc_ggx_d(double noh, double alpha) { return ggx_d_f64(noh, alpha); }
+double orc_ggx_vis(double nov, double nol, double alpha) { return ggx_vis_f64(nov, nol, alpha); }
+int orc_ggx_sample(const double wo_local[3], double alpha, double u1, double u2, double wi_local[3], double *weight)
+{
+    vec_f64 ll;
+    double  g, voh;
+    if (!ggx_sample_f64(mk_f64(wo_local[0], wo_local[1], wo_local[2]), alpha, u1, u2, &ll, &g, &voh)) return 0;
+    wi_local[0] = ll.x;
+    wi_local[1] = ll.y;
+    wi_local[2] = ll.z;
+    *weight     = g;
+    return 1;
+}

@@ -212,6 +212,38 @@ static inline V F_(schlick)(R u, V f0)
     return F_(mk)(f0.x + ((R)1 - f0.x) * f, f0.y + ((R)1 - f0.y) * f, f0.z + ((R)1 - f0.z) * f);
 }
 
+/* GGX visible-normal sampling (Heitz 2018, "Sampling the GGX Distribution of Visible
+ * Normals") in the local frame with the normal on +z.  Returns 0 when the sampled
+ * direction is below the surface; else the reflected direction, the weight G2/G1 of
+ * the estimator f*cos/pdf (before Fresnel) and v.h. */
+static int F_(ggx_sample)(V lv, R a, R u1, R u2, V *ll_out, R *g_out, R *voh_out)
+{
+    const V vh    = F_(norm)(F_(mk)(a * lv.x, a * lv.y, lv.z));
+    const R lensq = vh.x * vh.x + vh.y * vh.y;
+    const V t1v   = lensq > (R)0 ? F_(scl)(F_(mk)(-vh.y, vh.x, (R)0), (R)1 / SQRT_(lensq)) : F_(mk)(1, 0, 0);
+    const V t2v   = F_(cross)(vh, t1v);
+    const R r     = SQRT_(u1);
+    const R phi   = TAU_R * u2;
+    const R t1    = r * COS_(phi);
+    R       t2    = r * SIN_(phi);
+    const R s     = (R)0.5 * ((R)1 + vh.z);
+    t2            = ((R)1 - s) * SQRT_((R)1 - t1 * t1) + s * t2;
+    R       nz2   = (R)1 - t1 * t1 - t2 * t2;
+    if (nz2 < (R)0) nz2 = (R)0;
+    const V nh  = F_(add)(F_(add)(F_(scl)(t1v, t1), F_(scl)(t2v, t2)), F_(scl)(vh, SQRT_(nz2)));
+    const V ne  = F_(norm)(F_(mk)(a * nh.x, a * nh.y, nh.z > (R)0 ? nh.z : (R)0));
+    const R voh = F_(dot)(lv, ne);
+    const V ll  = F_(sub)(F_(scl)(ne, (R)2 * voh), lv);
+    if (ll.z <= (R)0 || lv.z <= (R)0) return 0;
+    const R a2 = a * a;
+    const R g1 = (R)2 * lv.z / (lv.z + SQRT_(a2 + ((R)1 - a2) * lv.z * lv.z));
+    const R g2 = F_(ggx_vis)(lv.z, ll.z, a) * (R)4 * lv.z * ll.z;
+    *ll_out  = ll;
+    *g_out   = g2 / g1;
+    *voh_out = voh;
+    return 1;
+}
+
 typedef struct {
     int  alpha;
     int  delta;       /* pbr: no NEE on this lobe */
@@ -324,34 +356,16 @@ static void F_(process_hit)(const orc_scene *sc, const orc_settings *st, const F
         V tb, bb;
         F_(onb)(nf, &tb, &bb);
         const V lv = F_(mk)(F_(dot)(wo, tb), F_(dot)(wo, bb), F_(dot)(wo, nf));
-        /* Heitz 2018, visible normals */
-        const V vh    = F_(norm)(F_(mk)(a * lv.x, a * lv.y, lv.z));
-        const R lensq = vh.x * vh.x + vh.y * vh.y;
-        const V t1v   = lensq > (R)0 ? F_(scl)(F_(mk)(-vh.y, vh.x, (R)0), (R)1 / SQRT_(lensq)) : F_(mk)(1, 0, 0);
-        const V t2v   = F_(cross)(vh, t1v);
-        const R r     = SQRT_((R)u[0]);
-        const R phi   = TAU_R * (R)u[1];
-        const R t1    = r * COS_(phi);
-        R       t2    = r * SIN_(phi);
-        const R s     = (R)0.5 * ((R)1 + vh.z);
-        t2            = ((R)1 - s) * SQRT_((R)1 - t1 * t1) + s * t2;
-        R       nz2   = (R)1 - t1 * t1 - t2 * t2;
-        if (nz2 < (R)0) nz2 = (R)0;
-        const V nh = F_(add)(F_(add)(F_(scl)(t1v, t1), F_(scl)(t2v, t2)), F_(scl)(vh, SQRT_(nz2)));
-        const V ne = F_(norm)(F_(mk)(a * nh.x, a * nh.y, nh.z > (R)0 ? nh.z : (R)0));
-        const R voh = F_(dot)(lv, ne);
-        const V ll  = F_(sub)(F_(scl)(ne, (R)2 * voh), lv);
-        if (ll.z <= (R)0 || lv.z <= (R)0) {
+        V       ll;
+        R       g, voh;
+        if (!F_(ggx_sample)(lv, a, (R)u[0], (R)u[1], &ll, &g, &voh)) {
             out->absorbed = 1;
             out->albedo   = F_(mk)(0, 0, 0);
             out->d        = nf;
             break;
         }
-        out->d       = F_(norm)(F_(add)(F_(add)(F_(scl)(tb, ll.x), F_(scl)(bb, ll.y)), F_(scl)(nf, ll.z)));
-        const R a2   = a * a;
-        const R g1   = (R)2 * lv.z / (lv.z + SQRT_(a2 + ((R)1 - a2) * lv.z * lv.z));
-        const R g2   = F_(ggx_vis)(lv.z, ll.z, a) * (R)4 * lv.z * ll.z;
-        out->albedo  = F_(scl)(F_(schlick)(voh, f0), g2 / g1);
+        out->d      = F_(norm)(F_(add)(F_(add)(F_(scl)(tb, ll.x), F_(scl)(bb, ll.y)), F_(scl)(nf, ll.z)));
+        out->albedo = F_(scl)(F_(schlick)(voh, f0), g);
         break;
     }
     default: {

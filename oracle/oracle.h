@@ -177,6 +177,14 @@ void orc_get_uv(const orc_image *img, float u, float v, float out[4]);
 /* Four uniforms in [0,1) for counter {pixel, sample, bounce, block}, key {seed, ORC_KEY_HI}. */
 void orc_rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t block, float u[4]);
 
+/* GGX in FP64: D (sampling.h:93-99), height-correlated visibility (sampling.h:109-117) and
+ * visible-normal sampling of the reflected direction in the local frame (normal = +z);
+ * weight = G2/G1, the f*cos/pdf of the sample without Fresnel.  Returns 0 below the
+ * surface. */
+double orc_ggx_d(double noh, double alpha);
+double orc_ggx_vis(double nov, double nol, double alpha);
+int    orc_ggx_sample(const double wo_local[3], double alpha, double u1, double u2, double wi_local[3], double *weight);
+
 /* Display value of one channel: pow(clamp(sum/n, 0, 1), 1/2.2) (renderer.cpp:371-383). */
 float orc_display(float sum, uint64_t n);
 
