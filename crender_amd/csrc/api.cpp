@@ -254,12 +254,12 @@ int alloc_renderer_buffers(crt_renderer *r)
     RA(&b.aov_dep, P);
     RA(&r->display, P);
     RA(&b.cnt, (size_t)kCntWords);
-    RA(&b.stats, (size_t)4);
+    RA(&b.stats, (size_t)4 * kStatSlots);
 #undef RA
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * P, s));
     HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
-    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4, s));
+    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
     // image.h:30-33: new images are filled with FLT_MAX
     HIP_CHECK(launch_fill(b.aov_alb, P, 3.402823466e+38f, s));
     HIP_CHECK(launch_fill(b.aov_nrm, P, 3.402823466e+38f, s));
@@ -418,7 +418,7 @@ int reset_accum(crt_renderer *r)
 {
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.P, s));
-    HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4, s));
+    HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
     HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipStreamSynchronize(s));
     r->samples.store(0);
@@ -1249,8 +1249,11 @@ int crt_renderer_device_buffer(crt_renderer *r, int buffer, void **dev_ptr)
 int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
 {
     if (!r || !stats) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    std::vector<unsigned long long> slots(4 * kStatSlots);
+    HIP_CHECK(hipMemcpy(slots.data(), r->b.stats, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
     unsigned long long s[4] = {0, 0, 0, 0};
-    HIP_CHECK(hipMemcpy(s, r->b.stats, sizeof s, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < kStatSlots; ++i)
+        for (int k = 0; k < 4; ++k) s[k] += slots[4 * i + k];
     std::memset(stats, 0, sizeof *stats);
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - r->t0).count();
     stats->running_time       = t;

@@ -171,8 +171,9 @@ struct PathBuffers {
     float    *raw;              // W*H*3 running sum, flipped as renderer.cpp:358-365
     float4   *aov_alb, *aov_nrm, *aov_dep;
     uint32_t *cnt;              // see Cnt below
-    unsigned long long *stats;  // [0] extension rays, [1] shadow rays, [2] reference segments
+    unsigned long long *stats;  // [kStatSlots][4]: extension rays, shadow rays, reference segments (summed on read)
 };
+constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
 enum : uint32_t {
     kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4,

@@ -24,24 +24,42 @@ constexpr float kInf = __builtin_huge_valf();
 
 __device__ __forceinline__ uint32_t lane() { return __lane_id(); }
 
-// Append to a queue with one atomic per wave: returns this lane's slot.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
+// Append to a queue with one atomic per 256-thread block: returns this lane's slot.
+// Every thread of the block must call it (it contains barriers).
+__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool want)
 {
+    __shared__ uint32_t wcnt[4];
+    __shared__ uint32_t base;
     const uint64_t m = __ballot(want);
-    if (m == 0) return 0;
-    const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t       base   = 0;
-    if (lane() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane()) - 1ull));
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane() == 0) wcnt[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        base = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = base;
+    for (uint32_t j = 0; j < w; ++j) off += wcnt[j];
+    const uint32_t slot = off + (uint32_t)__popcll(m & ((1ull << lane()) - 1ull));
+    __syncthreads();   // wcnt / base reused by the next call
+    return slot;
 }
 
-__device__ __forceinline__ void wave_add64(unsigned long long *p, uint32_t v)
+// Block reduction of a per-thread count, added to one of kStatSlots striped slots so
+// that concurrent blocks rarely meet on the same address.  Every thread must call it.
+__device__ __forceinline__ void block_add_stat(unsigned long long *stats, uint32_t which, uint32_t v)
 {
-    // wave reduction then one atomic
+    __shared__ uint32_t wsum[4];
     uint32_t s = v;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane() == 0 && s) atomicAdd(p, (unsigned long long)s);
+    if (lane() == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (tot) atomicAdd(&stats[(blockIdx.x % kStatSlots) * 4 + which], (unsigned long long)tot);
+    }
+    __syncthreads();
 }
 
 // image.h:104-109 nearest texel with wrap-around
@@ -121,7 +139,7 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
         if (p == 0) b.cnt[kCntQ0] = rp.max_bounces > 0 ? b.P : 0u;
     }
     // every path starts at total_bounces = 1 (renderer.cpp:271)
-    wave_add64(&b.stats[2], live ? 1u : 0u);
+    block_add_stat(b.stats, 2, live ? 1u : 0u);
 }
 
 // Before bounce i: clear the next queue, the shadow queue and the traversal work counters;
@@ -390,18 +408,18 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
         b.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
         b.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
     }
-    const uint32_t ns = wave_append(&b.cnt[kCntShadow], push_shadow);
+    const uint32_t ns = block_append(&b.cnt[kCntShadow], push_shadow);
     if (push_shadow) {
         put_ray(b.sq, ns, so, sd);
         b.spath[ns]    = p;
         b.scontrib[ns] = make_float4(scontrib.x, scontrib.y, scontrib.z, 0.0f);
     }
-    const uint32_t nn = wave_append(&b.cnt[cur ^ 1u], push_next);
+    const uint32_t nn = block_append(&b.cnt[cur ^ 1u], push_next);
     if (push_next) {
         put_ray(b.q[cur ^ 1u], nn, no, nd);
         b.qpath[cur ^ 1u][nn] = p;
     }
-    wave_add64(&b.stats[2], seg);
+    block_add_stat(b.stats, 2, seg);
 }
 
 __global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
@@ -452,7 +470,7 @@ __global__ __launch_bounds__(256) void k_shadow_alpha(SceneDev sc, PathBuffers b
         }
     }
     // continuation rays go to a staging count, then the host swaps them in
-    const uint32_t slot = wave_append(&b.cnt[kCntShadow2], cont);
+    const uint32_t slot = block_append(&b.cnt[kCntShadow2], cont);
     if (cont) {
         // staged after the live entries; k_shadow_compact moves them to the front
         const uint32_t dst = n + slot;

@@ -135,6 +135,11 @@ struct NodeOps<4> {
 
 template <>
 struct NodeOps<8> {
+    // Plane distances in FMA form: t = q * (2^e * inv) + (origin - o) * inv, one rounding per
+    // plane instead of decoding the plane first.  The rounding of the per-node base and of
+    // the FMA is bounded by delta = 2^-20 * (|base| + 255 |2^e inv|) per axis; subtracting
+    // it from the base of the near plane and adding it to the far plane keeps every child
+    // interval a superset of the exact one, at no per-child cost.
     static __device__ __forceinline__ int expand(const void *nodes, uint32_t idx, const RayState &r, uint32_t (&key)[8],
                                                  uint32_t (&ref)[8])
     {
@@ -144,10 +149,20 @@ struct NodeOps<8> {
         const uint4  qx = p[3], qy = p[4], qz = p[5];
         ref[0] = c0.x; ref[1] = c0.y; ref[2] = c0.z; ref[3] = c0.w;
         ref[4] = c1.x; ref[5] = c1.y; ref[6] = c1.z; ref[7] = c1.w;
-        const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
-        const float sx = __uint_as_float((h.w & 0xFFu) << 23);
-        const float sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23);
-        const float sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23);
+        const float sx = __uint_as_float((h.w & 0xFFu) << 23) * r.inv.x;
+        const float sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23) * r.inv.y;
+        const float sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23) * r.inv.z;
+        const float bx = (__uint_as_float(h.x) - r.o.x) * r.inv.x;
+        const float by = (__uint_as_float(h.y) - r.o.y) * r.inv.y;
+        const float bz = (__uint_as_float(h.z) - r.o.z) * r.inv.z;
+        const float dx = 9.5367431640625e-07f * (fabsf(bx) + 255.0f * fabsf(sx));
+        const float dy = 9.5367431640625e-07f * (fabsf(by) + 255.0f * fabsf(sy));
+        const float dz = 9.5367431640625e-07f * (fabsf(bz) + 255.0f * fabsf(sz));
+        // lo plane is the near one when inv > 0
+        const float blx = r.inv.x >= 0.0f ? bx - dx : bx + dx, bhx = r.inv.x >= 0.0f ? bx + dx : bx - dx;
+        const float bly = r.inv.y >= 0.0f ? by - dy : by + dy, bhy = r.inv.y >= 0.0f ? by + dy : by - dy;
+        const float blz = r.inv.z >= 0.0f ? bz - dz : bz + dz, bhz = r.inv.z >= 0.0f ? bz + dz : bz - dz;
+        const float tn0 = fmaxf(r.tn, 0.0f);
         const uint32_t nc = h.w >> 24;
         int nh = 0;
 #pragma unroll
@@ -156,16 +171,16 @@ struct NodeOps<8> {
             const uint32_t wlo_x = j < 4 ? qx.x : qx.y, whi_x = j < 4 ? qx.z : qx.w;
             const uint32_t wlo_y = j < 4 ? qy.x : qy.y, whi_y = j < 4 ? qy.z : qy.w;
             const uint32_t wlo_z = j < 4 ? qz.x : qz.y, whi_z = j < 4 ? qz.z : qz.w;
-            // lo = fl(origin + q*2^e): q*2^e is exact, one rounding, monotone -> conservative
-            const float lx = fmaf((float)((wlo_x >> sh) & 0xFFu), sx, ox);
-            const float hx = fmaf((float)((whi_x >> sh) & 0xFFu), sx, ox);
-            const float ly = fmaf((float)((wlo_y >> sh) & 0xFFu), sy, oy);
-            const float hy = fmaf((float)((whi_y >> sh) & 0xFFu), sy, oy);
-            const float lz = fmaf((float)((wlo_z >> sh) & 0xFFu), sz, oz);
-            const float hz = fmaf((float)((whi_z >> sh) & 0xFFu), sz, oz);
-            float t;
-            const bool hit = (uint32_t)j < nc && slab(lx, hx, ly, hy, lz, hz, r, t);
-            key[j] = hit ? make_key(t, j) : kMissKey;
+            const float t0x = fmaf((float)((wlo_x >> sh) & 0xFFu), sx, blx);
+            const float t1x = fmaf((float)((whi_x >> sh) & 0xFFu), sx, bhx);
+            const float t0y = fmaf((float)((wlo_y >> sh) & 0xFFu), sy, bly);
+            const float t1y = fmaf((float)((whi_y >> sh) & 0xFFu), sy, bhy);
+            const float t0z = fmaf((float)((wlo_z >> sh) & 0xFFu), sz, blz);
+            const float t1z = fmaf((float)((whi_z >> sh) & 0xFFu), sz, bhz);
+            const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tn0));
+            const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tmax));
+            const bool hit = (uint32_t)j < nc && tmin <= tmax;
+            key[j] = hit ? make_key(tmin, j) : kMissKey;
             nh += hit ? 1 : 0;
         }
         return nh;
@@ -302,9 +317,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
             if (exhausted) break;
             continue;
         }
-        // ---- one traversal step per active lane ----
-        if (active && cur != kEmpty) {
-            if (!ref_is_leaf(cur)) {
+        // ---- inner-node phase: lanes expand nodes until every lane holds a leaf or is done ----
+        for (;;) {
+            const bool inner = active && cur != kEmpty && !ref_is_leaf(cur);
+            if (__ballot(inner) == 0) break;
+            if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(sc.nodes, cur, r, key, ref);
                 if (nh == 0) {
@@ -317,7 +334,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                         if (key[k] != kMissKey) st.push(ref[k], __uint_as_float(key[k] & ~7u));
                     }
                 }
-            } else {
+            }
+        }
+        // ---- leaf phase ----
+        for (;;) {
+            const bool leaf = active && cur != kEmpty && ref_is_leaf(cur);
+            if (__ballot(leaf) == 0) break;
+            if (leaf) {
                 const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
                 bool any = false;
                 for (uint32_t k = 0; k < cnt; ++k) {
@@ -329,7 +352,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                     if (wt_test(r.wt, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r.o, r.tn, r.tmax,
                                 t, u, v)) {
                         const uint32_t prim = __float_as_uint(a.w);
-                        if (t < r.tmax || prim < r.best_prim || r.best_tri == kInvalidRef) {
+                        if (t < r.tmax || prim < r.best_prim) {
                             r.tmax      = t;
                             r.best_tri  = (uint32_t)ti;
                             r.best_prim = prim;

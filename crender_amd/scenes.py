@@ -264,3 +264,101 @@ def sunlit(seed: int = 4, mode: int = 0) -> SceneSpec:
     return SceneSpec(name=f"sunlit-{seed}", vertices=verts, vertex_indices=idx, material_indices=mi,
                      materials=materials, sky=constant_sky(0.4), sun_enabled=True, sun=None, camera=cam,
                      res=(96, 64), spp=4, max_bounces=6, rr_start=3, mode=mode)
+
+
+def _value_noise(x, z, freq, st: Stream, octave: int):
+    """Value noise on a (freq+1)^2 lattice of Philox uniforms in [-1, 1], smoothstep-blended."""
+    n = freq + 1
+    lat = st.uniform(n * n, block=octave, dims=1)[:, 0].astype(np.float64).reshape(n, n) * 2.0 - 1.0
+    gx, gz = x * freq, z * freq
+    i0 = np.clip(np.floor(gx).astype(np.int64), 0, freq - 1)
+    j0 = np.clip(np.floor(gz).astype(np.int64), 0, freq - 1)
+    fx, fz = gx - i0, gz - j0
+    sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+    a = lat[i0, j0] + (lat[i0 + 1, j0] - lat[i0, j0]) * sx
+    b = lat[i0, j0 + 1] + (lat[i0 + 1, j0 + 1] - lat[i0, j0 + 1]) * sx
+    return a + (b - a) * sz
+
+
+def fbm(x, z, st: Stream, octaves=8, lacunarity=2.0, gain=0.5, base=4):
+    total, amp, norm = 0.0, 1.0, 0.0
+    for o in range(octaves):
+        total = total + amp * _value_noise(x, z, int(base * lacunarity ** o), st, o)
+        norm += amp
+        amp *= gain
+    return total / norm
+
+
+def icosphere(subdiv: int = 2):
+    """Unit icosphere: 20 * 4^subdiv outward-facing triangles."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(verts, np.float64), np.array(f, np.int64)
+
+
+def terrain(grid: int = 2048, spheres: int = 20_000, seed: int = 3, res=(1920, 1080)) -> SceneSpec:
+    """fBm terrain (8 octaves, amplitude 0.2, lacunarity 2, gain 0.5) on a grid x grid
+    vertex lattice over [0,1]^2, plus icospheres (subdivision 2, 320 triangles) with radius
+    log-uniform in [0.002, 0.05] resting on it.  Materials: terrain diffuse; spheres 70%
+    diffuse, 20% GGX metal (roughness U[0.05, 0.5]), 10% dielectric (IOR 1.5).  Sun and a
+    constant sky; pbr estimator."""
+    st = Stream(seed, stream=5)
+    g = np.linspace(0.0, 1.0, grid)
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    Hh = 0.2 * fbm(X, Z, st)
+    tv = np.stack([X, Hh, Z], -1).reshape(-1, 3)
+    i = np.arange(grid - 1)
+    I, J = np.meshgrid(i, i, indexing="ij")
+    a = (I * grid + J).reshape(-1)
+    b = a + grid
+    c = a + 1
+    d = b + 1
+    # two triangles per cell with +y normals: (a, c, b) and (c, d, b)
+    tfaces = np.stack([np.stack([a, c, b], 1), np.stack([c, d, b], 1)], 1).reshape(-1, 3)
+    sv, sf = icosphere(2)
+    u = Stream(seed, stream=6).uniform(spheres, dims=8).astype(np.float64)
+    rad = np.exp(np.log(0.002) + u[:, 0] * (np.log(0.05) - np.log(0.002)))
+    px, pz = 0.05 + 0.9 * u[:, 1], 0.05 + 0.9 * u[:, 2]
+    py = 0.2 * fbm(px, pz, st) + rad
+    centres = np.stack([px, py, pz], 1)
+    sverts = (sv[None, :, :] * rad[:, None, None] + centres[:, None, :]).reshape(-1, 3)
+    sfaces = (sf[None, :, :] + (np.arange(spheres) * len(sv))[:, None, None]).reshape(-1, 3) + len(tv)
+    verts = np.concatenate([tv, sverts]).astype(np.float32)
+    faces = np.concatenate([tfaces, sfaces]).astype(np.uint32)
+    mats = [MaterialSpec(colour=(0.45, 0.42, 0.35, 1.0))]
+    kind = u[:, 3]
+    for k in range(spheres):
+        col = (0.2 + 0.7 * u[k, 4], 0.2 + 0.7 * u[k, 5], 0.2 + 0.7 * u[k, 6], 1.0)
+        if kind[k] < 0.7:
+            mats.append(MaterialSpec(colour=col))
+        elif kind[k] < 0.9:
+            mats.append(MaterialSpec(type=METAL, roughness=0.05 + 0.45 * u[k, 7], colour=col))
+        else:
+            mats.append(MaterialSpec(type=GLASS, ior=1.5, colour=(0.95, 0.95, 0.95, 1.0)))
+    mi = np.concatenate([np.zeros(len(tfaces), np.uint32),
+                         np.repeat(np.arange(1, spheres + 1, dtype=np.uint32), len(sf))])
+    sky = np.array([[[0.45, 0.55, 0.75, 1.0]]], np.float32)
+    cam = camera_dict((0.5, 0.32, -0.25), rotation=(0.0, -18.0, 0.0), fov=55.0)
+    return SceneSpec(name=f"terrain-{grid}-{spheres}-{seed}", vertices=verts, vertex_indices=faces.reshape(-1),
+                     material_indices=mi, materials=mats, sky=sky, sun_enabled=True, sun=None, camera=cam,
+                     res=tuple(res), spp=64, max_bounces=8, rr_start=3, mode=1)

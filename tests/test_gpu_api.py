@@ -197,3 +197,16 @@ def test_golden_cornell_image():
     r.close()
     sc.close()
     ctx.close()
+
+
+def test_cpp_host_example_runs():
+    """The C++ host mirror (crender_amd/host/crt.hpp) driving the library end to end."""
+    import subprocess
+
+    exe = Path(__file__).resolve().parent.parent / "crender_amd" / "examples" / "render_box"
+    assert exe.exists(), "build() compiles the C++ example"
+    out = subprocess.run([str(exe), "8"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "samples 8," in out.stdout
+    mean = float(out.stdout.strip().split("mean display ")[-1])
+    assert 0.05 < mean < 1.0
