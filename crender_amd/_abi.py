@@ -87,13 +87,23 @@ class Stats(C.Structure):
                 ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("ms_generate", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("ms_render", C.c_double),
-                ("extend_launches", C.c_uint64)]
+                ("extend_launches", C.c_uint64), ("ms_nrc_train", C.c_double), ("nrc_loss", C.c_float)]
 
 
 class BvhInfo(C.Structure):
     _fields_ = [("width", C.c_uint32), ("node_count", C.c_uint32), ("bvh2_node_count", C.c_uint32),
                 ("depth", C.c_uint32), ("max_stack", C.c_uint32), ("sah_cost", C.c_double),
                 ("build_ms", C.c_double), ("bytes", C.c_uint64)]
+
+
+class NrcOptions(C.Structure):
+    _fields_ = [("max_batch", C.c_uint32), ("seed", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class NrcRenderOptions(C.Structure):
+    _fields_ = [("term_bounce", C.c_uint32), ("footprint_c", C.c_float), ("train_paths", C.c_uint32),
+                ("train_suffix", C.c_uint32), ("train_steps", C.c_uint32), ("train", C.c_uint32)]
 
 
 class Rays(C.Structure):
@@ -151,6 +161,17 @@ SIGNATURES = {
     "crt_renderer_checkpoint_size": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "crt_renderer_checkpoint_save": (C.c_int, [P, P, C.c_uint64]),
     "crt_renderer_checkpoint_load": (C.c_int, [P, P, C.c_uint64]),
+    "crt_nrc_options_default": (None, [C.POINTER(NrcOptions)]),
+    "crt_nrc_create": (C.c_int, [P, C.POINTER(NrcOptions), C.POINTER(P)]),
+    "crt_nrc_destroy": (C.c_int, [P]),
+    "crt_nrc_param_count": (C.c_int, []),
+    "crt_nrc_get_params": (C.c_int, [P, P]),
+    "crt_nrc_set_params": (C.c_int, [P, P]),
+    "crt_nrc_infer": (C.c_int, [P, P, C.c_uint32, P, C.c_int]),
+    "crt_nrc_gradient": (C.c_int, [P, P, P, C.c_uint32, P, C.POINTER(C.c_float), C.c_int]),
+    "crt_nrc_train_step": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
+    "crt_nrc_render_options_default": (None, [C.POINTER(NrcRenderOptions)]),
+    "crt_renderer_set_nrc": (C.c_int, [P, P, C.POINTER(NrcRenderOptions)]),
     "crt_camera_default": (None, [C.POINTER(Camera)]),
     "crt_camera_update": (None, [C.POINTER(Camera)]),
     "crt_sun_default": (None, [C.POINTER(Sun)]),

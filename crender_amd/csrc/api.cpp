@@ -20,6 +20,7 @@
 
 #include "../../include/crt.h"
 #include "crt_internal.h"
+#include "nrc.h"
 
 using namespace crt;
 
@@ -121,6 +122,20 @@ struct crt_scene {
     uint32_t               users = 0;  // renderers referencing this scene
 };
 
+struct crt_nrc {
+    crt_context    *ctx = nullptr;
+    crt_nrc_options opt{};
+    float          *params = nullptr, *m = nullptr, *v = nullptr;
+    __bf16         *W = nullptr, *WT = nullptr;
+    __bf16         *act = nullptr;
+    float          *partial = nullptr, *partial_loss = nullptr, *grad = nullptr;
+    float          *q_stage = nullptr, *t_stage = nullptr, *o_stage = nullptr;
+    uint32_t        stage_cap = 0;
+    uint32_t        tiles_alloc = 0;
+    uint64_t        step = 0;
+    std::mutex      mu;
+};
+
 struct crt_renderer {
     crt_context *ctx = nullptr;
     crt_scene   *scene = nullptr;
@@ -149,6 +164,14 @@ struct crt_renderer {
     hipEvent_t ev[64] = {};
     double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
     uint64_t extend_launches = 0;
+    uint32_t ms_phase_pending = 0;
+    // neural radiance cache
+    crt_nrc               *nrc = nullptr;
+    crt_nrc_render_options nopt{};
+    PathBuffers            tb{};            // training paths
+    std::vector<void *>    nrc_allocs;
+    double                 ms_nrc_train = 0.0;
+    float                  last_train_loss = 0.0f;
 };
 
 namespace {
@@ -202,20 +225,69 @@ void free_renderer_buffers(crt_renderer *r)
 }
 
 template <class T>
-int ralloc(crt_renderer *r, T **p, size_t count)
+int lalloc(std::vector<void *> &list, T **p, size_t count)
 {
     HIP_CHECK(dalloc(p, count));
-    r->allocs.push_back(*p);
+    list.push_back(*p);
     return CRT_OK;
 }
 
-int alloc_rays(crt_renderer *r, RaySoA &q, size_t n)
+template <class T>
+int ralloc(crt_renderer *r, T **p, size_t count)
+{
+    return lalloc(r->allocs, p, count);
+}
+
+int alloc_rays_list(std::vector<void *> &list, RaySoA &q, size_t n)
 {
     float **f[8] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.tn, &q.tf};
     for (float **p : f) {
-        const int st = ralloc(r, p, n);
+        const int st = lalloc(list, p, n);
         if (st) return st;
     }
+    return CRT_OK;
+}
+
+int alloc_rays(crt_renderer *r, RaySoA &q, size_t n) { return alloc_rays_list(r->allocs, q, n); }
+
+// Queue, hit, path-state and cache-query buffers of P paths (no image buffers).
+int alloc_path_core(std::vector<void *> &L, PathBuffers &b, size_t P)
+{
+    int st;
+#define LA(ptr, n)                               \
+    if ((st = lalloc(L, ptr, n)) != CRT_OK) return st
+    if ((st = alloc_rays_list(L, b.q[0], P))) return st;
+    if ((st = alloc_rays_list(L, b.q[1], P))) return st;
+    if ((st = alloc_rays_list(L, b.sq, 2 * P))) return st;
+    LA(&b.qpath[0], P);
+    LA(&b.qpath[1], P);
+    LA(&b.hits.t, P);
+    LA(&b.hits.u, P);
+    LA(&b.hits.v, P);
+    LA(&b.hits.tri, P);
+    LA(&b.shits.t, P);
+    LA(&b.shits.u, P);
+    LA(&b.shits.v, P);
+    LA(&b.shits.tri, P);
+    LA(&b.spath, 2 * P);
+    LA(&b.scontrib, 2 * P);
+    LA(&b.occ, P);
+    LA(&b.thr, P);
+    LA(&b.rad, P);
+    LA(&b.cnt, (size_t)kCntWords);
+    LA(&b.stats, (size_t)4 * kStatSlots);
+#undef LA
+    return CRT_OK;
+}
+
+int alloc_nrc_queries(std::vector<void *> &L, PathBuffers &b, size_t P)
+{
+    int st;
+    if ((st = lalloc(L, &b.nstate, P))) return st;
+    if ((st = lalloc(L, &b.nq, P * 13))) return st;
+    if ((st = lalloc(L, &b.nq_path, P))) return st;
+    if ((st = lalloc(L, &b.nq_thr, P))) return st;
+    if ((st = lalloc(L, &b.nq_out, P * 3))) return st;
     return CRT_OK;
 }
 
@@ -256,6 +328,7 @@ int alloc_renderer_buffers(crt_renderer *r)
     RA(&b.cnt, (size_t)kCntWords);
     RA(&b.stats, (size_t)4 * kStatSlots);
 #undef RA
+    if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * P, s));
     HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
@@ -300,31 +373,19 @@ struct PhaseTimer {
     }
 };
 
-// One sample pass: renderer.cpp:121-125 (one barrier per pass) as a kernel sequence.
-// With CRT_FLAG_PHASE_TIMING, HIP events are recorded on the context stream around each
-// kernel and read after the pass's final synchronisation, so timing adds no host waits.
-int render_pass(crt_renderer *r)
+// One set of paths: generate, then per bounce extend / shade / shadow.  With timing on,
+// HIP events are recorded around the kernels and read after the pass's final sync.
+int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, bool timing)
 {
-    crt_context       *ctx = r->ctx;
-    const SceneDev    &sc  = r->scene->dev;
-    const PathBuffers &b   = r->b;
-    hipStream_t        s   = ctx->stream;
-    RenderParams       rp;
-    rp.cam         = to_camera_dev(r->scene->cam);
-    rp.aspect      = r->aspect;
-    rp.max_bounces = r->bounces;
-    rp.rr_start    = r->opt.rr_start;
-    rp.mode        = r->opt.mode;
-    rp.seed        = r->opt.seed;
-    rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
-    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0;
+    crt_context    *ctx = r->ctx;
+    const SceneDev &sc  = r->scene->dev;
+    hipStream_t     s   = ctx->stream;
     auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(r->ev[i], s) : hipSuccess; };
-    HIP_CHECK(hipEventRecord(r->ev[0], s));
     HIP_CHECK(launch_generate(sc, b, rp, s));
     HIP_CHECK(mark(1));
     uint32_t cur = 0;
-    const uint32_t timed = std::min<uint32_t>(r->bounces, crt_renderer::kTimedBounces);
-    for (uint32_t i = 0; i < r->bounces; ++i) {
+    const uint32_t timed = std::min<uint32_t>(rp.max_bounces, crt_renderer::kTimedBounces);
+    for (uint32_t i = 0; i < rp.max_bounces; ++i) {
         const bool t = timing && i < timed;
         HIP_CHECK(launch_bounce_prep(b, cur, s));
         if (t) HIP_CHECK(mark(2 + 4 * (int)i));
@@ -355,7 +416,55 @@ int render_pass(crt_renderer *r)
         if (t) HIP_CHECK(mark(5 + 4 * (int)i));
         cur ^= 1u;
     }
-    HIP_CHECK(mark(crt_renderer::kEvAcc));
+    if (timing) {
+        r->extend_launches += timed;
+        r->ms_phase_pending = timed;
+    }
+    return CRT_OK;
+}
+
+NrcParams nrc_params(const crt_renderer *r)
+{
+    NrcParams np{};
+    const SceneDev &sc = r->scene->dev;
+    np.bmin = sc.bmin;
+    const f3 ext = sc.bmax - sc.bmin;
+    np.binv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
+    if (r->nrc) {
+        np.enabled     = 1;
+        np.term        = r->nopt.term_bounce;
+        np.footprint_c = r->nopt.footprint_c;
+    }
+    return np;
+}
+
+int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render);
+
+// One sample pass: renderer.cpp:121-125 (one barrier per pass) as a kernel sequence.
+int render_pass(crt_renderer *r)
+{
+    crt_context       *ctx = r->ctx;
+    const PathBuffers &b   = r->b;
+    hipStream_t        s   = ctx->stream;
+    RenderParams       rp;
+    rp.cam         = to_camera_dev(r->scene->cam);
+    rp.aspect      = r->aspect;
+    rp.max_bounces = r->bounces;
+    rp.rr_start    = r->opt.rr_start;
+    rp.mode        = r->opt.mode;
+    rp.seed        = r->opt.seed;
+    rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
+    rp.nrc         = nrc_params(r);
+    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0;
+    HIP_CHECK(hipEventRecord(r->ev[0], s));
+    r->ms_phase_pending = 0;
+    int st = run_paths(r, b, rp, timing);
+    if (st) return st;
+    if (r->nrc) {
+        HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
+        HIP_CHECK(launch_nrc_apply(b, s));
+    }
+    if (timing) HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvAcc], s));
     HIP_CHECK(launch_accumulate(b, s));
     HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvEnd], s));
     HIP_CHECK(hipEventSynchronize(r->ev[crt_renderer::kEvEnd]));
@@ -365,7 +474,7 @@ int render_pass(crt_renderer *r)
     if (timing) {
         HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
         r->ms_phase[0] += ms;
-        for (uint32_t i = 0; i < timed; ++i) {
+        for (uint32_t i = 0; i < r->ms_phase_pending; ++i) {
             const int k = 2 + 4 * (int)i;
             HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k], r->ev[k + 1]));
             r->ms_phase[1] += ms;
@@ -374,14 +483,70 @@ int render_pass(crt_renderer *r)
             HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k + 2], r->ev[k + 3]));
             r->ms_phase[3] += ms;
         }
-        r->extend_launches += timed;
         HIP_CHECK(hipEventElapsedTime(&ms, r->ev[crt_renderer::kEvAcc], r->ev[crt_renderer::kEvEnd]));
         r->ms_phase[4] += ms;
     }
     uint32_t err = 0;
     HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
     if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    if (r->nrc && r->nopt.train) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((st = nrc_training_pass(r, rp))) return st;
+        r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     r->samples.fetch_add(1);
+    return CRT_OK;
+}
+
+// Training paths of the cache: random pixels, traced train_suffix bounces past the query
+// vertex with every vertex from term_bounce on recorded, then terminated into the cache
+// (self-training).  Targets L_v = (L_path - L_before_v) / T_v train the cache with
+// train_steps Adam steps.
+int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render)
+{
+    crt_nrc           *n = r->nrc;
+    const PathBuffers &t = r->tb;
+    hipStream_t        s = r->ctx->stream;
+    RenderParams       rp = rp_render;
+    const uint32_t     S  = r->nopt.train_suffix;
+    rp.nrc.enabled     = 1;
+    rp.nrc.train       = 1;
+    rp.nrc.record      = 1;
+    rp.nrc.rec_first   = r->nopt.term_bounce;
+    rp.nrc.rec_slots   = S;
+    rp.nrc.term        = r->nopt.term_bounce + S;
+    rp.nrc.footprint_c = 0.0f;
+    rp.max_bounces     = rp.nrc.term + 1;
+    rp.sample          = rp_render.sample | 0x80000000u;
+    int st = run_paths(r, t, rp, false);
+    if (st) return st;
+    HIP_CHECK(nrc_launch_infer_dev(t.nq, t.nq_out, t.cnt + kCntNrcQ, t.P, n->WT, s));
+    HIP_CHECK(launch_nrc_apply(t, s));
+    HIP_CHECK(hipMemsetAsync(t.cnt + kCntTrain, 0, sizeof(uint32_t), s));
+    const uint32_t max_out = t.P * S;
+    HIP_CHECK(launch_nrc_targets(t, S, max_out, s));
+    uint32_t count = 0;
+    HIP_CHECK(hipMemcpyAsync(&count, t.cnt + kCntTrain, sizeof count, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    count = std::min(count, max_out);
+    const uint32_t steps = std::max<uint32_t>(1u, r->nopt.train_steps);
+    const uint32_t batch = std::min(count / steps, n->opt.max_batch);
+    if (batch < 64) return CRT_OK;
+    std::lock_guard<std::mutex> g(n->mu);
+    float loss = 0.0f;
+    for (uint32_t k = 0; k < steps; ++k) {
+        const float *q  = t.tq + (size_t)k * batch * 13;
+        const float *tt = t.tt + (size_t)k * batch * 3;
+        HIP_CHECK(nrc_launch_train(q, tt, batch, n->W, n->WT, n->act, n->partial, n->partial_loss, s));
+        n->step++;
+        HIP_CHECK(nrc_launch_adam(n->partial, (batch + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
+                                  n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
+    }
+    float pl = 0.0f;
+    HIP_CHECK(hipMemcpyAsync(&pl, n->partial_loss, sizeof pl, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    (void)loss;
+    r->last_train_loss = pl;
     return CRT_OK;
 }
 
@@ -424,6 +589,7 @@ int reset_accum(crt_renderer *r)
     r->samples.store(0);
     for (double &m : r->ms_phase) m = 0.0;
     r->extend_launches = 0;
+    r->ms_nrc_train    = 0.0;
     r->t0 = std::chrono::steady_clock::now();
     return CRT_OK;
 }
@@ -891,6 +1057,18 @@ int crt_scene_commit(crt_scene *scene)
     }
     const uint32_t ntri = (uint32_t)mat_idx.size();
     if (corners.size() / 9 >= (1u << 27)) return fail(CRT_ERR_UNSUPPORTED, "more than 2^27 triangles");
+    float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
+    if (!corners.empty()) {
+        for (int k = 0; k < 3; ++k) {
+            bmin[k] = corners[k];
+            bmax[k] = corners[k];
+        }
+        for (size_t i = 0; i < corners.size(); i += 3)
+            for (int k = 0; k < 3; ++k) {
+                bmin[k] = std::min(bmin[k], corners[i + k]);
+                bmax[k] = std::max(bmax[k], corners[i + k]);
+            }
+    }
     hipStream_t s = ctx->stream;
     float *d_corners = nullptr;
     HIP_CHECK(dalloc(&d_corners, corners.size()));
@@ -952,6 +1130,8 @@ int crt_scene_commit(crt_scene *scene)
     d.any_alpha   = any_alpha ? 1 : 0;
     d.model_first = scene->d_model_first;
     d.nmodels     = (uint32_t)model_first.size();
+    d.bmin        = mk3(bmin[0], bmin[1], bmin[2]);
+    d.bmax        = mk3(bmax[0], bmax[1], bmax[2]);
     crt_bvh_info &bi = scene->info;
     bi.width           = width;
     bi.node_count      = br.node_count;
@@ -1050,6 +1230,8 @@ int crt_renderer_destroy(crt_renderer *r)
     join_worker(r);
     (void)hipSetDevice(r->ctx->device);
     free_renderer_buffers(r);
+    for (void *p : r->nrc_allocs) dfree(p);
+    r->nrc_allocs.clear();
     for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
     r->scene->users--;
     delete r;
@@ -1129,6 +1311,8 @@ int crt_renderer_set_resolution(crt_renderer *r, uint32_t x, uint32_t y)
     r->W      = x;
     r->H      = y;
     r->aspect = (float)x / (float)y;   // renderer.cpp:199
+    r->tb.W   = x;
+    r->tb.H   = y;
     if ((st = alloc_renderer_buffers(r))) return st;
     r->samples.store(0);
     return CRT_OK;
@@ -1270,6 +1454,8 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->ms_accumulate = r->ms_phase[4];
     stats->ms_render     = r->ms_phase[5];
     stats->extend_launches = r->extend_launches;
+    stats->ms_nrc_train    = r->ms_nrc_train;
+    stats->nrc_loss        = r->last_train_loss;
     return CRT_OK;
 }
 
@@ -1329,6 +1515,272 @@ int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t byte
     r->opt.seed          = h.seed;
     r->opt.sample_offset = (uint32_t)h.sample_offset;
     r->samples.store(h.samples);
+    return CRT_OK;
+}
+
+}  // extern "C"
+
+// ---- neural radiance cache --------------------------------------------------------------
+
+namespace {
+void free_nrc_paths(crt_renderer *r)
+{
+    for (void *p : r->nrc_allocs) dfree(p);
+    r->nrc_allocs.clear();
+    r->tb = PathBuffers{};
+}
+}  // namespace
+
+extern "C" void crt_nrc_render_options_default(crt_nrc_render_options *o)
+{
+    if (!o) return;
+    o->term_bounce  = 1;
+    o->footprint_c  = 0.01f;
+    o->train_paths  = 65536;
+    o->train_suffix = 4;
+    o->train_steps  = 4;
+    o->train        = 1;
+}
+
+extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt)
+{
+    int st = require_paused(r);
+    if (st) return st;
+    if (nrc && nrc->ctx != r->ctx) return fail(CRT_ERR_INVALID_ARG, "cache belongs to another context");
+    free_nrc_paths(r);
+    r->nrc = nrc;
+    if (!nrc) return CRT_OK;
+    crt_nrc_render_options o;
+    crt_nrc_render_options_default(&o);
+    if (opt) o = *opt;
+    if (o.term_bounce < 1 || o.train_suffix < 1 || o.train_paths < 64 || o.train_paths > (1u << 22))
+        return fail(CRT_ERR_INVALID_ARG, "bad cache render options");
+    r->nopt = o;
+    PathBuffers &t = r->tb;
+    t.W = r->W;
+    t.H = r->H;
+    t.P = o.train_paths;
+    const size_t P = t.P, S = o.train_suffix;
+    auto &L = r->nrc_allocs;
+    if ((st = alloc_path_core(L, t, P))) return st;
+    if ((st = alloc_nrc_queries(L, t, P))) return st;
+    if ((st = lalloc(L, &t.rq, P * S * 13))) return st;
+    if ((st = lalloc(L, &t.r_thr, P * S))) return st;
+    if ((st = lalloc(L, &t.r_rad, P * S))) return st;
+    if ((st = lalloc(L, &t.tq, P * S * 13))) return st;
+    if ((st = lalloc(L, &t.tt, P * S * 3))) return st;
+    HIP_CHECK(hipMemset(t.cnt, 0, sizeof(uint32_t) * kCntWords));
+    HIP_CHECK(hipMemset(t.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots));
+    return CRT_OK;
+}
+
+
+
+
+namespace {
+
+uint32_t nrc_tiles_alloc(uint32_t max_batch) { return ((max_batch + 63) / 64 + 3) / 4 * 4; }
+
+int nrc_stage(crt_nrc *n, uint32_t count)
+{
+    if (count <= n->stage_cap) return CRT_OK;
+    dfree(n->q_stage);
+    dfree(n->t_stage);
+    dfree(n->o_stage);
+    n->q_stage = n->t_stage = n->o_stage = nullptr;
+    HIP_CHECK(dalloc(&n->q_stage, (size_t)count * kNrcQuery));
+    HIP_CHECK(dalloc(&n->t_stage, (size_t)count * 3));
+    HIP_CHECK(dalloc(&n->o_stage, (size_t)count * 3));
+    n->stage_cap = count;
+    return CRT_OK;
+}
+
+// Kaiming-uniform initialisation from Philox, zero for padded inputs and outputs.
+void nrc_init_params(std::vector<float> &w, uint32_t seed)
+{
+    w.assign(kNrcParams, 0.0f);
+    uint32_t ctr = 0;
+    for (int l = 0; l < kNrcLayers; ++l) {
+        const int K = l == 0 ? kNrcIn : kNrcHid, N = l == kNrcLayers - 1 ? kNrcOut : kNrcHid;
+        const int fan_in = l == 0 ? 83 : K;
+        const float bound = std::sqrt((l == kNrcLayers - 1 ? 3.0f : 6.0f) / (float)fan_in);
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < N; ++c) {
+                const u4 r = philox4x32_10(u4{ctr++, 0u, 0xFFFFFFF0u, 0u}, seed, kKeyHi);
+                float x = (2.0f * u01(r.x) - 1.0f) * bound;
+                if ((l == 0 && k >= 83) || (l == kNrcLayers - 1 && c >= 3)) x = 0.0f;
+                w[nrc_w_off(l) + k * N + c] = x;
+            }
+    }
+}
+
+int nrc_upload(crt_nrc *n, const float *host)
+{
+    hipStream_t s = n->ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(n->params, host, sizeof(float) * kNrcParams, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(n->m, 0, sizeof(float) * kNrcParams, s));
+    HIP_CHECK(hipMemsetAsync(n->v, 0, sizeof(float) * kNrcParams, s));
+    HIP_CHECK(nrc_launch_adam(nullptr, 0, n->params, n->m, n->v, n->W, n->WT, 0, 0, 0, 0, 1, 0, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    n->step = 0;
+    return CRT_OK;
+}
+
+const float *nrc_input(crt_nrc *n, const float *p, size_t count, float *stage, int memory, int *st)
+{
+    *st = CRT_OK;
+    if (memory == CRT_MEM_DEVICE) return p;
+    hipError_t e = hipMemcpyAsync(stage, p, sizeof(float) * count, hipMemcpyHostToDevice, n->ctx->stream);
+    if (e != hipSuccess) *st = fail(CRT_ERR_HIP, hipGetErrorString(e));
+    return stage;
+}
+
+}  // namespace
+
+extern "C" {
+
+void crt_nrc_options_default(crt_nrc_options *opt)
+{
+    if (!opt) return;
+    opt->max_batch = 65536;
+    opt->seed      = 1;
+    opt->lr        = 1e-3f;
+    opt->beta1     = 0.9f;
+    opt->beta2     = 0.99f;
+    opt->eps       = 1e-10f;
+}
+
+int crt_nrc_param_count(void) { return kNrcParams; }
+
+int crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out)
+{
+    if (!ctx || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    auto *n = new crt_nrc();
+    n->ctx = ctx;
+    crt_nrc_options_default(&n->opt);
+    if (opt) n->opt = *opt;
+    if (n->opt.max_batch == 0 || n->opt.max_batch > (1u << 24)) {
+        delete n;
+        return fail(CRT_ERR_INVALID_ARG, "max_batch must be in [1, 2^24]");
+    }
+    HIP_CHECK(hipSetDevice(ctx->device));
+    n->tiles_alloc = nrc_tiles_alloc(n->opt.max_batch);
+    int st = CRT_OK;
+    auto A = [&](auto **p, size_t c) {
+        if (st == CRT_OK && dalloc(p, c) != hipSuccess) st = fail(CRT_ERR_OUT_OF_MEMORY, "nrc allocation");
+    };
+    A(&n->params, kNrcParams);
+    A(&n->m, kNrcParams);
+    A(&n->v, kNrcParams);
+    A(&n->W, kNrcParams);
+    A(&n->WT, kNrcParams);
+    A(&n->grad, kNrcParams);
+    A(&n->act, (size_t)n->tiles_alloc * 64 * kNrcActs);
+    A(&n->partial, (size_t)n->tiles_alloc * kNrcParams);
+    A(&n->partial_loss, n->tiles_alloc);
+    if (st == CRT_OK) {
+        std::vector<float> w;
+        nrc_init_params(w, n->opt.seed);
+        st = nrc_upload(n, w.data());
+    }
+    if (st != CRT_OK) {
+        crt_nrc_destroy(n);
+        return st;
+    }
+    *out = n;
+    return CRT_OK;
+}
+
+int crt_nrc_destroy(crt_nrc *n)
+{
+    if (!n) return CRT_OK;
+    void *ps[] = {n->params, n->m, n->v, n->W, n->WT, n->grad, n->act, n->partial, n->partial_loss,
+                  n->q_stage, n->t_stage, n->o_stage};
+    for (void *p : ps) dfree(p);
+    delete n;
+    return CRT_OK;
+}
+
+int crt_nrc_get_params(crt_nrc *n, float *params)
+{
+    if (!n || !params) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    HIP_CHECK(hipMemcpy(params, n->params, sizeof(float) * kNrcParams, hipMemcpyDeviceToHost));
+    return CRT_OK;
+}
+
+int crt_nrc_set_params(crt_nrc *n, const float *params)
+{
+    if (!n || !params) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(n->mu);
+    return nrc_upload(n, params);
+}
+
+int crt_nrc_infer(crt_nrc *n, const float *queries, uint32_t count, float *out, int memory)
+{
+    if (!n || ((!queries || !out) && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (count == 0) return CRT_OK;
+    std::lock_guard<std::mutex> g(n->mu);
+    int st = memory == CRT_MEM_HOST ? nrc_stage(n, count) : CRT_OK;
+    if (st) return st;
+    const float *q = nrc_input(n, queries, (size_t)count * kNrcQuery, n->q_stage, memory, &st);
+    if (st) return st;
+    float *o = memory == CRT_MEM_DEVICE ? out : n->o_stage;
+    hipStream_t s = n->ctx->stream;
+    HIP_CHECK(nrc_launch_infer(q, o, count, n->WT, s));
+    if (memory == CRT_MEM_HOST) HIP_CHECK(hipMemcpyAsync(out, o, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return CRT_OK;
+}
+
+static int nrc_fwd_bwd(crt_nrc *n, const float *queries, const float *targets, uint32_t count, int memory,
+                       float *loss)
+{
+    if (count > n->opt.max_batch) return fail(CRT_ERR_INVALID_ARG, "batch larger than max_batch");
+    int st = memory == CRT_MEM_HOST ? nrc_stage(n, count) : CRT_OK;
+    if (st) return st;
+    const float *q = nrc_input(n, queries, (size_t)count * kNrcQuery, n->q_stage, memory, &st);
+    if (st) return st;
+    const float *t = nrc_input(n, targets, (size_t)count * 3, n->t_stage, memory, &st);
+    if (st) return st;
+    hipStream_t s = n->ctx->stream;
+    HIP_CHECK(nrc_launch_train(q, t, count, n->W, n->WT, n->act, n->partial, n->partial_loss, s));
+    if (loss) {
+        const uint32_t tiles = (count + 63) / 64;
+        std::vector<float> pl(tiles);
+        HIP_CHECK(hipMemcpyAsync(pl.data(), n->partial_loss, sizeof(float) * tiles, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (float x : pl) acc += x;
+        *loss = (float)acc;
+    }
+    return CRT_OK;
+}
+
+int crt_nrc_gradient(crt_nrc *n, const float *queries, const float *targets, uint32_t count, float *grad, float *loss,
+                     int memory)
+{
+    if (!n || !grad || ((!queries || !targets) && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (count == 0) return fail(CRT_ERR_INVALID_ARG, "empty batch");
+    std::lock_guard<std::mutex> g(n->mu);
+    int st = nrc_fwd_bwd(n, queries, targets, count, memory, loss);
+    if (st) return st;
+    HIP_CHECK(nrc_launch_sum_partials(n->partial, (count + 63) / 64, n->grad, n->ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(grad, n->grad, sizeof(float) * kNrcParams, hipMemcpyDeviceToHost, n->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(n->ctx->stream));
+    return CRT_OK;
+}
+
+int crt_nrc_train_step(crt_nrc *n, const float *queries, const float *targets, uint32_t count, float *loss, int memory)
+{
+    if (!n || ((!queries || !targets) && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (count == 0) return CRT_OK;
+    std::lock_guard<std::mutex> g(n->mu);
+    int st = nrc_fwd_bwd(n, queries, targets, count, memory, loss);
+    if (st) return st;
+    n->step++;
+    HIP_CHECK(nrc_launch_adam(n->partial, (count + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
+                              n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, n->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(n->ctx->stream));
     return CRT_OK;
 }
 

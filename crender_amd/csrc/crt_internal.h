@@ -109,6 +109,7 @@ struct SceneDev {
     int32_t            any_alpha;  // some material can be alpha (colour.w == 0 or textured)
     const uint32_t    *model_first; // first primID of each model (for geomID)
     uint32_t           nmodels;
+    f3                 bmin, bmax;  // scene bounding box
 };
 
 // SoA ray buffer
@@ -172,18 +173,39 @@ struct PathBuffers {
     float4   *aov_alb, *aov_nrm, *aov_dep;
     uint32_t *cnt;              // see Cnt below
     unsigned long long *stats;  // [kStatSlots][4]: extension rays, shadow rays, reference segments (summed on read)
+    // neural radiance cache
+    float4   *nstate;           // per path: (a0 footprint, pdf of the last scatter, -, -)
+    float    *nq;               // cache queries [P][13]
+    uint32_t *nq_path;
+    float4   *nq_thr;           // throughput arriving at the query vertex
+    float    *nq_out;           // predictions [P][3]
+    float    *rq;               // training records: queries [P * rec_slots][13]
+    float4   *r_thr, *r_rad;    // throughput and radiance before each recorded vertex
+    float    *tq, *tt;          // compacted training set: queries [n][13], targets [n][3]
 };
 constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
 enum : uint32_t {
-    kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4,
+    kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4, kCntNrcQ = 5, kCntTrain = 6,
     kCntWorkExt = 8, kCntWorkShadow = 16, kCntWords = 32
+};
+
+// Neural radiance cache hooks of the integrator.
+struct NrcParams {
+    int32_t  enabled;       // terminate paths into the cache
+    uint32_t term;          // bounce at which paths query the cache
+    float    footprint_c;   // > 0: query at `term` only if the path spread exceeds c * a0, else at term + 1
+    int32_t  record;        // training paths: record vertices rec_first .. rec_first + rec_slots - 1
+    uint32_t rec_first, rec_slots;
+    int32_t  train;         // training paths: random pixels, no AOVs, no accumulation
+    f3       bmin, binv;    // scene box, for normalising query positions to [0,1]
 };
 
 struct RenderParams {
     camera_dev cam;
     float      aspect;
     uint32_t   max_bounces, rr_start, mode, seed, sample;
+    NrcParams  nrc;
 };
 
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s);
@@ -196,6 +218,10 @@ hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hi
 hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s);
 hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s);
 hipError_t launch_fill(float4 *p, uint64_t n, float v, hipStream_t s);
+hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s);
+hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s);
+hipError_t nrc_launch_infer_dev(const float *q, float *out, const uint32_t *n_dev, uint32_t n_max, const __bf16 *WT,
+                                hipStream_t s);
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
                               float *ngx, float *ngy, float *ngz, hipStream_t s);
 

@@ -126,8 +126,14 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
     const uint32_t p    = blockIdx.x * blockDim.x + threadIdx.x;
     const bool     live = p < b.P;
     if (live) {
-        const uint32_t x = p % b.W, y = p / b.W;
+        // training paths of the radiance cache start at random pixels
+        const uint32_t pix = rp.nrc.train ? philox4x32_10(u4{p, rp.sample, 0xFFFFFFFEu, 0u}, rp.seed, kKeyHi).x % (b.W * b.H)
+                                          : p;
+        const uint32_t x = pix % b.W, y = pix / b.W;
         const rnd4     j = rng4(rp.seed, p, rp.sample, kJitterBounce, 0);
+        if (rp.nrc.record)
+            for (uint32_t r = 0; r < rp.nrc.rec_slots; ++r) b.r_thr[(size_t)p * rp.nrc.rec_slots + r] = make_float4(0, 0, 0, 0);
+        if (rp.nrc.enabled) b.nstate[p] = make_float4(0, 0, 0, 0);
         f3 o, d;
         // renderer.cpp:260-263
         camera_ray(rp.cam, ((float)x + j.a) / (float)b.W, ((float)y + j.b) / (float)b.H, rp.aspect, o, d);
@@ -135,8 +141,11 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
         b.qpath[0][p] = p;
         b.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         b.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (rp.max_bounces == 0) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
-        if (p == 0) b.cnt[kCntQ0] = rp.max_bounces > 0 ? b.P : 0u;
+        if (rp.max_bounces == 0 && !rp.nrc.train) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+        if (p == 0) {
+            b.cnt[kCntQ0]   = rp.max_bounces > 0 ? b.P : 0u;
+            b.cnt[kCntNrcQ] = 0;
+        }
     }
     // every path starts at total_bounces = 1 (renderer.cpp:271)
     block_add_stat(b.stats, 2, live ? 1u : 0u);
@@ -202,6 +211,46 @@ __device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, 
     return h;
 }
 
+// Termination into the cache (Mueller et al. 2021, "Real-time Neural Radiance Caching
+// for Path Tracing"): query at bounce `term`, unless the footprint heuristic is on and the
+// path spread |x_{i-1} - x_i|^2 / (pdf |cos|) is still below c * a0, in which case the
+// query moves to the next bounce.
+__device__ __forceinline__ bool nrc_terminates(const NrcParams &np, const PathBuffers &b, uint32_t p, uint32_t bounce,
+                                               const struct HitRec &h, f3 d)
+{
+    if (bounce < np.term) return false;
+    if (np.footprint_c <= 0.0f || bounce > np.term) return true;
+    const float4 st  = b.nstate[p];
+    const float  pdf = st.y;
+    if (!(pdf > 0.0f)) return false;            // delta lobe: footprint unchanged
+    const float c = fmaxf(fabsf(dot(h.normal, normalize(d))), 1e-4f);
+    const float spread = h.dist * h.dist / (pdf * c);
+    return spread > np.footprint_c * st.x;
+}
+
+// Cache query of a vertex: position normalised to the scene box, outgoing direction,
+// normal facing the incoming ray, albedo, roughness (1 diffuse, alpha for metal, 0 glass).
+__device__ __forceinline__ void nrc_query(const NrcParams &np, f3 point, f3 d, f3 n, f3 cc, const MaterialDev &m,
+                                          float *q)
+{
+    const f3 pn = (point - np.bmin) * np.binv;
+    const f3 wo = -normalize(d);
+    const f3 nf = dot(d, n) > 0.0f ? -n : n;
+    q[0]  = fminf(fmaxf(pn.x, 0.0f), 1.0f);
+    q[1]  = fminf(fmaxf(pn.y, 0.0f), 1.0f);
+    q[2]  = fminf(fmaxf(pn.z, 0.0f), 1.0f);
+    q[3]  = wo.x;
+    q[4]  = wo.y;
+    q[5]  = wo.z;
+    q[6]  = nf.x;
+    q[7]  = nf.y;
+    q[8]  = nf.z;
+    q[9]  = cc.x;
+    q[10] = cc.y;
+    q[11] = cc.z;
+    q[12] = m.type == 1 ? 1.0f : (m.type == 0 ? m.roughness : 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, RenderParams rp, uint32_t bounce,
                                                uint32_t cur)
 {
@@ -209,8 +258,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
     const uint32_t n = b.cnt[cur];
     const bool     live = k < n;
     const RaySoA  &q = b.q[cur];
-    bool     push_next = false, push_shadow = false;
+    bool     push_next = false, push_shadow = false, do_query = false;
     f3       no = mk3(0, 0, 0), nd = mk3(0, 0, 0), so = mk3(0, 0, 0), sd = mk3(0, 0, 0), scontrib = mk3(0, 0, 0);
+    f3       qthr = mk3(0, 0, 0);
+    float    nq[13];
     uint32_t p = 0, seg = 0;
     if (live) {
         p = b.qpath[cur][k];
@@ -222,9 +273,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
         float4         rad4 = b.rad[p];
         f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
         const bool     can_continue = bounce + 1 < rp.max_bounces;
+        qthr = thr;
         if (tri == kInvalidRef) {
             const f3 s = sky(sc, d);
-            if (bounce == 0) aov_write(b, p, s, mk3(0, 0, 0), 0.0f);
+            if (bounce == 0 && !rp.nrc.train) aov_write(b, p, s, mk3(0, 0, 0), 0.0f);
             rad = rad + thr * s;
         } else {
             const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k]);
@@ -233,14 +285,31 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
             seg = 1;
             if (col.w == 0.0f) {
                 // alpha pass-through (renderer.cpp:294-301)
-                if (bounce == 0) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
                 no = h.point + d * 0.1f;
                 nd = d;
                 push_next = can_continue;
+            } else if (rp.nrc.enabled && nrc_terminates(rp.nrc, b, p, bounce, h, d)) {
+                // the cache stands in for this vertex's emission, direct light and everything after it
+                const f3 cc = mk3(col.x, col.y, col.z);
+                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, cc, h.normal, h.dist);
+                nrc_query(rp.nrc, h.point, d, h.normal, cc, m, nq);
+                do_query = true;
             } else {
                 const rnd4 r  = rng4(rp.seed, p, rp.sample, bounce, 0);
                 const f3   n  = h.normal;
                 const f3   cc = mk3(col.x, col.y, col.z);
+                if (rp.nrc.record && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
+                    const size_t slot = (size_t)p * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
+                    nrc_query(rp.nrc, h.point, d, n, cc, m, b.rq + slot * 13);
+                    b.r_thr[slot] = make_float4(thr.x, thr.y, thr.z, 1.0f);
+                    b.r_rad[slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+                }
+                if (rp.nrc.enabled && bounce == 0) {
+                    // primary footprint a0 = |x1 - x0|^2 / (4 pi |cos|)
+                    const float c0 = fabsf(dot(n, normalize(d)));
+                    b.nstate[p].x = h.dist * h.dist / (4.0f * kPi * fmaxf(c0, 1e-4f));
+                }
                 f3   alb = cc, nee_n = n, f0 = mk3(0, 0, 0);
                 bool delta = false, absorbed = false, ggx = false;
                 float ar = 0.0f;
@@ -348,7 +417,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
                         nee_n = nf;
                     }
                 }
-                if (bounce == 0) aov_write(b, p, rp.mode == kModeCompat ? alb : cc, n, h.dist);
+                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, rp.mode == kModeCompat ? alb : cc, n, h.dist);
                 const f3 thr_prev = thr;
                 if (rp.mode == kModeCompat) {
                     thr = thr * alb;
@@ -396,6 +465,23 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
                 } else {
                     push_next = can_continue;
                 }
+                if (rp.nrc.enabled && rp.nrc.footprint_c > 0.0f) {
+                    // pdf of the scattered direction (0 for delta lobes: the footprint does not grow)
+                    float pdf = 0.0f;
+                    if (!delta && !absorbed) {
+                        if (ggx) {
+                            const f3    wo = -normalize(d);
+                            const f3    hv = normalize(wo + nd);
+                            const float nv = fmaxf(dot(nee_n, wo), 1e-4f);
+                            const float a2 = ar * ar;
+                            const float g1 = 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
+                            pdf = ggx_d(dot(nee_n, hv), ar) * g1 / (4.0f * nv);
+                        } else {
+                            pdf = fabsf(dot(rp.mode == kModeCompat ? n : nee_n, nd)) * kInvPi;
+                        }
+                    }
+                    b.nstate[p].y = pdf;
+                }
             }
         }
         // Russian roulette at the top of the next iteration
@@ -407,6 +493,12 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
         }
         b.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
         b.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+    const uint32_t nqs = rp.nrc.enabled ? block_append(&b.cnt[kCntNrcQ], do_query) : 0u;
+    if (do_query) {
+        for (int k = 0; k < 13; ++k) b.nq[(size_t)nqs * 13 + k] = nq[k];
+        b.nq_path[nqs] = p;
+        b.nq_thr[nqs]  = make_float4(qthr.x, qthr.y, qthr.z, 0.0f);
     }
     const uint32_t ns = block_append(&b.cnt[kCntShadow], push_shadow);
     if (push_shadow) {
@@ -574,6 +666,51 @@ __global__ void k_hits_to_api(SceneDev sc, HitSoA h, uint32_t n, uint32_t *prim,
     if (ngz) ngz[i] = ng.z;
 }
 
+// rad[path] += throughput * max(prediction, 0) for every cache query
+__global__ __launch_bounds__(256) void k_nrc_apply(PathBuffers b)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.cnt[kCntNrcQ]) return;
+    const uint32_t p = b.nq_path[k];
+    const float4   t = b.nq_thr[k];
+    float4         r = b.rad[p];
+    r.x += t.x * fmaxf(b.nq_out[3 * (size_t)k + 0], 0.0f);
+    r.y += t.y * fmaxf(b.nq_out[3 * (size_t)k + 1], 0.0f);
+    r.z += t.z * fmaxf(b.nq_out[3 * (size_t)k + 2], 0.0f);
+    b.rad[p] = r;
+}
+
+// Training targets from the recorded vertices: L_v = (L_path - L_before_v) / T_v per
+// channel; valid, finite samples are compacted into (tq, tt).
+__global__ __launch_bounds__(256) void k_nrc_targets(PathBuffers b, uint32_t rec_slots, uint32_t max_out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = i / rec_slots;
+    bool  ok = false;
+    float tg[3] = {0, 0, 0};
+    if (p < b.P) {
+        const float4 t = b.r_thr[i];
+        if (t.w > 0.0f) {
+            const float4 rb = b.rad[p], r0 = b.r_rad[i];
+            const float  thr[3] = {t.x, t.y, t.z}, num[3] = {rb.x - r0.x, rb.y - r0.y, rb.z - r0.z};
+            ok = true;
+            for (int c = 0; c < 3; ++c) {
+                if (!(thr[c] > 1e-6f)) {
+                    ok = false;
+                    break;
+                }
+                tg[c] = fmaxf(num[c] / thr[c], 0.0f);
+                if (!(tg[c] < 1e4f)) ok = false;
+            }
+        }
+    }
+    const uint32_t slot = block_append(&b.cnt[kCntTrain], ok);
+    if (ok && slot < max_out) {
+        for (int k = 0; k < 13; ++k) b.tq[(size_t)slot * 13 + k] = b.rq[(size_t)i * 13 + k];
+        for (int c = 0; c < 3; ++c) b.tt[(size_t)slot * 3 + c] = tg[c];
+    }
+}
+
 }  // namespace
 
 static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -625,6 +762,16 @@ hipError_t launch_fill(float4 *p, uint64_t n, float v, hipStream_t s)
 {
     const uint32_t g = (uint32_t)std::min<uint64_t>(blocks(n), 4096);
     if (n) k_fill<<<g, 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s)
+{
+    k_nrc_apply<<<blocks(b.P), 256, 0, s>>>(b);
+    return hipGetLastError();
+}
+hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s)
+{
+    k_nrc_targets<<<blocks((uint64_t)b.P * rec_slots), 256, 0, s>>>(b, rec_slots, max_out);
     return hipGetLastError();
 }
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,

@@ -301,6 +301,15 @@ class Renderer:
     def render(self, passes: int) -> None:
         check(self._L.crt_renderer_render(self._h, passes))
 
+    def set_nrc(self, nrc, term_bounce=1, footprint_c=0.01, train_paths=65536, train_suffix=4, train_steps=4,
+                train=True) -> None:
+        """Render with the neural radiance cache `nrc` (None turns it off)."""
+        o = A.NrcRenderOptions()
+        self._L.crt_nrc_render_options_default(C.byref(o))
+        o.term_bounce, o.footprint_c, o.train_paths = term_bounce, footprint_c, train_paths
+        o.train_suffix, o.train_steps, o.train = train_suffix, train_steps, 1 if train else 0
+        check(self._L.crt_renderer_set_nrc(self._h, nrc.handle if nrc is not None else None, C.byref(o)))
+
     def reset(self) -> None:
         check(self._L.crt_renderer_reset(self._h))
 
@@ -356,6 +365,68 @@ class Renderer:
     def close(self):
         if self._h:
             self._L.crt_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Nrc:
+    """Neural radiance cache (crt_nrc_*): 13-float queries -> RGB, trained with Adam."""
+
+    def __init__(self, ctx: Context, max_batch: int = 65536, seed: int = 1, lr: float = 1e-3):
+        self._L = A.lib()
+        o = A.NrcOptions()
+        self._L.crt_nrc_options_default(C.byref(o))
+        o.max_batch = max_batch
+        o.seed = seed
+        o.lr = lr
+        h = C.c_void_p()
+        check(self._L.crt_nrc_create(ctx.handle, C.byref(o), C.byref(h)))
+        self._h = h
+        self.n_params = self._L.crt_nrc_param_count()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        check(self._L.crt_nrc_get_params(self._h, vptr(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        check(self._L.crt_nrc_set_params(self._h, vptr(p)))
+
+    def infer(self, queries) -> np.ndarray:
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 13)
+        out = np.empty((q.shape[0], 3), np.float32)
+        check(self._L.crt_nrc_infer(self._h, vptr(q), q.shape[0], vptr(out), A.CRT_MEM_HOST))
+        return out
+
+    def gradient(self, queries, targets):
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 13)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        check(self._L.crt_nrc_gradient(self._h, vptr(q), vptr(t), q.shape[0], vptr(g), C.byref(loss), A.CRT_MEM_HOST))
+        return loss.value, g
+
+    def train_step(self, queries, targets) -> float:
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 13)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        check(self._L.crt_nrc_train_step(self._h, vptr(q), vptr(t), q.shape[0], C.byref(loss), A.CRT_MEM_HOST))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            self._L.crt_nrc_destroy(self._h)
             self._h = None
 
     def __del__(self):

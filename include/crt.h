@@ -144,6 +144,8 @@ typedef struct crt_stats {
     double   ms_generate, ms_extend, ms_shade, ms_shadow, ms_accumulate; /* with CRT_FLAG_PHASE_TIMING */
     double   ms_render;          /* GPU time of all passes (HIP events) */
     uint64_t extend_launches;    /* closest-hit launches covered by ms_extend */
+    double   ms_nrc_train;       /* host time of the cache's training passes */
+    float    nrc_loss;           /* training loss of the last step (one tile) */
 } crt_stats;
 
 /* BVH summary returned by crt_scene_bvh_info */
@@ -250,6 +252,45 @@ int crt_renderer_cancel(crt_renderer *r);
 int crt_renderer_checkpoint_size(crt_renderer *r, uint64_t *bytes);
 int crt_renderer_checkpoint_save(crt_renderer *r, void *buf, uint64_t bytes);
 int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t bytes);
+
+/* ---- neural radiance cache (no counterpart in the reference: the learned part) -------- *
+ * MLP over an encoded query (position normalised to the scene box, direction, normal,
+ * albedo, roughness: 13 floats) -> RGB radiance; bf16 weights, FP32 accumulation, Adam. */
+typedef struct crt_nrc crt_nrc;
+typedef struct crt_nrc_options {
+    uint32_t max_batch;   /* largest training batch per step (default 65536) */
+    uint32_t seed;        /* weight initialisation (Philox) */
+    float    lr, beta1, beta2, eps;   /* Adam (default 1e-3, 0.9, 0.99, 1e-10) */
+} crt_nrc_options;
+void crt_nrc_options_default(crt_nrc_options *opt);
+int  crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out);
+int  crt_nrc_destroy(crt_nrc *nrc);
+int  crt_nrc_param_count(void);
+int  crt_nrc_get_params(crt_nrc *nrc, float *params);          /* FP32 master weights, host */
+int  crt_nrc_set_params(crt_nrc *nrc, const float *params);    /* also resets the Adam state */
+/* queries: n x 13 floats; out: n x 3 floats */
+int  crt_nrc_infer(crt_nrc *nrc, const float *queries, uint32_t n, float *out, int memory);
+/* gradient of the relative L2 loss (host output, kNrcParams floats) without updating */
+int  crt_nrc_gradient(crt_nrc *nrc, const float *queries, const float *targets, uint32_t n, float *grad,
+                      float *loss, int memory);
+/* forward + backward + one Adam step */
+int  crt_nrc_train_step(crt_nrc *nrc, const float *queries, const float *targets, uint32_t n, float *loss,
+                        int memory);
+
+/* Rendering with the cache: paths query it at bounce `term_bounce` (one bounce later when
+ * the path footprint is still small, footprint_c > 0) instead of continuing; every pass
+ * also traces train_paths training paths train_suffix bounces past the query vertex and
+ * takes train_steps Adam steps on their radiance.  nrc = NULL turns the cache off. */
+typedef struct crt_nrc_render_options {
+    uint32_t term_bounce;   /* default 1 */
+    float    footprint_c;   /* default 0.01; 0 = always query at term_bounce */
+    uint32_t train_paths;   /* default 65536 */
+    uint32_t train_suffix;  /* default 4 */
+    uint32_t train_steps;   /* default 4 */
+    uint32_t train;         /* 0 freezes the cache (inference only) */
+} crt_nrc_render_options;
+void crt_nrc_render_options_default(crt_nrc_render_options *opt);
+int  crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt);
 
 /* camera.cpp:54-68: matrix = T(position) * Ry(rot.x) * Rx(rot.y) * Rz(rot.z) */
 void crt_camera_default(crt_camera *cam);   /* camera.h:21: position (5,5,0), fov 75, identity */
