@@ -533,7 +533,6 @@ int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render)
     const uint32_t batch = std::min(count / steps, n->opt.max_batch);
     if (batch < 64) return CRT_OK;
     std::lock_guard<std::mutex> g(n->mu);
-    float loss = 0.0f;
     for (uint32_t k = 0; k < steps; ++k) {
         const float *q  = t.tq + (size_t)k * batch * 13;
         const float *tt = t.tt + (size_t)k * batch * 3;
@@ -542,11 +541,14 @@ int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render)
         HIP_CHECK(nrc_launch_adam(n->partial, (batch + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                                   n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
     }
-    float pl = 0.0f;
-    HIP_CHECK(hipMemcpyAsync(&pl, n->partial_loss, sizeof pl, hipMemcpyDeviceToHost, s));
+    // loss of the last step: sum of its per-tile partials
+    const uint32_t tiles = (batch + 63) / 64;
+    std::vector<float> pl(tiles);
+    HIP_CHECK(hipMemcpyAsync(pl.data(), n->partial_loss, sizeof(float) * tiles, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    (void)loss;
-    r->last_train_loss = pl;
+    double acc = 0.0;
+    for (float x : pl) acc += x;
+    r->last_train_loss = (float)acc;
     return CRT_OK;
 }
 

@@ -122,11 +122,12 @@ __device__ __forceinline__ void hidden(bf16 *X, const bf16 *WT)
     __syncthreads();
 }
 
+// n_dev (when set) is a device-side count, clamped to the buffers' capacity n_host.
 __global__ __launch_bounds__(256) void k_nrc_infer(const float *__restrict__ q, float *__restrict__ out, uint32_t n_host,
                                                    const uint32_t *__restrict__ n_dev, const bf16 *__restrict__ WT)
 {
     __shared__ bf16 xs[4 * 64 * kXStride];
-    const uint32_t n = n_dev ? *n_dev : n_host;
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r16 = lane & 15, g = lane >> 4;
     bf16 *X = xs + wv * 64 * kXStride;
     const uint32_t ntiles = (n + 255) / 256;
@@ -339,7 +340,7 @@ hipError_t nrc_launch_infer_dev(const float *q, float *out, const uint32_t *n_de
 {
     if (n_max == 0) return hipSuccess;
     const uint32_t tiles = (n_max + 255) / 256;
-    k_nrc_infer<<<std::min<uint32_t>(tiles, 2048u), 256, 0, s>>>(q, out, 0u, n_dev, WT);
+    k_nrc_infer<<<std::min<uint32_t>(tiles, 2048u), 256, 0, s>>>(q, out, n_max, n_dev, WT);
     return hipGetLastError();
 }
 

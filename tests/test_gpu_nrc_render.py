@@ -52,17 +52,19 @@ def test_cache_trains_and_renders(setup):
     ctx, sc, spec, ref = setup
     nrc = Nrc(ctx, max_batch=65536, seed=3)
     r = _renderer(ctx, sc, spec, seed=5)
-    r.set_nrc(nrc, train_paths=16384, train_steps=4)
-    losses = []
-    for _ in range(40):
-        r.render(1)
-        losses.append(r.current_stats()["nrc_loss"])
-    img = r.raw() / 40.0
+    try:
+        r.set_nrc(nrc, train_paths=16384, train_steps=4)
+        losses = []
+        for _ in range(40):
+            r.render(1)
+            losses.append(r.current_stats()["nrc_loss"])
+        img = r.raw() / 40.0
+    finally:
+        r.close()
+        nrc.close()
     assert np.all(np.isfinite(img))
     assert abs(img.mean() - ref.mean()) < 0.15 * ref.mean(), (img.mean(), ref.mean())
     assert np.mean(losses[-10:]) < np.mean(losses[:3])
-    r.close()
-    nrc.close()
 
 
 def _mse(a, b):
