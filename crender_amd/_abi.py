@@ -73,7 +73,7 @@ class Camera(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("bvh_width", C.c_uint32), ("max_leaf", C.c_uint32),
-                ("trace_waves_per_cu", C.c_uint32), ("flags", C.c_uint32)]
+                ("trace_waves_per_cu", C.c_uint32), ("flags", C.c_uint32), ("trace_stack", C.c_uint32)]
 
 
 class RenderOptions(C.Structure):
@@ -101,9 +101,31 @@ class NrcOptions(C.Structure):
                 ("beta2", C.c_float), ("eps", C.c_float)]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("options", RenderOptions), ("rgb_sum", C.c_void_p),
+                ("memory", C.c_int32)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("max_paths", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("rays_per_second", C.c_double), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("total_rays", C.c_uint64), ("replays", C.c_uint32),
+                ("max_paths", C.c_uint32), ("used_graph", C.c_uint32), ("bvh_width", C.c_uint32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("bvh_width", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("trace_stack", C.c_uint32),
+                ("max_paths", C.c_uint32), ("rays_per_second", C.c_double), ("trials", C.c_uint32)]
+
+
 class NrcRenderOptions(C.Structure):
     _fields_ = [("term_bounce", C.c_uint32), ("footprint_c", C.c_float), ("train_paths", C.c_uint32),
-                ("train_suffix", C.c_uint32), ("train_steps", C.c_uint32), ("train", C.c_uint32)]
+                ("train_suffix", C.c_uint32), ("train_steps", C.c_uint32), ("train", C.c_uint32),
+                ("warmup_passes", C.c_uint32)]
 
 
 class Rays(C.Structure):
@@ -172,6 +194,10 @@ SIGNATURES = {
     "crt_nrc_train_step": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
     "crt_nrc_render_options_default": (None, [C.POINTER(NrcRenderOptions)]),
     "crt_renderer_set_nrc": (C.c_int, [P, P, C.POINTER(NrcRenderOptions)]),
+    "crt_batch_options_default": (None, [C.POINTER(BatchOptions)]),
+    "crt_autotune": (C.c_int, [P, C.POINTER(Job), C.c_uint32, C.c_uint32, C.POINTER(TuneResult)]),
+    "crt_render_batch": (C.c_int, [P, C.POINTER(Job), C.c_uint32, C.POINTER(BatchOptions),
+                                   C.POINTER(C.c_double), C.POINTER(BatchStats)]),
     "crt_camera_default": (None, [C.POINTER(Camera)]),
     "crt_camera_update": (None, [C.POINTER(Camera)]),
     "crt_sun_default": (None, [C.POINTER(Sun)]),

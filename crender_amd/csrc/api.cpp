@@ -84,6 +84,8 @@ struct crt_context {
     hipStream_t stream = nullptr;
     uint32_t    grid   = 0;
     uint32_t   *spill  = nullptr;
+    uint32_t    stack  = kStackLds;   // LDS stack entries per traversal lane (8, 16, 32)
+    uint32_t    batch_paths = 0;   // tuned wavefront size of crt_render_batch (0 = default)
     // staging for crt_intersect / crt_occluded
     uint32_t    stage_cap = 0;
     float      *st_ray[8] = {};
@@ -172,9 +174,12 @@ struct crt_renderer {
     std::vector<void *>    nrc_allocs;
     double                 ms_nrc_train = 0.0;
     float                  last_train_loss = 0.0f;
+    uint32_t               nrc_warm_done = 0;   // warm-up passes run since set_nrc
 };
 
 namespace {
+
+TraceConfig trace_cfg(const crt_context *ctx) { return TraceConfig{ctx->grid, ctx->stack}; }
 
 int ensure_spill(crt_context *ctx)
 {
@@ -354,7 +359,7 @@ camera_dev to_camera_dev(const crt_camera &c)
 
 TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *count, uint32_t *work, uint32_t *err)
 {
-    TraceQueue q;
+    TraceQueue q{};
     q.rays     = rays;
     q.hits     = hits;
     q.count    = count;
@@ -381,6 +386,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
     const SceneDev &sc  = r->scene->dev;
     hipStream_t     s   = ctx->stream;
     auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(r->ev[i], s) : hipSuccess; };
+    if (int st = ensure_spill(ctx)) return st;   // the autotuner may have resized the grid
     HIP_CHECK(launch_generate(sc, b, rp, s));
     HIP_CHECK(mark(1));
     uint32_t cur = 0;
@@ -390,7 +396,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
         HIP_CHECK(launch_bounce_prep(b, cur, s));
         if (t) HIP_CHECK(mark(2 + 4 * (int)i));
         TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
-        HIP_CHECK(launch_trace_closest(sc, q, b.P, ctx->spill, ctx->grid, s));
+        HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
         if (t) HIP_CHECK(mark(3 + 4 * (int)i));
         HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
         if (t) HIP_CHECK(mark(4 + 4 * (int)i));
@@ -398,13 +404,13 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
             HIP_CHECK(launch_shadow_prep(b, s));
             if (!sc.any_alpha) {
                 TraceQueue qs = make_queue(b.sq, HitSoA{}, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
-                HIP_CHECK(launch_trace_any(sc, qs, b.P, b.occ, ctx->spill, ctx->grid, s));
+                HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, trace_cfg(ctx), s));
                 HIP_CHECK(launch_shadow_resolve(b, s));
             } else {
                 // closest hit + alpha pass-through, renderer.cpp:330-345 (bounded at 64 steps)
                 for (int step = 0; step < 64; ++step) {
                     TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
-                    HIP_CHECK(launch_trace_closest(sc, qs, b.P, ctx->spill, ctx->grid, s));
+                    HIP_CHECK(launch_trace_closest(sc, qs, ctx->spill, trace_cfg(ctx), s));
                     HIP_CHECK(launch_shadow_alpha_step(sc, b, s));
                     uint32_t left = 0;
                     HIP_CHECK(hipMemcpyAsync(&left, b.cnt + kCntShadow, sizeof left, hipMemcpyDeviceToHost, s));
@@ -455,6 +461,15 @@ int render_pass(crt_renderer *r)
     rp.seed        = r->opt.seed;
     rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
     rp.nrc         = nrc_params(r);
+    // cache warm-up pass: only the training paths (their own Philox stream), no image
+    if (r->nrc && r->nrc_warm_done < r->nopt.warmup_passes) {
+        rp.sample = 0x40000000u + r->nrc_warm_done;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (int st = nrc_training_pass(r, rp)) return st;
+        r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        r->nrc_warm_done++;
+        return CRT_OK;
+    }
     const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0;
     HIP_CHECK(hipEventRecord(r->ev[0], s));
     r->ms_phase_pending = 0;
@@ -660,7 +675,7 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
         h.v   = dev && hits->v ? hits->v : ctx->st_v;
         h.tri = ctx->st_tri;
         TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
-        HIP_CHECK(launch_trace_closest(scene->dev, q, n, ctx->spill, ctx->grid, s));
+        HIP_CHECK(launch_trace_closest(scene->dev, q, ctx->spill, trace_cfg(ctx), s));
         uint32_t *prim = dev ? hits->prim_id : ctx->st_prim;
         uint32_t *geom = dev ? hits->geom_id : (hits->geom_id ? ctx->st_geom : nullptr);
         float    *ng[3] = {hits->ng_x, hits->ng_y, hits->ng_z};
@@ -680,7 +695,7 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
     } else {
         TraceQueue q = make_queue(rs, HitSoA{}, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
         uint8_t   *occ = memory == CRT_MEM_DEVICE ? occluded : ctx->st_occ;
-        HIP_CHECK(launch_trace_any(scene->dev, q, n, occ, ctx->spill, ctx->grid, s));
+        HIP_CHECK(launch_trace_any(scene->dev, q, occ, ctx->spill, trace_cfg(ctx), s));
         if (memory == CRT_MEM_HOST) HIP_CHECK(hipMemcpyAsync(occluded, ctx->st_occ, n, hipMemcpyDeviceToHost, s));
     }
     uint32_t err = 0;
@@ -715,6 +730,8 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     if (opt) o = *opt;
     if (o.bvh_width != 0 && o.bvh_width != 4 && o.bvh_width != 8)
         return fail(CRT_ERR_INVALID_ARG, "bvh_width must be 0, 4 or 8");
+    if (o.trace_stack != 0 && o.trace_stack != 8 && o.trace_stack != 16 && o.trace_stack != 32)
+        return fail(CRT_ERR_INVALID_ARG, "trace_stack must be 0, 8, 16 or 32");
     int ndev = 0;
     HIP_CHECK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(CRT_ERR_INVALID_ARG, "no such HIP device");
@@ -727,8 +744,9 @@ int crt_context_create(const crt_options *opt, crt_context **out)
         delete ctx;
         return fail(CRT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    ctx->grid = trace_grid(o.device, o.trace_waves_per_cu ? (o.trace_waves_per_cu + 3) / 4 : 0);
-    *out      = ctx;
+    ctx->grid  = trace_grid(o.device, o.trace_waves_per_cu ? (o.trace_waves_per_cu + 3) / 4 : 0);
+    ctx->stack = o.trace_stack ? o.trace_stack : kStackLds;
+    *out       = ctx;
     return CRT_OK;
 }
 
@@ -1103,7 +1121,7 @@ int crt_scene_commit(crt_scene *scene)
     HIP_CHECK(hipStreamSynchronize(s));
 
     const uint32_t max_stack = ntri ? (width - 1) * br.depth + 1 : 0;
-    if (max_stack > kStackLds + kSpillDepth) {
+    if (max_stack > kSpillDepth + 8) {   // capacity of the smallest LDS stack configuration
         free_scene_device(scene);
         return fail(CRT_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
     }
@@ -1542,6 +1560,7 @@ extern "C" void crt_nrc_render_options_default(crt_nrc_render_options *o)
     o->train_suffix = 4;
     o->train_steps  = 4;
     o->train        = 1;
+    o->warmup_passes = 0;
 }
 
 extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt)
@@ -1557,7 +1576,8 @@ extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc
     if (opt) o = *opt;
     if (o.term_bounce < 1 || o.train_suffix < 1 || o.train_paths < 64 || o.train_paths > (1u << 22))
         return fail(CRT_ERR_INVALID_ARG, "bad cache render options");
-    r->nopt = o;
+    r->nopt          = o;
+    r->nrc_warm_done = 0;
     PathBuffers &t = r->tb;
     t.W = r->W;
     t.H = r->H;
@@ -1783,6 +1803,432 @@ int crt_nrc_train_step(crt_nrc *n, const float *queries, const float *targets, u
     HIP_CHECK(nrc_launch_adam(n->partial, (count + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                               n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, n->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(n->ctx->stream));
+    return CRT_OK;
+}
+
+}  // extern "C"
+
+// ---- batched rendering ------------------------------------------------------------------
+
+namespace {
+
+struct BatchPlan {
+    std::vector<BatchSeg>   segs;
+    std::vector<BatchChunk> chunks;
+    std::vector<uint32_t>   job_last;   // last chunk holding a pass of each job
+};
+
+// Packs every job's passes into chunks of at most P path slots.  A job whose image fits
+// takes as many whole passes as fit into the current chunk (one segment, sample-major);
+// a larger image is split by pixel ranges one pass at a time.  No chunk holds the same
+// pixel of a job twice, so accumulation needs no atomics and keeps pass order.
+void plan_batch(const crt_job *jobs, uint32_t njobs, uint32_t P, BatchPlan &plan)
+{
+    plan.job_last.assign(njobs, 0);
+    BatchChunk cur{0, 0, 0, 0};
+    auto close = [&]() {
+        if (cur.nseg == 0) return;
+        plan.chunks.push_back(cur);
+        cur = BatchChunk{(uint32_t)plan.segs.size(), 0, 0, 0};
+    };
+    auto add = [&](uint32_t j, uint32_t s0, uint32_t ns, uint32_t pix0, uint32_t np) {
+        BatchSeg sg{};
+        sg.job   = j;
+        sg.s0    = s0;
+        sg.nsamp = ns;
+        sg.pix0  = pix0;
+        sg.npix  = np;
+        sg.path0 = cur.npaths;
+        sg.acc0  = cur.nacc;
+        plan.segs.push_back(sg);
+        cur.nseg++;
+        cur.npaths += ns * np;
+        cur.nacc += np;
+        plan.job_last[j] = (uint32_t)plan.chunks.size();
+    };
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const uint32_t npix = jobs[j].width * jobs[j].height, spp = jobs[j].spp;
+        if (npix <= P) {
+            uint32_t s = 0;
+            while (s < spp) {
+                const uint32_t k = std::min(spp - s, (P - cur.npaths) / npix);
+                if (k == 0) {
+                    close();
+                    continue;
+                }
+                add(j, s, k, 0, npix);
+                s += k;
+            }
+        } else {
+            for (uint32_t s = 0; s < spp; ++s) {
+                uint32_t p = 0;
+                while (p < npix) {
+                    const uint32_t c = std::min(npix - p, P - cur.npaths);
+                    if (c == 0) {
+                        close();
+                        continue;
+                    }
+                    add(j, s, 1, p, c);
+                    p += c;
+                }
+            }
+        }
+    }
+    close();
+}
+
+struct BatchRun {
+    std::vector<void *> allocs;
+    hipStream_t         stream = nullptr;
+    hipGraph_t          graph  = nullptr;
+    hipGraphExec_t      exec   = nullptr;
+    std::vector<hipEvent_t> events;
+    ~BatchRun()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events)
+            if (e) (void)hipEventDestroy(e);
+        for (void *p : allocs) dfree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// One replay: generate the chunk's paths, trace / shade / shadow for max_bounces bounces,
+// accumulate, advance the chunk cursor.  Only kernel launches (capturable).
+int record_replay(const PathBuffers &b, const BatchDev &bd, uint32_t *cursor, const TraceScene *tscenes,
+                  uint32_t width, uint32_t max_bounces, bool any_sun, bool any_alpha, uint32_t *spill,
+                  const TraceConfig &tc, hipStream_t s)
+{
+    HIP_CHECK(launch_batch_generate(bd, b, s));
+    uint32_t cur = 0;
+    for (uint32_t i = 0; i < max_bounces; ++i) {
+        HIP_CHECK(launch_bounce_prep(b, cur, s));
+        TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
+        q.ray_path = b.qpath[cur];
+        q.path_job = b.pjob;
+        q.scenes   = tscenes;
+        HIP_CHECK(launch_trace_closest_multi(width, q, spill, tc, s));
+        HIP_CHECK(launch_shade_multi(bd, b, i, cur, s));
+        if (any_sun) {
+            HIP_CHECK(launch_shadow_prep(b, s));
+            if (!any_alpha) {
+                TraceQueue qs = make_queue(b.sq, HitSoA{}, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
+                qs.ray_path = b.spath;
+                qs.path_job = b.pjob;
+                qs.scenes   = tscenes;
+                HIP_CHECK(launch_trace_any_multi(width, qs, b.occ, spill, tc, s));
+                HIP_CHECK(launch_shadow_resolve(b, s));
+            } else {
+                // alpha pass-through (renderer.cpp:330-345): host-driven, never captured
+                for (int step = 0; step < 64; ++step) {
+                    TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
+                    qs.ray_path = b.spath;
+                    qs.path_job = b.pjob;
+                    qs.scenes   = tscenes;
+                    HIP_CHECK(launch_trace_closest_multi(width, qs, spill, tc, s));
+                    HIP_CHECK(launch_shadow_alpha_step_multi(bd, b, s));
+                    uint32_t left = 0;
+                    HIP_CHECK(hipMemcpyAsync(&left, b.cnt + kCntShadow, sizeof left, hipMemcpyDeviceToHost, s));
+                    HIP_CHECK(hipStreamSynchronize(s));
+                    if (left == 0) break;
+                }
+            }
+        }
+        cur ^= 1u;
+    }
+    HIP_CHECK(launch_batch_accumulate(bd, b, b.P, s));
+    HIP_CHECK(launch_batch_advance(cursor, s));
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void crt_batch_options_default(crt_batch_options *opt)
+{
+    if (!opt) return;
+    opt->max_paths = 0;
+    opt->use_graph = 1;
+}
+
+int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, const crt_batch_options *opt,
+                     double *job_ms, crt_batch_stats *stats)
+{
+    if (!ctx || (!jobs && njobs)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (njobs == 0) return CRT_OK;
+    if (njobs > (1u << 20)) return fail(CRT_ERR_INVALID_ARG, "too many jobs");
+    crt_batch_options o;
+    crt_batch_options_default(&o);
+    if (opt) o = *opt;
+    uint32_t width = 0, max_bounces = 0;
+    bool any_sun = false, any_alpha = false;
+    uint64_t work = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const crt_job &J = jobs[j];
+        if (!J.scene || !J.rgb_sum) return fail(CRT_ERR_INVALID_ARG, "job without scene or output");
+        if (J.scene->ctx != ctx) return fail(CRT_ERR_INVALID_ARG, "job scene belongs to another context");
+        if (!J.scene->committed) return fail(CRT_ERR_STATE, "job scene not committed");
+        if (J.width == 0 || J.height == 0 || J.spp == 0 || J.max_bounces == 0)
+            return fail(CRT_ERR_INVALID_ARG, "job needs width, height, spp and max_bounces >= 1");
+        if ((uint64_t)J.width * J.height > (1u << 28)) return fail(CRT_ERR_INVALID_ARG, "job image too large");
+        if (J.options.mode > 1) return fail(CRT_ERR_INVALID_ARG, "bad mode");
+        const uint32_t w = J.scene->dev.width;
+        if (width && w != width) return fail(CRT_ERR_INVALID_ARG, "scenes of one batch must share the BVH width");
+        width       = w;
+        max_bounces = std::max(max_bounces, J.max_bounces);
+        if (J.scene->dev.sun_enabled) {
+            any_sun = true;
+            any_alpha = any_alpha || J.scene->dev.any_alpha;
+        }
+        work += (uint64_t)J.width * J.height * J.spp;
+    }
+    uint64_t P64 = o.max_paths ? o.max_paths : (ctx->batch_paths ? ctx->batch_paths : (4u << 20));
+    P64 = std::min<uint64_t>(P64, work);
+    P64 = std::min<uint64_t>(P64, 1u << 27);
+    const uint32_t P = (uint32_t)((P64 + 255) / 256 * 256);
+    BatchPlan plan;
+    plan_batch(jobs, njobs, P, plan);
+    const bool use_graph = o.use_graph && !any_alpha;
+
+    HIP_CHECK(hipSetDevice(ctx->device));
+    BatchRun run;
+    HIP_CHECK(hipStreamCreateWithFlags(&run.stream, hipStreamNonBlocking));
+    hipStream_t s = run.stream;
+    auto &L = run.allocs;
+    int st;
+    PathBuffers b{};
+    b.P = P;
+    if ((st = alloc_path_core(L, b, P))) return st;
+    if ((st = lalloc(L, &b.pjob, P))) return st;
+    if ((st = lalloc(L, &b.ppix, P))) return st;
+    if ((st = lalloc(L, &b.psamp, P))) return st;
+    uint32_t *spill = nullptr;
+    if ((st = lalloc(L, reinterpret_cast<uint8_t **>(&spill), trace_spill_bytes(ctx->grid)))) return st;
+    // jobs, per-job traversal views, outputs
+    std::vector<JobDev>     hj(njobs);
+    std::vector<TraceScene> hts(njobs);
+    std::vector<float *>    dev_out(njobs, nullptr);
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const crt_job &J = jobs[j];
+        JobDev &d = hj[j];
+        d.sc             = J.scene->dev;
+        d.rp             = RenderParams{};
+        d.rp.cam         = to_camera_dev(J.scene->cam);
+        d.rp.aspect      = (float)J.width / (float)J.height;   // renderer.cpp:199
+        d.rp.max_bounces = J.max_bounces;
+        d.rp.rr_start    = J.options.rr_start;
+        d.rp.mode        = J.options.mode;
+        d.rp.seed        = J.options.seed;
+        d.rp.sample      = J.options.sample_offset;
+        d.W              = J.width;
+        d.H              = J.height;
+        const size_t n3  = (size_t)J.width * J.height * 3;
+        if (J.memory == CRT_MEM_DEVICE) {
+            dev_out[j] = J.rgb_sum;
+        } else if ((st = lalloc(L, &dev_out[j], n3))) {
+            return st;
+        }
+        HIP_CHECK(hipMemsetAsync(dev_out[j], 0, sizeof(float) * n3, s));
+        d.raw   = dev_out[j];
+        hts[j]  = TraceScene{J.scene->dev.nodes, J.scene->dev.tri, J.scene->dev.root, J.scene->dev.ntri};
+    }
+    JobDev     *d_jobs = nullptr;
+    TraceScene *d_ts = nullptr;
+    BatchSeg   *d_segs = nullptr;
+    BatchChunk *d_chunks = nullptr;
+    uint32_t   *d_cursor = nullptr;
+    if ((st = lalloc(L, &d_jobs, njobs))) return st;
+    if ((st = lalloc(L, &d_ts, njobs))) return st;
+    if ((st = lalloc(L, &d_segs, plan.segs.size()))) return st;
+    if ((st = lalloc(L, &d_chunks, plan.chunks.size()))) return st;
+    if ((st = lalloc(L, &d_cursor, 1))) return st;
+    HIP_CHECK(hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_ts, hts.data(), sizeof(TraceScene) * njobs, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_segs, plan.segs.data(), sizeof(BatchSeg) * plan.segs.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_chunks, plan.chunks.data(), sizeof(BatchChunk) * plan.chunks.size(),
+                             hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(uint32_t), s));
+    HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+    HIP_CHECK(hipStreamSynchronize(s));   // host staging vectors are pageable
+
+    BatchDev bd{d_jobs, d_segs, d_chunks, d_cursor};
+    const uint32_t nrep = (uint32_t)plan.chunks.size();
+    run.events.assign(nrep + 1, nullptr);
+    for (hipEvent_t &e : run.events) HIP_CHECK(hipEventCreate(&e));
+    if (use_graph) {
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        st = record_replay(b, bd, d_cursor, d_ts, width, max_bounces, any_sun, any_alpha, spill, trace_cfg(ctx), s);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &g);
+        run.graph = g;
+        if (st) return st;
+        HIP_CHECK(ce);
+        HIP_CHECK(hipGraphInstantiate(&run.exec, run.graph, nullptr, nullptr, 0));
+    }
+    HIP_CHECK(hipEventRecord(run.events[0], s));
+    for (uint32_t c = 0; c < nrep; ++c) {
+        if (use_graph) {
+            HIP_CHECK(hipGraphLaunch(run.exec, s));
+        } else if ((st = record_replay(b, bd, d_cursor, d_ts, width, max_bounces, any_sun, any_alpha, spill,
+                                       trace_cfg(ctx), s))) {
+            return st;
+        }
+        HIP_CHECK(hipEventRecord(run.events[c + 1], s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    for (uint32_t j = 0; j < njobs; ++j) {
+        if (jobs[j].memory != CRT_MEM_DEVICE)
+            HIP_CHECK(hipMemcpy(jobs[j].rgb_sum, dev_out[j], sizeof(float) * 3 * jobs[j].width * jobs[j].height,
+                                hipMemcpyDeviceToHost));
+        if (job_ms) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, run.events[0], run.events[plan.job_last[j] + 1]));
+            job_ms[j] = ms;
+        }
+    }
+    if (stats) {
+        std::vector<unsigned long long> sv(4 * kStatSlots);
+        HIP_CHECK(hipMemcpy(sv.data(), b.stats, sizeof(unsigned long long) * sv.size(), hipMemcpyDeviceToHost));
+        uint64_t tot[4] = {0, 0, 0, 0};
+        for (uint32_t i = 0; i < kStatSlots; ++i)
+            for (int k = 0; k < 4; ++k) tot[k] += sv[4 * i + k];
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, run.events[0], run.events[nrep]));
+        *stats = crt_batch_stats{};
+        stats->ms_total        = ms;
+        stats->extension_rays  = tot[0];
+        stats->shadow_rays     = tot[1];
+        stats->total_rays      = tot[2];
+        stats->rays_per_second = ms > 0 ? (double)(tot[0] + tot[1]) / (ms * 1e-3) : 0.0;
+        stats->replays         = nrep;
+        stats->max_paths       = P;
+        stats->used_graph      = use_graph ? 1u : 0u;
+        stats->bvh_width       = width;
+    }
+    return CRT_OK;
+}
+
+int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t flags, crt_tune_result *out)
+{
+    if (!ctx || !jobs || njobs == 0 || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    // the sample: every job once at 1 spp into scratch device outputs
+    std::vector<crt_job> sample(jobs, jobs + njobs);
+    std::vector<float *> scratch(njobs, nullptr);
+    uint64_t work = 0;
+    struct Free {
+        std::vector<float *> &v;
+        ~Free() { for (float *p : v) dfree(p); }
+    } guard{scratch};
+    for (uint32_t j = 0; j < njobs; ++j) {
+        if (!jobs[j].scene) return fail(CRT_ERR_INVALID_ARG, "job without scene");
+        HIP_CHECK(dalloc(&scratch[j], (size_t)jobs[j].width * jobs[j].height * 3));
+        sample[j].spp     = 1;
+        sample[j].rgb_sum = scratch[j];
+        sample[j].memory  = CRT_MEM_DEVICE;
+        work += (uint64_t)jobs[j].width * jobs[j].height;
+    }
+    std::vector<crt_scene *> scenes;
+    for (uint32_t j = 0; j < njobs; ++j)
+        if (std::find(scenes.begin(), scenes.end(), jobs[j].scene) == scenes.end()) scenes.push_back(jobs[j].scene);
+
+    uint32_t trials = 0;
+    auto measure = [&](double *rays_per_s) -> int {
+        crt_batch_options bo;
+        crt_batch_options_default(&bo);
+        bo.max_paths = ctx->batch_paths;
+        crt_batch_stats bs{};
+        double best = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {   // the first run pays for allocation and graph capture
+            const int st = crt_render_batch(ctx, sample.data(), njobs, &bo, nullptr, &bs);
+            if (st) return st;
+            best = std::max(best, bs.rays_per_second);
+        }
+        ++trials;
+        *rays_per_s = best;
+        return CRT_OK;
+    };
+    auto set_grid = [&](uint32_t blocks) {
+        dfree(ctx->spill);   // sized by the grid; renderers re-allocate it on demand
+        ctx->spill = nullptr;
+        ctx->grid  = trace_grid(ctx->device, blocks);
+        ctx->opt.trace_waves_per_cu = 4 * blocks;
+    };
+    auto recommit = [&](uint32_t width) -> int {
+        ctx->opt.bvh_width = width;
+        for (crt_scene *sc : scenes) {
+            if (sc->dev.width == width) continue;
+            const int st = crt_scene_commit(sc);
+            if (st) return st;
+        }
+        return CRT_OK;
+    };
+    if (ctx->batch_paths == 0) ctx->batch_paths = 4u << 20;
+    uint32_t blocks = ctx->opt.trace_waves_per_cu ? (ctx->opt.trace_waves_per_cu + 3) / 4 : 4;
+    uint32_t width  = scenes[0]->dev.width;
+    int st = recommit(width);   // one width for the batch
+    if (st) return st;
+    set_grid(blocks);
+    double best = 0.0;
+    if ((st = measure(&best))) return st;
+    if (flags & CRT_TUNE_WIDTH) {
+        const uint32_t other = width == 8 ? 4u : 8u;
+        if ((st = recommit(other))) return st;
+        double v = 0.0;
+        if ((st = measure(&v))) return st;
+        if (v > best) {
+            best  = v;
+            width = other;
+        } else if ((st = recommit(width))) {
+            return st;
+        }
+    }
+    for (uint32_t cand : {2u, 3u, 4u, 5u}) {
+        if (cand == blocks) continue;
+        const uint32_t keep = blocks;
+        set_grid(cand);
+        double v = 0.0;
+        if ((st = measure(&v))) return st;
+        if (v > best) {
+            best   = v;
+            blocks = cand;
+        } else {
+            set_grid(keep);
+        }
+    }
+    for (uint32_t cand : {8u, 16u, 32u}) {
+        if (cand == ctx->stack) continue;
+        const uint32_t keep = ctx->stack;
+        ctx->stack = cand;
+        double v = 0.0;
+        if ((st = measure(&v))) return st;
+        if (v > best) best = v;
+        else ctx->stack = keep;
+    }
+    for (uint32_t cand : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+        if (cand == ctx->batch_paths) continue;
+        if (cand > 2 * work && cand > (1u << 20)) continue;   // larger than the sample: nothing to learn
+        const uint32_t keep = ctx->batch_paths;
+        ctx->batch_paths = cand;
+        double v = 0.0;
+        if ((st = measure(&v))) return st;
+        if (v > best) best = v;
+        else ctx->batch_paths = keep;
+    }
+    ctx->opt.trace_stack = ctx->stack;
+    out->bvh_width           = width;
+    out->trace_blocks_per_cu = blocks;
+    out->trace_stack         = ctx->stack;
+    out->max_paths           = ctx->batch_paths;
+    out->rays_per_second     = best;
+    out->trials              = trials;
     return CRT_OK;
 }
 

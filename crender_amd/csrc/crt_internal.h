@@ -137,22 +137,39 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
                      hipStream_t s, BuildResult *out);
 
 // ---- traversal ----------------------------------------------------------------------
+struct TraceScene;
 struct TraceQueue {
     RaySoA          rays;
     HitSoA          hits;
     const uint32_t *count;     // device: number of rays
     uint32_t       *counters;  // device: work counters (kXcds entries), zeroed by the caller's kernel
     uint32_t       *error;     // device: set to 1 if a traversal stack would overflow
+    // batched rendering: the scene of ray i is scenes[path_job[ray_path[i]]]
+    const uint32_t   *ray_path;
+    const uint32_t   *path_job;
+    const TraceScene *scenes;
 };
-hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t max_rays,
-                                uint32_t *spill, uint32_t grid, hipStream_t s);
-hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint32_t max_rays, uint8_t *occ,
-                            uint32_t *spill, uint32_t grid, hipStream_t s);
+// Launch shape of the persistent traversal: blocks in the grid and LDS stack entries per
+// lane (8, 16 or 32).
+struct TraceConfig {
+    uint32_t grid;
+    uint32_t stack;
+};
+hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
+                                hipStream_t s);
+hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
+                            const TraceConfig &tc, hipStream_t s);
+// batched: every ray carries its scene (q.ray_path / q.path_job / q.scenes); all scenes
+// share one BVH width
+hipError_t launch_trace_closest_multi(uint32_t width, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
+                                      hipStream_t s);
+hipError_t launch_trace_any_multi(uint32_t width, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
+                                  const TraceConfig &tc, hipStream_t s);
 uint32_t trace_grid(int device, uint32_t waves_per_cu);
 uint64_t trace_spill_bytes(uint32_t grid);
 constexpr uint32_t kTraceBlock = 256;
 constexpr uint32_t kXcds       = 8;
-constexpr uint32_t kStackLds   = 16;    // LDS stack entries per lane
+constexpr uint32_t kStackLds   = 16;    // default LDS stack entries per lane
 constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multiple of kStackLds)
 
 // ---- integrator ---------------------------------------------------------------------
@@ -182,6 +199,8 @@ struct PathBuffers {
     float    *rq;               // training records: queries [P * rec_slots][13]
     float4   *r_thr, *r_rad;    // throughput and radiance before each recorded vertex
     float    *tq, *tt;          // compacted training set: queries [n][13], targets [n][3]
+    // batched rendering: job, pixel and absolute sample index of each path slot
+    uint32_t *pjob, *ppix, *psamp;
 };
 constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
@@ -208,6 +227,37 @@ struct RenderParams {
     NrcParams  nrc;
 };
 
+// ---- batched rendering ----------------------------------------------------------------
+// One job of a batch: a committed scene rendered at its own resolution, sample count and
+// depth into its own accumulation buffer.
+struct JobDev {
+    SceneDev     sc;
+    RenderParams rp;       // rp.sample: absolute sample index of the job's first pass
+    uint32_t     W, H;
+    float       *raw;      // W*H*3 running sum, flipped in x and y like the renderer's
+};
+// Traversal view of a job's scene.
+struct TraceScene {
+    const void   *nodes;
+    const float4 *tri;
+    uint32_t      root, ntri;
+};
+// A segment of a chunk: passes s0 .. s0+nsamp-1 of pixels pix0 .. pix0+npix-1 of one job,
+// laid out sample-major from path slot path0; acc0 = first accumulation thread.
+struct BatchSeg {
+    uint32_t job, s0, nsamp, pix0, npix, path0, acc0, pad;
+};
+// One chunk = one replay of the pass graph: segments seg0 .. seg0+nseg-1.
+struct BatchChunk {
+    uint32_t seg0, nseg, npaths, nacc;
+};
+struct BatchDev {
+    const JobDev     *jobs;
+    const BatchSeg   *segs;
+    const BatchChunk *chunks;
+    const uint32_t   *cursor;    // device: index of the chunk being rendered
+};
+
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s);
 hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s);
 hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
@@ -222,6 +272,11 @@ hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s);
 hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s);
 hipError_t nrc_launch_infer_dev(const float *q, float *out, const uint32_t *n_dev, uint32_t n_max, const __bf16 *WT,
                                 hipStream_t s);
+hipError_t launch_batch_generate(const BatchDev &bd, const PathBuffers &b, hipStream_t s);
+hipError_t launch_shade_multi(const BatchDev &bd, const PathBuffers &b, uint32_t bounce, uint32_t cur, hipStream_t s);
+hipError_t launch_shadow_alpha_step_multi(const BatchDev &bd, const PathBuffers &b, hipStream_t s);
+hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uint32_t max_acc, hipStream_t s);
+hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s);
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
                               float *ngx, float *ngy, float *ngz, hipStream_t s);
 

@@ -101,6 +101,7 @@ __device__ __forceinline__ f3 sun_radiance(const SunDev &s, f3 dir)
 
 __device__ __forceinline__ void aov_write(const PathBuffers &b, uint32_t p, f3 alb, f3 nrm, float dep)
 {
+    if (!b.aov_alb) return;   // batched jobs keep no AOVs
     const uint32_t x = p % b.W, y = p / b.W;
     const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
     b.aov_alb[q] = make_float4(alb.x, alb.y, alb.z, 1.0f);
@@ -251,267 +252,296 @@ __device__ __forceinline__ void nrc_query(const NrcParams &np, f3 point, f3 d, f
     q[12] = m.type == 1 ? 1.0f : (m.type == 0 ? m.roughness : 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, RenderParams rp, uint32_t bounce,
-                                               uint32_t cur)
+// Output of shading one path vertex: the rays and cache query it pushes.
+struct ShadeOut {
+    bool     push_next, push_shadow, do_query;
+    f3       no, nd, so, sd, scontrib, qthr;
+    float    nq[13];
+    uint32_t seg;
+};
+
+// Shades queue entry k (path slot p, pixel pix, absolute sample index `sample`) at `bounce`:
+// renderer.cpp:273-356 (compat) or the physically based variant (pbr).
+__device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t k,
+                                           uint32_t p, uint32_t pix, uint32_t sample, uint32_t bounce, uint32_t cur,
+                                           ShadeOut &out)
+{
+    const RaySoA &q = b.q[cur];
+    bool &push_next = out.push_next, &push_shadow = out.push_shadow, &do_query = out.do_query;
+    f3 &no = out.no, &nd = out.nd, &so = out.so, &sd = out.sd, &scontrib = out.scontrib, &qthr = out.qthr;
+    float (&nq)[13] = out.nq;
+    uint32_t &seg = out.seg;
+    const f3       o   = mk3(q.ox[k], q.oy[k], q.oz[k]);
+    const f3       d   = mk3(q.dx[k], q.dy[k], q.dz[k]);
+    const uint32_t tri = b.hits.tri[k];
+    float4         thr4 = b.thr[p];
+    f3             thr  = mk3(thr4.x, thr4.y, thr4.z);
+    float4         rad4 = b.rad[p];
+    f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
+    const bool     can_continue = bounce + 1 < rp.max_bounces;
+    qthr = thr;
+    if (tri == kInvalidRef) {
+        const f3 s = sky(sc, d);
+        if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
+        rad = rad + thr * s;
+    } else {
+        const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k]);
+        const MaterialDev &m = sc.mats[sc.mat_idx[h.prim]];
+        const float4       col = mat_colour(sc, m, h.u, h.v);
+        seg = 1;
+        if (col.w == 0.0f) {
+            // alpha pass-through (renderer.cpp:294-301)
+            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+            no = h.point + d * 0.1f;
+            nd = d;
+            push_next = can_continue;
+        } else if (rp.nrc.enabled && nrc_terminates(rp.nrc, b, p, bounce, h, d)) {
+            // the cache stands in for this vertex's emission, direct light and everything after it
+            const f3 cc = mk3(col.x, col.y, col.z);
+            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, cc, h.normal, h.dist);
+            nrc_query(rp.nrc, h.point, d, h.normal, cc, m, nq);
+            do_query = true;
+        } else {
+            const rnd4 r  = rng4(rp.seed, pix, sample, bounce, 0);
+            const f3   n  = h.normal;
+            const f3   cc = mk3(col.x, col.y, col.z);
+            if (rp.nrc.record && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
+                const size_t slot = (size_t)p * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
+                nrc_query(rp.nrc, h.point, d, n, cc, m, b.rq + slot * 13);
+                b.r_thr[slot] = make_float4(thr.x, thr.y, thr.z, 1.0f);
+                b.r_rad[slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+            }
+            if (rp.nrc.enabled && bounce == 0) {
+                // primary footprint a0 = |x1 - x0|^2 / (4 pi |cos|)
+                const float c0 = fabsf(dot(n, normalize(d)));
+                b.nstate[p].x = h.dist * h.dist / (4.0f * kPi * fmaxf(c0, 1e-4f));
+            }
+            f3   alb = cc, nee_n = n, f0 = mk3(0, 0, 0);
+            bool delta = false, absorbed = false, ggx = false;
+            float ar = 0.0f;
+            if (rp.mode == kModeCompat) {
+                // renderer.cpp:45-99
+                if (m.type == 2) {
+                    f3    out_n = n;
+                    const f3 reflected = reflect(d, n);
+                    float ni = 1.0f / m.ior;
+                    if (dot(d, n) > 0.0f) {
+                        out_n = -n;
+                        ni    = m.ior;
+                    }
+                    const f3    uvd  = normalize(d);
+                    const float dt   = dot(uvd, out_n);
+                    const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
+                    no = h.point + out_n * -0.0001f;
+                    nd = disc > 0.0f ? (uvd - out_n * dt) * ni - out_n * sqrtf(disc) : reflected;
+                } else if (m.type == 0) {
+                    no  = h.point + n * 0.0001f;
+                    nd  = reflect(d, n);
+                    alb = alb * m.reflectiveness;
+                } else {
+                    const f3 sp = sphere(r.a, r.b);
+                    no = h.point + n * 0.0001f;
+                    nd = normalize(n + sp);
+                }
+            } else {
+                const f3 nf = dot(d, n) > 0.0f ? -n : n;
+                if (m.type == 2) {
+                    delta = true;
+                    f3    out_n = n;
+                    float ni    = 1.0f / m.ior;
+                    if (dot(d, n) > 0.0f) {
+                        out_n = -n;
+                        ni    = m.ior;
+                    }
+                    const f3    uvd  = normalize(d);
+                    const float dt   = dot(uvd, out_n);
+                    const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
+                    bool refl = true;
+                    if (disc > 0.0f) {
+                        const rnd4  r2  = rng4(rp.seed, pix, sample, bounce, 1);
+                        const float cf  = ni <= 1.0f ? -dt : sqrtf(disc);
+                        const float r0  = (1.0f - m.ior) / (1.0f + m.ior);
+                        const float f0s = r0 * r0;
+                        const float fr  = f0s + (1.0f - f0s) * powf(1.0f - cf, 5.0f);
+                        refl = r2.b < fr;
+                    }
+                    if (refl) {
+                        no = h.point + out_n * 0.0001f;
+                        nd = reflect(uvd, out_n);
+                    } else {
+                        no = h.point + out_n * -0.0001f;
+                        nd = (uvd - out_n * dt) * ni - out_n * sqrtf(disc);
+                    }
+                } else if (m.type == 0) {
+                    const float a  = m.roughness;
+                    f0   = cc * m.reflectiveness;
+                    const f3 wo = -normalize(d);
+                    no    = h.point + nf * 0.0001f;
+                    nee_n = nf;
+                    if (a < 1e-3f) {
+                        delta = true;
+                        nd    = reflect(d, nf);
+                        alb   = schlick(dot(wo, nf), f0);
+                    } else {
+                        ggx = true;
+                        ar  = a;
+                        f3 tb, bb;
+                        onb(nf, tb, bb);
+                        const f3 lv = mk3(dot(wo, tb), dot(wo, bb), dot(wo, nf));
+                        // Heitz 2018, sampling the GGX distribution of visible normals
+                        const f3    vh    = normalize(mk3(a * lv.x, a * lv.y, lv.z));
+                        const float lensq = vh.x * vh.x + vh.y * vh.y;
+                        const f3    t1v   = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk3(1, 0, 0);
+                        const f3    t2v   = cross(vh, t1v);
+                        const float rr    = sqrtf(r.a);
+                        const float phi   = kTau * r.b;
+                        const float t1    = rr * cosf(phi);
+                        float       t2    = rr * sinf(phi);
+                        const float s     = 0.5f * (1.0f + vh.z);
+                        t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+                        const float nz2 = fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2);
+                        const f3 nh = t1v * t1 + t2v * t2 + vh * sqrtf(nz2);
+                        const f3 ne = normalize(mk3(a * nh.x, a * nh.y, fmaxf(nh.z, 0.0f)));
+                        const float voh = dot(lv, ne);
+                        const f3 ll = ne * (2.0f * voh) - lv;
+                        if (ll.z <= 0.0f || lv.z <= 0.0f) {
+                            absorbed = true;
+                            alb = mk3(0, 0, 0);
+                            nd  = nf;
+                        } else {
+                            nd = normalize(tb * ll.x + bb * ll.y + nf * ll.z);
+                            const float a2 = a * a;
+                            const float g1 = 2.0f * lv.z / (lv.z + sqrtf(a2 + (1.0f - a2) * lv.z * lv.z));
+                            const float g2 = ggx_vis(lv.z, ll.z, a) * 4.0f * lv.z * ll.z;
+                            alb = schlick(voh, f0) * (g2 / g1);
+                        }
+                    }
+                } else {
+                    const f3 sp = sphere(r.a, r.b);
+                    no    = h.point + nf * 0.0001f;
+                    nd    = normalize(nf + sp);
+                    nee_n = nf;
+                }
+            }
+            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, rp.mode == kModeCompat ? alb : cc, n, h.dist);
+            const f3 thr_prev = thr;
+            if (rp.mode == kModeCompat) {
+                thr = thr * alb;
+                rad = rad + thr * m.emission;
+            } else {
+                rad = rad + (thr_prev * cc) * m.emission;
+                thr = thr * alb;
+            }
+            // sun NEE (renderer.cpp:315-354)
+            if (sc.sun_enabled && !(rp.mode == kModePbr && delta)) {
+                const SunDev &sun = sc.sun;
+                so = h.point + nee_n * 0.001f;
+                const f3 nd_sun = -sun.dir;
+                f3 tb, bb;
+                onb(nd_sun, tb, bb);
+                const float ct  = 1.0f - r.d * (1.0f - cosf(sun.size));
+                const float st  = sqrtf(1.0f - ct * ct);
+                const float phi = kTau * r.c;
+                const f3    loc = mk3(cosf(phi) * st, ct, sinf(phi) * st);
+                sd = tb * loc.x + nd_sun * loc.y + bb * loc.z;
+                const float pdf  = 1.0f / (kTau * (1.0f - cosf(sun.size)));
+                const float cosv = fminf(fmaxf(dot(nee_n, sd), 0.0f), 1.0f);
+                if (cosv > 0.0f) {
+                    const f3 L = sun_radiance(sun, sd);
+                    if (rp.mode == kModeCompat) {
+                        const f3 tc = (thr * cc) * cosv * L;
+                        scontrib = mk3(tc.x / pdf, tc.y / pdf, tc.z / pdf);
+                    } else if (ggx) {
+                        const f3    wo  = -normalize(d);
+                        const f3    hv  = normalize(wo + sd);
+                        const float nov = dot(nee_n, wo);
+                        if (nov > 0.0f) {
+                            const float dv = ggx_d(dot(nee_n, hv), ar) * ggx_vis(nov, cosv, ar);
+                            scontrib = ((thr_prev * schlick(dot(wo, hv), f0)) * L) * (dv * cosv / pdf);
+                        }
+                    } else {
+                        scontrib = (thr * L) * (cosv / (pdf * kPi));
+                    }
+                    push_shadow = scontrib.x != 0.0f || scontrib.y != 0.0f || scontrib.z != 0.0f ||
+                                  rp.mode == kModeCompat;
+                }
+            }
+            if (absorbed) {
+                seg = 0;
+            } else {
+                push_next = can_continue;
+            }
+            if (rp.nrc.enabled && rp.nrc.footprint_c > 0.0f) {
+                // pdf of the scattered direction (0 for delta lobes: the footprint does not grow)
+                float pdf = 0.0f;
+                if (!delta && !absorbed) {
+                    if (ggx) {
+                        const f3    wo = -normalize(d);
+                        const f3    hv = normalize(wo + nd);
+                        const float nv = fmaxf(dot(nee_n, wo), 1e-4f);
+                        const float a2 = ar * ar;
+                        const float g1 = 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
+                        pdf = ggx_d(dot(nee_n, hv), ar) * g1 / (4.0f * nv);
+                    } else {
+                        pdf = fabsf(dot(rp.mode == kModeCompat ? n : nee_n, nd)) * kInvPi;
+                    }
+                }
+                b.nstate[p].y = pdf;
+            }
+        }
+    }
+    // Russian roulette at the top of the next iteration
+    if (push_next && rp.rr_start && bounce + 1 >= rp.rr_start) {
+        const rnd4 r2 = rng4(rp.seed, pix, sample, bounce + 1, 1);
+        const float qv = fminf(maxc(thr), 0.95f);
+        if (r2.a >= qv) push_next = false;
+        else thr = thr * (1.0f / qv);
+    }
+    b.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+    b.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+}
+
+// One thread per queued extension ray.  kMulti: paths of many jobs (batched rendering),
+// each path's job, pixel and sample index come from pjob / ppix / psamp.
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, RenderParams rp, const JobDev *jobs,
+                                               uint32_t bounce, uint32_t cur)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.cnt[cur];
     const bool     live = k < n;
-    const RaySoA  &q = b.q[cur];
-    bool     push_next = false, push_shadow = false, do_query = false;
-    f3       no = mk3(0, 0, 0), nd = mk3(0, 0, 0), so = mk3(0, 0, 0), sd = mk3(0, 0, 0), scontrib = mk3(0, 0, 0);
-    f3       qthr = mk3(0, 0, 0);
-    float    nq[13];
-    uint32_t p = 0, seg = 0;
+    ShadeOut o;
+    o.push_next = o.push_shadow = o.do_query = false;
+    o.no = o.nd = o.so = o.sd = o.scontrib = o.qthr = mk3(0, 0, 0);
+    o.seg = 0;
+    uint32_t p = 0;
     if (live) {
         p = b.qpath[cur][k];
-        const f3       o   = mk3(q.ox[k], q.oy[k], q.oz[k]);
-        const f3       d   = mk3(q.dx[k], q.dy[k], q.dz[k]);
-        const uint32_t tri = b.hits.tri[k];
-        float4         thr4 = b.thr[p];
-        f3             thr  = mk3(thr4.x, thr4.y, thr4.z);
-        float4         rad4 = b.rad[p];
-        f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
-        const bool     can_continue = bounce + 1 < rp.max_bounces;
-        qthr = thr;
-        if (tri == kInvalidRef) {
-            const f3 s = sky(sc, d);
-            if (bounce == 0 && !rp.nrc.train) aov_write(b, p, s, mk3(0, 0, 0), 0.0f);
-            rad = rad + thr * s;
+        if constexpr (kMulti) {
+            const JobDev &J = jobs[b.pjob[p]];
+            shade_path(J.sc, b, J.rp, k, p, b.ppix[p], b.psamp[p], bounce, cur, o);
         } else {
-            const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k]);
-            const MaterialDev &m = sc.mats[sc.mat_idx[h.prim]];
-            const float4       col = mat_colour(sc, m, h.u, h.v);
-            seg = 1;
-            if (col.w == 0.0f) {
-                // alpha pass-through (renderer.cpp:294-301)
-                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
-                no = h.point + d * 0.1f;
-                nd = d;
-                push_next = can_continue;
-            } else if (rp.nrc.enabled && nrc_terminates(rp.nrc, b, p, bounce, h, d)) {
-                // the cache stands in for this vertex's emission, direct light and everything after it
-                const f3 cc = mk3(col.x, col.y, col.z);
-                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, cc, h.normal, h.dist);
-                nrc_query(rp.nrc, h.point, d, h.normal, cc, m, nq);
-                do_query = true;
-            } else {
-                const rnd4 r  = rng4(rp.seed, p, rp.sample, bounce, 0);
-                const f3   n  = h.normal;
-                const f3   cc = mk3(col.x, col.y, col.z);
-                if (rp.nrc.record && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
-                    const size_t slot = (size_t)p * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
-                    nrc_query(rp.nrc, h.point, d, n, cc, m, b.rq + slot * 13);
-                    b.r_thr[slot] = make_float4(thr.x, thr.y, thr.z, 1.0f);
-                    b.r_rad[slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-                }
-                if (rp.nrc.enabled && bounce == 0) {
-                    // primary footprint a0 = |x1 - x0|^2 / (4 pi |cos|)
-                    const float c0 = fabsf(dot(n, normalize(d)));
-                    b.nstate[p].x = h.dist * h.dist / (4.0f * kPi * fmaxf(c0, 1e-4f));
-                }
-                f3   alb = cc, nee_n = n, f0 = mk3(0, 0, 0);
-                bool delta = false, absorbed = false, ggx = false;
-                float ar = 0.0f;
-                if (rp.mode == kModeCompat) {
-                    // renderer.cpp:45-99
-                    if (m.type == 2) {
-                        f3    out_n = n;
-                        const f3 reflected = reflect(d, n);
-                        float ni = 1.0f / m.ior;
-                        if (dot(d, n) > 0.0f) {
-                            out_n = -n;
-                            ni    = m.ior;
-                        }
-                        const f3    uvd  = normalize(d);
-                        const float dt   = dot(uvd, out_n);
-                        const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
-                        no = h.point + out_n * -0.0001f;
-                        nd = disc > 0.0f ? (uvd - out_n * dt) * ni - out_n * sqrtf(disc) : reflected;
-                    } else if (m.type == 0) {
-                        no  = h.point + n * 0.0001f;
-                        nd  = reflect(d, n);
-                        alb = alb * m.reflectiveness;
-                    } else {
-                        const f3 sp = sphere(r.a, r.b);
-                        no = h.point + n * 0.0001f;
-                        nd = normalize(n + sp);
-                    }
-                } else {
-                    const f3 nf = dot(d, n) > 0.0f ? -n : n;
-                    if (m.type == 2) {
-                        delta = true;
-                        f3    out_n = n;
-                        float ni    = 1.0f / m.ior;
-                        if (dot(d, n) > 0.0f) {
-                            out_n = -n;
-                            ni    = m.ior;
-                        }
-                        const f3    uvd  = normalize(d);
-                        const float dt   = dot(uvd, out_n);
-                        const float disc = 1.0f - ni * ni * (1.0f - dt * dt);
-                        bool refl = true;
-                        if (disc > 0.0f) {
-                            const rnd4  r2  = rng4(rp.seed, p, rp.sample, bounce, 1);
-                            const float cf  = ni <= 1.0f ? -dt : sqrtf(disc);
-                            const float r0  = (1.0f - m.ior) / (1.0f + m.ior);
-                            const float f0s = r0 * r0;
-                            const float fr  = f0s + (1.0f - f0s) * powf(1.0f - cf, 5.0f);
-                            refl = r2.b < fr;
-                        }
-                        if (refl) {
-                            no = h.point + out_n * 0.0001f;
-                            nd = reflect(uvd, out_n);
-                        } else {
-                            no = h.point + out_n * -0.0001f;
-                            nd = (uvd - out_n * dt) * ni - out_n * sqrtf(disc);
-                        }
-                    } else if (m.type == 0) {
-                        const float a  = m.roughness;
-                        f0   = cc * m.reflectiveness;
-                        const f3 wo = -normalize(d);
-                        no    = h.point + nf * 0.0001f;
-                        nee_n = nf;
-                        if (a < 1e-3f) {
-                            delta = true;
-                            nd    = reflect(d, nf);
-                            alb   = schlick(dot(wo, nf), f0);
-                        } else {
-                            ggx = true;
-                            ar  = a;
-                            f3 tb, bb;
-                            onb(nf, tb, bb);
-                            const f3 lv = mk3(dot(wo, tb), dot(wo, bb), dot(wo, nf));
-                            // Heitz 2018, sampling the GGX distribution of visible normals
-                            const f3    vh    = normalize(mk3(a * lv.x, a * lv.y, lv.z));
-                            const float lensq = vh.x * vh.x + vh.y * vh.y;
-                            const f3    t1v   = lensq > 0.0f ? mk3(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk3(1, 0, 0);
-                            const f3    t2v   = cross(vh, t1v);
-                            const float rr    = sqrtf(r.a);
-                            const float phi   = kTau * r.b;
-                            const float t1    = rr * cosf(phi);
-                            float       t2    = rr * sinf(phi);
-                            const float s     = 0.5f * (1.0f + vh.z);
-                            t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
-                            const float nz2 = fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2);
-                            const f3 nh = t1v * t1 + t2v * t2 + vh * sqrtf(nz2);
-                            const f3 ne = normalize(mk3(a * nh.x, a * nh.y, fmaxf(nh.z, 0.0f)));
-                            const float voh = dot(lv, ne);
-                            const f3 ll = ne * (2.0f * voh) - lv;
-                            if (ll.z <= 0.0f || lv.z <= 0.0f) {
-                                absorbed = true;
-                                alb = mk3(0, 0, 0);
-                                nd  = nf;
-                            } else {
-                                nd = normalize(tb * ll.x + bb * ll.y + nf * ll.z);
-                                const float a2 = a * a;
-                                const float g1 = 2.0f * lv.z / (lv.z + sqrtf(a2 + (1.0f - a2) * lv.z * lv.z));
-                                const float g2 = ggx_vis(lv.z, ll.z, a) * 4.0f * lv.z * ll.z;
-                                alb = schlick(voh, f0) * (g2 / g1);
-                            }
-                        }
-                    } else {
-                        const f3 sp = sphere(r.a, r.b);
-                        no    = h.point + nf * 0.0001f;
-                        nd    = normalize(nf + sp);
-                        nee_n = nf;
-                    }
-                }
-                if (bounce == 0 && !rp.nrc.train) aov_write(b, p, rp.mode == kModeCompat ? alb : cc, n, h.dist);
-                const f3 thr_prev = thr;
-                if (rp.mode == kModeCompat) {
-                    thr = thr * alb;
-                    rad = rad + thr * m.emission;
-                } else {
-                    rad = rad + (thr_prev * cc) * m.emission;
-                    thr = thr * alb;
-                }
-                // sun NEE (renderer.cpp:315-354)
-                if (sc.sun_enabled && !(rp.mode == kModePbr && delta)) {
-                    const SunDev &sun = sc.sun;
-                    so = h.point + nee_n * 0.001f;
-                    const f3 nd_sun = -sun.dir;
-                    f3 tb, bb;
-                    onb(nd_sun, tb, bb);
-                    const float ct  = 1.0f - r.d * (1.0f - cosf(sun.size));
-                    const float st  = sqrtf(1.0f - ct * ct);
-                    const float phi = kTau * r.c;
-                    const f3    loc = mk3(cosf(phi) * st, ct, sinf(phi) * st);
-                    sd = tb * loc.x + nd_sun * loc.y + bb * loc.z;
-                    const float pdf  = 1.0f / (kTau * (1.0f - cosf(sun.size)));
-                    const float cosv = fminf(fmaxf(dot(nee_n, sd), 0.0f), 1.0f);
-                    if (cosv > 0.0f) {
-                        const f3 L = sun_radiance(sun, sd);
-                        if (rp.mode == kModeCompat) {
-                            const f3 tc = (thr * cc) * cosv * L;
-                            scontrib = mk3(tc.x / pdf, tc.y / pdf, tc.z / pdf);
-                        } else if (ggx) {
-                            const f3    wo  = -normalize(d);
-                            const f3    hv  = normalize(wo + sd);
-                            const float nov = dot(nee_n, wo);
-                            if (nov > 0.0f) {
-                                const float dv = ggx_d(dot(nee_n, hv), ar) * ggx_vis(nov, cosv, ar);
-                                scontrib = ((thr_prev * schlick(dot(wo, hv), f0)) * L) * (dv * cosv / pdf);
-                            }
-                        } else {
-                            scontrib = (thr * L) * (cosv / (pdf * kPi));
-                        }
-                        push_shadow = scontrib.x != 0.0f || scontrib.y != 0.0f || scontrib.z != 0.0f ||
-                                      rp.mode == kModeCompat;
-                    }
-                }
-                if (absorbed) {
-                    seg = 0;
-                } else {
-                    push_next = can_continue;
-                }
-                if (rp.nrc.enabled && rp.nrc.footprint_c > 0.0f) {
-                    // pdf of the scattered direction (0 for delta lobes: the footprint does not grow)
-                    float pdf = 0.0f;
-                    if (!delta && !absorbed) {
-                        if (ggx) {
-                            const f3    wo = -normalize(d);
-                            const f3    hv = normalize(wo + nd);
-                            const float nv = fmaxf(dot(nee_n, wo), 1e-4f);
-                            const float a2 = ar * ar;
-                            const float g1 = 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
-                            pdf = ggx_d(dot(nee_n, hv), ar) * g1 / (4.0f * nv);
-                        } else {
-                            pdf = fabsf(dot(rp.mode == kModeCompat ? n : nee_n, nd)) * kInvPi;
-                        }
-                    }
-                    b.nstate[p].y = pdf;
-                }
-            }
+            shade_path(sc, b, rp, k, p, p, rp.sample, bounce, cur, o);
         }
-        // Russian roulette at the top of the next iteration
-        if (push_next && rp.rr_start && bounce + 1 >= rp.rr_start) {
-            const rnd4 r2 = rng4(rp.seed, p, rp.sample, bounce + 1, 1);
-            const float qv = fminf(maxc(thr), 0.95f);
-            if (r2.a >= qv) push_next = false;
-            else thr = thr * (1.0f / qv);
-        }
-        b.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-        b.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
     }
-    const uint32_t nqs = rp.nrc.enabled ? block_append(&b.cnt[kCntNrcQ], do_query) : 0u;
-    if (do_query) {
-        for (int k = 0; k < 13; ++k) b.nq[(size_t)nqs * 13 + k] = nq[k];
+    const uint32_t nqs = (!kMulti && rp.nrc.enabled) ? block_append(&b.cnt[kCntNrcQ], o.do_query) : 0u;
+    if (o.do_query) {
+        for (int k = 0; k < 13; ++k) b.nq[(size_t)nqs * 13 + k] = o.nq[k];
         b.nq_path[nqs] = p;
-        b.nq_thr[nqs]  = make_float4(qthr.x, qthr.y, qthr.z, 0.0f);
+        b.nq_thr[nqs]  = make_float4(o.qthr.x, o.qthr.y, o.qthr.z, 0.0f);
     }
-    const uint32_t ns = block_append(&b.cnt[kCntShadow], push_shadow);
-    if (push_shadow) {
-        put_ray(b.sq, ns, so, sd);
+    const uint32_t ns = block_append(&b.cnt[kCntShadow], o.push_shadow);
+    if (o.push_shadow) {
+        put_ray(b.sq, ns, o.so, o.sd);
         b.spath[ns]    = p;
-        b.scontrib[ns] = make_float4(scontrib.x, scontrib.y, scontrib.z, 0.0f);
+        b.scontrib[ns] = make_float4(o.scontrib.x, o.scontrib.y, o.scontrib.z, 0.0f);
     }
-    const uint32_t nn = block_append(&b.cnt[cur ^ 1u], push_next);
-    if (push_next) {
-        put_ray(b.q[cur ^ 1u], nn, no, nd);
+    const uint32_t nn = block_append(&b.cnt[cur ^ 1u], o.push_next);
+    if (o.push_next) {
+        put_ray(b.q[cur ^ 1u], nn, o.no, o.nd);
         b.qpath[cur ^ 1u][nn] = p;
     }
-    block_add_stat(b.stats, 2, seg);
+    block_add_stat(b.stats, 2, o.seg);
 }
 
 __global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
@@ -528,10 +558,22 @@ __global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
     b.rad[p] = r;
 }
 
+// True if the shadow ray k stopped on a transparent texel (colour.w == 0); o2 = where it continues.
+__device__ __forceinline__ bool alpha_passes(const SceneDev &sc, const PathBuffers &b, uint32_t k, f3 o, f3 d,
+                                             uint32_t tri, f3 &o2)
+{
+    const HitRec       h   = hit_record(sc, o, d, b.shits.t[k], tri, b.shits.u[k], b.shits.v[k]);
+    const MaterialDev &m   = sc.mats[sc.mat_idx[h.prim]];
+    const float4       col = mat_colour(sc, m, h.u, h.v);
+    o2 = h.point + d * 0.1f;
+    return col.w == 0.0f;
+}
+
 // Alpha-aware shadow step (renderer.cpp:330-345): after a closest-hit trace of the shadow
 // queue, escaped rays add their contribution, rays stopped by an alpha surface continue
 // from hit + 0.1*dir (compacted into the front of the same queue), others are blocked.
-__global__ __launch_bounds__(256) void k_shadow_alpha(SceneDev sc, PathBuffers b)
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_shadow_alpha(SceneDev sc_one, PathBuffers b, const JobDev *jobs)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.cnt[kCntShadow];
@@ -552,13 +594,8 @@ __global__ __launch_bounds__(256) void k_shadow_alpha(SceneDev sc, PathBuffers b
             r.z += c.z;
             b.rad[p] = r;
         } else {
-            const HitRec       h   = hit_record(sc, o, d, b.shits.t[k], tri, b.shits.u[k], b.shits.v[k]);
-            const MaterialDev &m   = sc.mats[sc.mat_idx[h.prim]];
-            const float4       col = mat_colour(sc, m, h.u, h.v);
-            if (col.w == 0.0f) {
-                cont = true;
-                o2   = h.point + d * 0.1f;
-            }
+            if constexpr (kMulti) cont = alpha_passes(jobs[b.pjob[p]].sc, b, k, o, d, tri, o2);
+            else cont = alpha_passes(sc_one, b, k, o, d, tri, o2);
         }
     }
     // continuation rays go to a staging count, then the host swaps them in
@@ -711,6 +748,85 @@ __global__ __launch_bounds__(256) void k_nrc_targets(PathBuffers b, uint32_t rec
     }
 }
 
+// ---- batched rendering ----------------------------------------------------------------
+
+// Segment of the current chunk that holds `key` (a path slot, or an accumulation thread
+// when by_acc): segments are stored in increasing path0 / acc0 order.
+__device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t seg0, uint32_t nseg, uint32_t key,
+                                             bool by_acc)
+{
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid   = (lo + hi) / 2;
+        const uint32_t start = by_acc ? segs[seg0 + mid].acc0 : segs[seg0 + mid].path0;
+        if (start <= key) lo = mid;
+        else hi = mid;
+    }
+    return seg0 + lo;
+}
+
+// Camera rays of every path of the current chunk (renderer.cpp:260-263 per job), the same
+// Philox stream as the single-scene renderer: (job seed, pixel, sample, jitter).
+__global__ __launch_bounds__(256) void k_batch_generate(BatchDev bd, PathBuffers b)
+{
+    const BatchChunk ch   = bd.chunks[*bd.cursor];
+    const uint32_t   p    = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool       live = p < ch.npaths && p < b.P;
+    if (live) {
+        const BatchSeg sg     = bd.segs[find_seg(bd.segs, ch.seg0, ch.nseg, p, false)];
+        const uint32_t local  = p - sg.path0;
+        const uint32_t si     = local / sg.npix;
+        const uint32_t pix    = sg.pix0 + (local - si * sg.npix);
+        const JobDev  &J      = bd.jobs[sg.job];
+        const uint32_t sample = J.rp.sample + sg.s0 + si;
+        const uint32_t x = pix % J.W, y = pix / J.W;
+        const rnd4     j = rng4(J.rp.seed, pix, sample, kJitterBounce, 0);
+        f3 o, d;
+        camera_ray(J.rp.cam, ((float)x + j.a) / (float)J.W, ((float)y + j.b) / (float)J.H, J.rp.aspect, o, d);
+        put_ray(b.q[0], p, o, d);
+        b.qpath[0][p] = p;
+        b.thr[p]      = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        b.rad[p]      = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.pjob[p]     = sg.job;
+        b.ppix[p]     = pix;
+        b.psamp[p]    = sample;
+    }
+    if (p == 0) b.cnt[kCntQ0] = min(ch.npaths, b.P);
+    block_add_stat(b.stats, 2, live ? 1u : 0u);
+}
+
+// raw[pixel] += radiance of each of the segment's passes, in pass order: the same sums,
+// in the same order, as one accumulate per pass in the single-scene renderer.
+__global__ __launch_bounds__(256) void k_batch_accumulate(BatchDev bd, PathBuffers b)
+{
+    const BatchChunk ch = bd.chunks[*bd.cursor];
+    const uint32_t   t  = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ch.nacc) return;
+    const BatchSeg sg    = bd.segs[find_seg(bd.segs, ch.seg0, ch.nseg, t, true)];
+    const uint32_t local = t - sg.acc0;
+    const uint32_t pix   = sg.pix0 + local;
+    const JobDev  &J     = bd.jobs[sg.job];
+    const uint32_t x = pix % J.W, y = pix / J.W;
+    float *raw = J.raw + 3 * ((size_t)(J.W - 1 - x) + (size_t)(J.H - 1 - y) * J.W);
+    float  r0 = raw[0], r1 = raw[1], r2 = raw[2];
+    for (uint32_t s = 0; s < sg.nsamp; ++s) {
+        const size_t path = (size_t)sg.path0 + (size_t)s * sg.npix + local;
+        if (path >= b.P) break;
+        const float4 r = b.rad[path];
+        r0 += r.x;
+        r1 += r.y;
+        r2 += r.z;
+    }
+    raw[0] = r0;
+    raw[1] = r1;
+    raw[2] = r2;
+}
+
+__global__ void k_batch_advance(uint32_t *cursor)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cursor += 1u;
+}
+
 }  // namespace
 
 static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -728,7 +844,7 @@ hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)
 hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
                         uint32_t cur, hipStream_t s)
 {
-    k_shade<<<blocks(b.P), 256, 0, s>>>(sc, b, rp, bounce, cur);
+    k_shade<false><<<blocks(b.P), 256, 0, s>>>(sc, b, rp, nullptr, bounce, cur);
     return hipGetLastError();
 }
 hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s)
@@ -743,7 +859,7 @@ hipError_t launch_shadow_resolve(const PathBuffers &b, hipStream_t s)
 }
 hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hipStream_t s)
 {
-    k_shadow_alpha<<<blocks(b.P), 256, 0, s>>>(sc, b);
+    k_shadow_alpha<false><<<blocks(b.P), 256, 0, s>>>(sc, b, nullptr);
     k_shadow_compact<<<blocks(b.P), 256, 0, s>>>(b);
     k_shadow_swap<<<1, 64, 0, s>>>(b);
     return hipGetLastError();
@@ -778,6 +894,38 @@ hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, u
                               float *ngx, float *ngy, float *ngz, hipStream_t s)
 {
     if (n) k_hits_to_api<<<blocks(n), 256, 0, s>>>(sc, h, n, prim, geom, ngx, ngy, ngz);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_generate(const BatchDev &bd, const PathBuffers &b, hipStream_t s)
+{
+    k_batch_generate<<<blocks(b.P), 256, 0, s>>>(bd, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_multi(const BatchDev &bd, const PathBuffers &b, uint32_t bounce, uint32_t cur, hipStream_t s)
+{
+    k_shade<true><<<blocks(b.P), 256, 0, s>>>(SceneDev{}, b, RenderParams{}, bd.jobs, bounce, cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow_alpha_step_multi(const BatchDev &bd, const PathBuffers &b, hipStream_t s)
+{
+    k_shadow_alpha<true><<<blocks(b.P), 256, 0, s>>>(SceneDev{}, b, bd.jobs);
+    k_shadow_compact<<<blocks(b.P), 256, 0, s>>>(b);
+    k_shadow_swap<<<1, 64, 0, s>>>(b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uint32_t max_acc, hipStream_t s)
+{
+    k_batch_accumulate<<<blocks(max_acc), 256, 0, s>>>(bd, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s)
+{
+    k_batch_advance<<<1, 64, 0, s>>>(cursor);
     return hipGetLastError();
 }
 

@@ -12,8 +12,8 @@
 //     an XCD that drains its slice steals from the others.
 //   * refill: when 16 or more lanes of a wave are idle, one lane reserves a run of rays
 //     for all idle lanes with a single atomic; lanes take their ray by ballot rank.
-//   * per-lane short stack in LDS (entry-major, conflict-free), spilled 16 entries at a
-//     time to a per-lane global region when full.  Entries carry the child's entry
+//   * per-lane short stack in LDS (entry-major, conflict-free; 8, 16 or 32 entries), spilled
+//     a full stack at a time to a per-lane global region when full.  Entries carry the child's entry
 //     distance so popped nodes beyond the current hit are skipped.
 //   * children are ordered by entry distance with a sorting network; the nearest is
 //     visited next and the others pushed far-to-near.
@@ -187,8 +187,9 @@ struct NodeOps<8> {
     }
 };
 
+template <uint32_t S>
 struct Stack {
-    uint32_t *s_ref;    // LDS [kStackLds][kTraceBlock]
+    uint32_t *s_ref;    // LDS [S][kTraceBlock]
     float    *s_dist;
     uint32_t *g;        // global spill, entry-major over all lanes of the grid: [k][2][nlanes]
     uint32_t  nlanes, gl, tid;
@@ -198,18 +199,18 @@ struct Stack {
 
     __device__ __forceinline__ void push(uint32_t ref, float dist)
     {
-        if (sp == kStackLds) {
-            if (gsp + kStackLds > kSpillDepth) {
+        if (sp == S) {
+            if (gsp + S > kSpillDepth) {
                 // cannot happen for trees admitted at commit; never write out of bounds
                 *err = 1u;
                 return;
             }
-            for (uint32_t k = 0; k < kStackLds; ++k) {
+            for (uint32_t k = 0; k < S; ++k) {
                 const size_t e = (size_t)(gsp + k);
                 g[(e * 2 + 0) * nlanes + gl] = s_ref[k * kTraceBlock + tid];
                 g[(e * 2 + 1) * nlanes + gl] = __float_as_uint(s_dist[k * kTraceBlock + tid]);
             }
-            gsp += kStackLds;
+            gsp += S;
             sp = 0;
         }
         s_ref[sp * kTraceBlock + tid]  = ref;
@@ -222,13 +223,13 @@ struct Stack {
         for (;;) {
             if (sp == 0) {
                 if (gsp == 0) return kEmpty;
-                gsp -= kStackLds;
-                for (uint32_t k = 0; k < kStackLds; ++k) {
+                gsp -= S;
+                for (uint32_t k = 0; k < S; ++k) {
                     const size_t e = (size_t)(gsp + k);
                     s_ref[k * kTraceBlock + tid]  = g[(e * 2 + 0) * nlanes + gl];
                     s_dist[k * kTraceBlock + tid] = __uint_as_float(g[(e * 2 + 1) * nlanes + gl]);
                 }
-                sp = kStackLds;
+                sp = S;
             }
             --sp;
             const float d = s_dist[sp * kTraceBlock + tid];
@@ -239,18 +240,20 @@ struct Stack {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-template <int W, bool kAny>
+// kMulti: rays of many scenes in one launch (batched rendering); each lane takes its
+// ray's scene (nodes, triangles, root) when it fetches the ray.
+template <int W, bool kAny, bool kMulti, uint32_t S>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ, uint32_t *spill)
 {
-    __shared__ uint32_t s_ref[kStackLds * kTraceBlock];
-    __shared__ float    s_dist[kStackLds * kTraceBlock];
+    __shared__ uint32_t s_ref[S * kTraceBlock];
+    __shared__ float    s_dist[S * kTraceBlock];
     const uint32_t n = *q.count;
     if (n == 0) return;
     const uint32_t lane  = lane_id();
     const uint32_t xcd   = blockIdx.x % kXcds;
     const uint32_t slice = (n + kXcds - 1) / kXcds;
 
-    Stack st;
+    Stack<S> st;
     st.s_ref  = s_ref;
     st.s_dist = s_dist;
     st.g      = spill;
@@ -261,6 +264,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
     st.err = q.error;
 
     RayState r;
+    const void   *nodes = sc.nodes;
+    const float4 *tris  = sc.tri;
     uint32_t rid = 0, cur = kEmpty;
     bool     active = false, exhausted = false;
     uint32_t next_slice = 0;   // slices tried by this wave (wave-uniform)
@@ -305,9 +310,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                 r.best_prim = kInvalidRef;
                 r.bu = r.bv = 0.0f;
                 st.sp = st.gsp = 0;
-                const bool valid = sc.ntri > 0 && r.tn <= r.tmax && r.tmax == r.tmax && r.tn == r.tn &&
+                uint32_t root = sc.root, ntri = sc.ntri;
+                if constexpr (kMulti) {
+                    const TraceScene ts = q.scenes[q.path_job[q.ray_path[rid]]];
+                    nodes = ts.nodes;
+                    tris  = ts.tri;
+                    root  = ts.root;
+                    ntri  = ts.ntri;
+                }
+                const bool valid = ntri > 0 && r.tn <= r.tmax && r.tmax == r.tmax && r.tn == r.tn &&
                                    (r.d.x != 0.0f || r.d.y != 0.0f || r.d.z != 0.0f);
-                cur    = valid ? sc.root : kEmpty;
+                cur    = valid ? root : kEmpty;
                 active = true;
             }
             idle  = __ballot(!active);
@@ -323,7 +336,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
             if (__ballot(inner) == 0) break;
             if (inner) {
                 uint32_t key[W], ref[W];
-                const int nh = NodeOps<W>::expand(sc.nodes, cur, r, key, ref);
+                const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
                 if (nh == 0) {
                     cur = st.pop(r.tmax);
                 } else {
@@ -345,9 +358,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                 bool any = false;
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const size_t  ti = (size_t)(first + k);
-                    const float4  a  = sc.tri[3 * ti + 0];
-                    const float4  b  = sc.tri[3 * ti + 1];
-                    const float4  c  = sc.tri[3 * ti + 2];
+                    const float4  a  = tris[3 * ti + 0];
+                    const float4  b  = tris[3 * ti + 1];
+                    const float4  c  = tris[3 * ti + 2];
                     float t, u, v;
                     if (wt_test(r.wt, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), r.o, r.tn, r.tmax,
                                 t, u, v)) {
@@ -401,20 +414,56 @@ uint64_t trace_spill_bytes(uint32_t grid)
     return (uint64_t)grid * kTraceBlock * kSpillDepth * 2u * sizeof(uint32_t);
 }
 
-hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t, uint32_t *spill, uint32_t grid,
-                                hipStream_t s)
+namespace {
+
+// Instantiates k_trace for the BVH width and LDS stack depth of the launch.
+template <bool kAny, bool kMulti>
+hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev &sc, const TraceQueue &q, uint8_t *occ,
+                          uint32_t *spill, hipStream_t s)
 {
-    if (sc.width == 4) k_trace<4, false><<<grid, kTraceBlock, 0, s>>>(sc, q, nullptr, spill);
-    else k_trace<8, false><<<grid, kTraceBlock, 0, s>>>(sc, q, nullptr, spill);
+    const dim3 g(tc.grid), b(kTraceBlock);
+#define CRT_TRACE(Wd, Sd) k_trace<Wd, kAny, kMulti, Sd><<<g, b, 0, s>>>(sc, q, occ, spill)
+    if (width == 4) {
+        if (tc.stack == 8) CRT_TRACE(4, 8);
+        else if (tc.stack == 32) CRT_TRACE(4, 32);
+        else CRT_TRACE(4, 16);
+    } else {
+        if (tc.stack == 8) CRT_TRACE(8, 8);
+        else if (tc.stack == 32) CRT_TRACE(8, 32);
+        else CRT_TRACE(8, 16);
+    }
+#undef CRT_TRACE
     return hipGetLastError();
 }
 
-hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint32_t, uint8_t *occ, uint32_t *spill,
-                            uint32_t grid, hipStream_t s)
+}  // namespace
+
+hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
+                                hipStream_t s)
 {
-    if (sc.width == 4) k_trace<4, true><<<grid, kTraceBlock, 0, s>>>(sc, q, occ, spill);
-    else k_trace<8, true><<<grid, kTraceBlock, 0, s>>>(sc, q, occ, spill);
-    return hipGetLastError();
+    return dispatch_trace<false, false>(sc.width, tc, sc, q, nullptr, spill, s);
+}
+
+hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
+                            const TraceConfig &tc, hipStream_t s)
+{
+    return dispatch_trace<true, false>(sc.width, tc, sc, q, occ, spill, s);
+}
+
+hipError_t launch_trace_closest_multi(uint32_t width, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
+                                      hipStream_t s)
+{
+    SceneDev none{};
+    none.ntri = 1;   // per-ray scenes decide
+    return dispatch_trace<false, true>(width, tc, none, q, nullptr, spill, s);
+}
+
+hipError_t launch_trace_any_multi(uint32_t width, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
+                                  const TraceConfig &tc, hipStream_t s)
+{
+    SceneDev none{};
+    none.ntri = 1;
+    return dispatch_trace<true, true>(width, tc, none, q, occ, spill, s);
 }
 
 }  // namespace crt

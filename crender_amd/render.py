@@ -23,7 +23,7 @@ from ._abi import CrtError, check, fptr, uptr, vptr
 
 class Context:
     def __init__(self, device: int = 0, bvh_width: int = 0, max_leaf: int = 0, phase_timing: bool = False,
-                 trace_waves_per_cu: int = 0):
+                 trace_waves_per_cu: int = 0, trace_stack: int = 0):
         L = A.lib()
         o = A.Options()
         L.crt_options_default(C.byref(o))
@@ -31,6 +31,7 @@ class Context:
         o.bvh_width = bvh_width
         o.max_leaf = max_leaf
         o.trace_waves_per_cu = trace_waves_per_cu
+        o.trace_stack = trace_stack
         o.flags = A.CRT_FLAG_PHASE_TIMING if phase_timing else 0
         h = C.c_void_p()
         check(L.crt_context_create(C.byref(o), C.byref(h)))
@@ -302,12 +303,13 @@ class Renderer:
         check(self._L.crt_renderer_render(self._h, passes))
 
     def set_nrc(self, nrc, term_bounce=1, footprint_c=0.01, train_paths=65536, train_suffix=4, train_steps=4,
-                train=True) -> None:
+                train=True, warmup_passes=0) -> None:
         """Render with the neural radiance cache `nrc` (None turns it off)."""
         o = A.NrcRenderOptions()
         self._L.crt_nrc_render_options_default(C.byref(o))
         o.term_bounce, o.footprint_c, o.train_paths = term_bounce, footprint_c, train_paths
         o.train_suffix, o.train_steps, o.train = train_suffix, train_steps, 1 if train else 0
+        o.warmup_passes = warmup_passes
         check(self._L.crt_renderer_set_nrc(self._h, nrc.handle if nrc is not None else None, C.byref(o)))
 
     def reset(self) -> None:
@@ -434,3 +436,45 @@ class Nrc:
             self.close()
         except Exception:
             pass
+
+
+def _job_array(jobs, outs=None):
+    n = len(jobs)
+    arr = (A.Job * n)()
+    for i, j in enumerate(jobs):
+        W, H = int(j["width"]), int(j["height"])
+        arr[i].scene = j["scene"].handle
+        arr[i].width, arr[i].height = W, H
+        arr[i].spp, arr[i].max_bounces = int(j["spp"]), int(j["max_bounces"])
+        arr[i].options = A.RenderOptions(int(j.get("mode", 0)), int(j.get("rr_start", 0)), int(j.get("seed", 0)),
+                                         int(j.get("sample_offset", 0)))
+        if outs is not None:
+            arr[i].rgb_sum = outs[i].ctypes.data
+        arr[i].memory = 0
+    return arr
+
+
+def render_batch(ctx: Context, jobs, max_paths: int = 0, use_graph: bool = True):
+    """Renders many jobs together (crt_render_batch).  Each job is a dict with keys scene,
+    width, height, spp, max_bounces and optionally mode, rr_start, seed, sample_offset.
+    Returns (list of (H, W, 3) running sums, per-job latency in ms, stats dict)."""
+    L = A.lib()
+    n = len(jobs)
+    outs = [np.zeros((int(j["height"]), int(j["width"]), 3), np.float32) for j in jobs]
+    arr = _job_array(jobs, outs)
+    o = A.BatchOptions()
+    L.crt_batch_options_default(C.byref(o))
+    o.max_paths, o.use_graph = int(max_paths), 1 if use_graph else 0
+    ms = (C.c_double * n)()
+    st = A.BatchStats()
+    check(L.crt_render_batch(ctx.handle, arr, n, C.byref(o), ms, C.byref(st)))
+    return outs, list(ms), {f: getattr(st, f) for f, _ in A.BatchStats._fields_}
+
+
+def autotune(ctx: Context, jobs, tune_width: bool = True) -> dict:
+    """crt_autotune over `jobs` (dicts as for render_batch); returns the chosen shape."""
+    L = A.lib()
+    arr = _job_array(jobs)
+    res = A.TuneResult()
+    check(L.crt_autotune(ctx.handle, arr, len(jobs), 1 if tune_width else 0, C.byref(res)))
+    return {f: getattr(res, f) for f, _ in A.TuneResult._fields_}

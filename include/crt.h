@@ -121,6 +121,7 @@ typedef struct crt_options {
     uint32_t max_leaf;      /* 0 = default (4) */
     uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy */
     uint32_t flags;         /* CRT_FLAG_* */
+    uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 16), 8, 16 or 32 */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
@@ -288,9 +289,66 @@ typedef struct crt_nrc_render_options {
     uint32_t train_suffix;  /* default 4 */
     uint32_t train_steps;   /* default 4 */
     uint32_t train;         /* 0 freezes the cache (inference only) */
+    uint32_t warmup_passes; /* first passes after set_nrc only train the cache: their image is
+                               discarded and they do not count as samples (default 0) */
 } crt_nrc_render_options;
 void crt_nrc_render_options_default(crt_nrc_render_options *opt);
 int  crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt);
+
+/* ---- batched rendering (no counterpart in the reference, whose renderer owns one scene) --
+ * Many independent jobs -- a committed scene at its own resolution, sample count and depth
+ * -- render together: every traversal launch carries rays of all jobs, and one pass of
+ * the wavefront (generate, per bounce extend / shade / shadow, accumulate) is replayed from
+ * a hipGraph until all jobs are done.  Each job's result equals, bit for bit, what a
+ * crt_renderer with the same scene, resolution (set_resolution, so aspect = W/H), bounces
+ * and options accumulates over the same number of passes (renderer.cpp:116-144,258-384). */
+typedef struct crt_job {
+    crt_scene         *scene;        /* committed; all scenes of a batch share one BVH width */
+    uint32_t           width, height;
+    uint32_t           spp;          /* passes, >= 1 */
+    uint32_t           max_bounces;  /* >= 1 */
+    crt_render_options options;      /* mode, rr_start, seed, sample_offset */
+    float             *rgb_sum;      /* out: W*H*3 FP32 running sum, flipped like _raw_buffer */
+    int32_t            memory;       /* crt_memory of rgb_sum */
+} crt_job;
+
+typedef struct crt_batch_options {
+    uint32_t max_paths;   /* path slots per replay (wavefront size); 0 = the context's tuned value */
+    uint32_t use_graph;   /* 1 (default): replay a captured hipGraph; 0: launch kernels directly */
+} crt_batch_options;
+
+typedef struct crt_batch_stats {
+    double   ms_total;          /* first launch to the end of the last job */
+    double   rays_per_second;   /* (extension + shadow rays) / ms_total */
+    uint64_t extension_rays, shadow_rays;
+    uint64_t total_rays;        /* reference-style count: 1 + loop iterations per path */
+    uint32_t replays;           /* passes of the wavefront (graph launches) */
+    uint32_t max_paths;
+    uint32_t used_graph;
+    uint32_t bvh_width;
+} crt_batch_stats;
+
+/* Autotuner of the traversal and wavefront shape for a workload: times short batches of
+ * `jobs` (every job at 1 spp, outputs discarded) over BVH width {4, 8} (CRT_TUNE_WIDTH:
+ * re-commits the jobs' scenes), traversal blocks per CU {2, 3, 4, 5}, LDS stack depth
+ * {8, 16, 32} and wavefront size {1, 2, 4, 8} M paths, one parameter at a time, and keeps
+ * the fastest on the context (used by later batches and renderers).  The context must be
+ * idle. */
+#define CRT_TUNE_WIDTH 1u
+typedef struct crt_tune_result {
+    uint32_t bvh_width;
+    uint32_t trace_blocks_per_cu;
+    uint32_t trace_stack;
+    uint32_t max_paths;
+    double   rays_per_second;   /* of the fastest trial */
+    uint32_t trials;
+} crt_tune_result;
+int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t flags, crt_tune_result *out);
+
+void crt_batch_options_default(crt_batch_options *opt);
+/* job_ms (optional, njobs entries): ms from the batch start to the end of each job. */
+int  crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, const crt_batch_options *opt,
+                      double *job_ms, crt_batch_stats *stats);
 
 /* camera.cpp:54-68: matrix = T(position) * Ry(rot.x) * Rx(rot.y) * Rz(rot.z) */
 void crt_camera_default(crt_camera *cam);   /* camera.h:21: position (5,5,0), fov 75, identity */
