@@ -54,7 +54,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     for ex_src in sorted((PKG / "examples").glob("*.cpp")):
         exe = PKG / "examples" / ex_src.stem

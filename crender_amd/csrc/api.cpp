@@ -78,6 +78,11 @@ struct ModelHost {
 
 }  // namespace
 
+namespace crt {
+// Error reporting for the other host sources of the library (same thread-local message).
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace crt
+
 struct crt_context {
     int         device = 0;
     crt_options opt{};

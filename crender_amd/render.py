@@ -238,6 +238,53 @@ class Scene:
             pass
 
 
+class ObjModel:
+    """crt_obj_load (asset_loader::load_model, asset_loader.cpp:182-303): a parsed OBJ/MTL
+    model.  Arrays are copies; add_to() hands the library's own view to a scene."""
+
+    def __init__(self, path, folder=None):
+        L = A.lib()
+        h = C.c_void_p()
+        check(L.crt_obj_load(str(path).encode(), None if folder is None else str(folder).encode(), C.byref(h)))
+        self._h, self._L = h, L
+        md = A.ModelData()
+        check(L.crt_obj_model(h, C.byref(md)))
+        self.name = md.name.decode()
+        self.warnings = L.crt_obj_warnings(h).decode()
+        nv, nt, ntc = md.num_vertices, md.num_triangles, md.num_texture_coords
+        self.vertices = np.ctypeslib.as_array(md.vertices, (nv, 3)).copy() if nv else np.zeros((0, 3), np.float32)
+        self.vertex_indices = (np.ctypeslib.as_array(md.vertex_indices, (nt, 3)).copy() if nt
+                               else np.zeros((0, 3), np.uint32))
+        self.material_indices = (np.ctypeslib.as_array(md.material_indices, (nt,)).copy() if nt
+                                 else np.zeros(0, np.uint32))
+        self.texture_coords = np.ctypeslib.as_array(md.texture_coords, (ntc, 2)).copy() if ntc else None
+        self.texture_indices = (np.ctypeslib.as_array(md.texture_indices, (nt, 3)).copy()
+                                if ntc and nt else None)
+        self.materials = [{"type": m.type, "colour": tuple(m.colour), "emission": m.emission, "tex": m.tex}
+                          for m in (md.materials[i] for i in range(md.num_materials))]
+        self.textures = [np.ctypeslib.as_array(md.textures[i].rgba,
+                                               (md.textures[i].height, md.textures[i].width, 4)).copy()
+                         for i in range(md.num_textures)]
+
+    def add_to(self, scene: "Scene") -> int:
+        md = A.ModelData()
+        check(self._L.crt_obj_model(self._h, C.byref(md)))
+        mid = C.c_uint32()
+        check(self._L.crt_scene_add_model(scene.handle, C.byref(md), C.byref(mid)))
+        return mid.value
+
+    def close(self):
+        if self._h:
+            self._L.crt_obj_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def scene_from_spec(ctx: Context, spec, commit: bool = True) -> Scene:
     """Loads a SceneSpec into a Scene: model, sky, sun, camera; then commits."""
     sc = Scene(ctx)

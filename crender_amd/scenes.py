@@ -362,3 +362,29 @@ def terrain(grid: int = 2048, spheres: int = 20_000, seed: int = 3, res=(1920, 1
     return SceneSpec(name=f"terrain-{grid}-{spheres}-{seed}", vertices=verts, vertex_indices=faces.reshape(-1),
                      material_indices=mi, materials=mats, sky=sky, sun_enabled=True, sun=None, camera=cam,
                      res=tuple(res), spp=64, max_bounces=8, rr_start=3, mode=1)
+
+
+def write_obj(spec: SceneSpec, path) -> None:
+    """Writes a scene's model as Wavefront OBJ + MTL (one `usemtl` group per material, Kd =
+    colour).  OBJ carries no emission: the reference's loader sets every material's emission
+    to 0 (asset_loader.cpp:233), so emitters are restored with Scene.set_materials."""
+    from pathlib import Path
+
+    path = Path(path)
+    mtl = path.with_suffix(".mtl")
+    with open(mtl, "w") as f:
+        for i, m in enumerate(spec.materials):
+            c = m.colour
+            f.write(f"newmtl m{i}\nKd {c[0]:.9g} {c[1]:.9g} {c[2]:.9g}\n\n")
+    tris = np.asarray(spec.vertex_indices, dtype=np.int64).reshape(-1, 3)
+    mi = np.asarray(spec.material_indices, dtype=np.int64)
+    with open(path, "w") as f:
+        f.write(f"# {spec.name}\nmtllib {mtl.name}\n")
+        for v in np.asarray(spec.vertices, dtype=np.float32):
+            f.write(f"v {v[0]:.9g} {v[1]:.9g} {v[2]:.9g}\n")
+        cur = None
+        for t, m in zip(tris, mi):
+            if m != cur:
+                f.write(f"usemtl m{m}\n")
+                cur = m
+            f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")

@@ -295,6 +295,19 @@ typedef struct crt_nrc_render_options {
 void crt_nrc_render_options_default(crt_nrc_render_options *opt);
 int  crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt);
 
+/* ---- scene-load format: Wavefront OBJ + MTL --------------------------------------------
+ * asset_loader::load_model (asset_loader.cpp:182-303): triangulates polygons as
+ * tinyobjloader does (tinobj.h:1393-1714), keeps Kd as the colour and map_Kd as the
+ * texture (PNG / PPM, flipped vertically) of a smooth material with emission 0.  Faces
+ * without a material and corners without a texture coordinate get an appended default
+ * (the reference would index out of bounds).  folder NULL: the OBJ file's directory.
+ * The model view stays valid until crt_obj_free; pass it to crt_scene_add_model. */
+typedef struct crt_obj crt_obj;
+int         crt_obj_load(const char *file, const char *folder, crt_obj **out);
+int         crt_obj_model(const crt_obj *obj, crt_model_data *model);
+const char *crt_obj_warnings(const crt_obj *obj);
+int         crt_obj_free(crt_obj *obj);
+
 /* ---- batched rendering (no counterpart in the reference, whose renderer owns one scene) --
  * Many independent jobs -- a committed scene at its own resolution, sample count and depth
  * -- render together: every traversal launch carries rays of all jobs, and one pass of
