@@ -194,6 +194,7 @@ SIGNATURES = {
     "crt_nrc_train_step": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_float), C.c_int]),
     "crt_nrc_render_options_default": (None, [C.POINTER(NrcRenderOptions)]),
     "crt_renderer_set_nrc": (C.c_int, [P, P, C.POINTER(NrcRenderOptions)]),
+    "crt_image_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
     "crt_obj_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "crt_obj_model": (C.c_int, [P, C.POINTER(ModelData)]),
     "crt_obj_warnings": (C.c_char_p, [P]),

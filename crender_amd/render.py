@@ -238,6 +238,18 @@ class Scene:
             pass
 
 
+IMAGE_FORMATS = {"png": 0, "jpg": 1, "exr": 2, "hdr": 3, "pfm": 4}
+
+
+def write_image(path, rgba, fmt=None) -> None:
+    """crt_image_write of an (H, W, 4) float image; the format defaults to the suffix."""
+    img = np.ascontiguousarray(rgba, dtype=np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError("expected an (H, W, 4) image")
+    fmt = (fmt or str(path).rsplit(".", 1)[-1]).lower()
+    check(A.lib().crt_image_write(str(path).encode(), IMAGE_FORMATS[fmt], fptr(img), img.shape[1], img.shape[0]))
+
+
 class ObjModel:
     """crt_obj_load (asset_loader::load_model, asset_loader.cpp:182-303): a parsed OBJ/MTL
     model.  Arrays are copies; add_to() hands the library's own view to a scene."""
@@ -390,6 +402,10 @@ class Renderer:
 
     def current_depths(self) -> np.ndarray:
         return self._read("depths")
+
+    def export(self, path, fmt=None, buffer: str = "progress") -> None:
+        """asset_loader::export_framebuffer (asset_loader.cpp:348-377) of a renderer buffer."""
+        write_image(path, self._read(buffer), fmt)
 
     def raw(self) -> np.ndarray:
         """_raw_buffer: (H, W, 3) running sums, flipped in x and y."""

@@ -308,6 +308,15 @@ int         crt_obj_model(const crt_obj *obj, crt_model_data *model);
 const char *crt_obj_warnings(const crt_obj *obj);
 int         crt_obj_free(crt_obj *obj);
 
+/* ---- output formats: asset_loader::export_framebuffer (asset_loader.cpp:89-178,348-377) --
+ * Writes a W*H RGBA FP32 image (row 0 first, as crt_renderer_read returns it): PNG =
+ * RGBA8 of min(v*255, 255); EXR = half-float B, G, R, uncompressed; HDR = Radiance RGBE of
+ * pow(v, 2.2); PFM = FP32 RGB (no counterpart; lossless HDR sums).  JPG is reported as
+ * CRT_ERR_UNSUPPORTED.  The reference chooses a free name under ./out/; here the caller
+ * gives the path. */
+enum crt_image_format { CRT_IMAGE_PNG = 0, CRT_IMAGE_JPG = 1, CRT_IMAGE_EXR = 2, CRT_IMAGE_HDR = 3, CRT_IMAGE_PFM = 4 };
+int crt_image_write(const char *path, int format, const float *rgba, uint32_t width, uint32_t height);
+
 /* ---- batched rendering (no counterpart in the reference, whose renderer owns one scene) --
  * Many independent jobs -- a committed scene at its own resolution, sample count and depth
  * -- render together: every traversal launch carries rays of all jobs, and one pass of
