@@ -135,7 +135,7 @@ class Rays(C.Structure):
 
 class Hits(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in
-                ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z")]
+                ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z", "inst_id")]
 
 
 P = C.c_void_p

@@ -96,6 +96,7 @@ struct crt_context {
     float      *st_ray[8] = {};
     float      *st_t = nullptr, *st_u = nullptr, *st_v = nullptr;
     uint32_t   *st_tri = nullptr, *st_prim = nullptr, *st_geom = nullptr;
+    uint32_t   *st_inst = nullptr, *st_inst_out = nullptr;
     float      *st_ng[3] = {};
     uint8_t    *st_occ = nullptr;
     uint32_t   *st_cnt = nullptr;   // [0] count, [8..15] work counters, [16] error
@@ -124,6 +125,9 @@ struct crt_scene {
     float4                *d_texels = nullptr;
     TextureDev            *d_texs = nullptr;
     uint32_t              *d_model_first = nullptr;
+    InstDev               *d_inst = nullptr;
+    uint32_t              *d_tlas_inst = nullptr;
+    uint32_t               ninst = 0;
     crt_bvh_info           info{};
     uint64_t               hash = 0;
     uint32_t               users = 0;  // renderers referencing this scene
@@ -205,6 +209,42 @@ bool is_identity(const float *m)
     return true;
 }
 
+// World->object transform of a column-major affine object->world matrix m, as a
+// row-major 3x4 in double; false if singular.
+bool affine_inverse(const float *m, double *inv)
+{
+    const double a[3][3] = {{m[0], m[4], m[8]}, {m[1], m[5], m[9]}, {m[2], m[6], m[10]}};
+    const double t[3]    = {m[12], m[13], m[14]};
+    const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1], c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2],
+                 c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+    if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return false;
+    const double id = 1.0 / det;
+    double r[3][3];
+    r[0][0] = c00 * id;
+    r[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+    r[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+    r[1][0] = c01 * id;
+    r[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+    r[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+    r[2][0] = c02 * id;
+    r[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+    r[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) inv[4 * i + j] = r[i][j];
+        inv[4 * i + 3] = -(r[i][0] * t[0] + r[i][1] * t[1] + r[i][2] * t[2]);
+    }
+    return true;
+}
+
+// true unless every model has exactly the identity instance
+bool scene_instanced(const crt_scene *scene)
+{
+    for (const auto &m : scene->models)
+        if (m.instances.size() != 16 || !is_identity(m.instances.data())) return true;
+    return false;
+}
+
 void free_scene_device(crt_scene *s)
 {
     dfree(s->d_nodes);
@@ -215,6 +255,11 @@ void free_scene_device(crt_scene *s)
     dfree(s->d_texels);
     dfree(s->d_texs);
     dfree(s->d_model_first);
+    dfree(s->d_inst);
+    dfree(s->d_tlas_inst);
+    s->d_inst      = nullptr;
+    s->d_tlas_inst = nullptr;
+    s->ninst       = 0;
     s->d_nodes = nullptr;
     s->d_tri = nullptr;
     s->d_mat_idx = nullptr;
@@ -275,6 +320,8 @@ int alloc_path_core(std::vector<void *> &L, PathBuffers &b, size_t P)
     LA(&b.hits.u, P);
     LA(&b.hits.v, P);
     LA(&b.hits.tri, P);
+    LA(&b.hits.inst, P);
+    LA(&b.shits.inst, P);
     LA(&b.shits.t, P);
     LA(&b.shits.u, P);
     LA(&b.shits.v, P);
@@ -321,6 +368,8 @@ int alloc_renderer_buffers(crt_renderer *r)
     RA(&b.hits.u, P);
     RA(&b.hits.v, P);
     RA(&b.hits.tri, P);
+    RA(&b.hits.inst, P);
+    RA(&b.shits.inst, P);
     RA(&b.shits.t, P);
     RA(&b.shits.u, P);
     RA(&b.shits.v, P);
@@ -626,6 +675,8 @@ int stage_reserve(crt_context *ctx, uint32_t n)
     dfree(ctx->st_tri);
     dfree(ctx->st_prim);
     dfree(ctx->st_geom);
+    dfree(ctx->st_inst);
+    dfree(ctx->st_inst_out);
     for (float *&p : ctx->st_ng) dfree(p), p = nullptr;
     dfree(ctx->st_occ);
     dfree(ctx->st_cnt);
@@ -637,6 +688,8 @@ int stage_reserve(crt_context *ctx, uint32_t n)
     HIP_CHECK(dalloc(&ctx->st_tri, cap));
     HIP_CHECK(dalloc(&ctx->st_prim, cap));
     HIP_CHECK(dalloc(&ctx->st_geom, cap));
+    HIP_CHECK(dalloc(&ctx->st_inst, cap));
+    HIP_CHECK(dalloc(&ctx->st_inst_out, cap));
     for (float *&p : ctx->st_ng) HIP_CHECK(dalloc(&p, cap));
     HIP_CHECK(dalloc(&ctx->st_occ, cap));
     HIP_CHECK(dalloc(&ctx->st_cnt, 32));
@@ -673,12 +726,13 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
     HIP_CHECK(hipMemsetAsync(ctx->st_cnt, 0, sizeof(uint32_t) * 32, s));
     HIP_CHECK(hipMemcpyAsync(ctx->st_cnt, &n, sizeof n, hipMemcpyHostToDevice, s));
     if (hits) {
-        HitSoA h;
+        HitSoA h{};
         const bool dev = memory == CRT_MEM_DEVICE;
-        h.t   = dev ? hits->t : ctx->st_t;
-        h.u   = dev && hits->u ? hits->u : ctx->st_u;
-        h.v   = dev && hits->v ? hits->v : ctx->st_v;
-        h.tri = ctx->st_tri;
+        h.t    = dev ? hits->t : ctx->st_t;
+        h.u    = dev && hits->u ? hits->u : ctx->st_u;
+        h.v    = dev && hits->v ? hits->v : ctx->st_v;
+        h.tri  = ctx->st_tri;
+        h.inst = ctx->st_inst;
         TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
         HIP_CHECK(launch_trace_closest(scene->dev, q, ctx->spill, trace_cfg(ctx), s));
         uint32_t *prim = dev ? hits->prim_id : ctx->st_prim;
@@ -686,7 +740,8 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
         float    *ng[3] = {hits->ng_x, hits->ng_y, hits->ng_z};
         float    *ngd[3];
         for (int k = 0; k < 3; ++k) ngd[k] = ng[k] ? (dev ? ng[k] : ctx->st_ng[k]) : nullptr;
-        HIP_CHECK(launch_hits_to_api(scene->dev, h, n, prim, geom, ngd[0], ngd[1], ngd[2], s));
+        uint32_t *inst = hits->inst_id ? (dev ? hits->inst_id : ctx->st_inst_out) : nullptr;
+        HIP_CHECK(launch_hits_to_api(scene->dev, h, n, prim, geom, ngd[0], ngd[1], ngd[2], inst, s));
         if (!dev) {
             HIP_CHECK(hipMemcpyAsync(hits->t, ctx->st_t, sizeof(float) * n, hipMemcpyDeviceToHost, s));
             if (hits->u) HIP_CHECK(hipMemcpyAsync(hits->u, ctx->st_u, sizeof(float) * n, hipMemcpyDeviceToHost, s));
@@ -694,6 +749,8 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
             HIP_CHECK(hipMemcpyAsync(hits->prim_id, ctx->st_prim, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
             if (hits->geom_id)
                 HIP_CHECK(hipMemcpyAsync(hits->geom_id, ctx->st_geom, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+            if (hits->inst_id)
+                HIP_CHECK(hipMemcpyAsync(hits->inst_id, ctx->st_inst_out, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
             for (int k = 0; k < 3; ++k)
                 if (ng[k]) HIP_CHECK(hipMemcpyAsync(ng[k], ctx->st_ng[k], sizeof(float) * n, hipMemcpyDeviceToHost, s));
         }
@@ -768,6 +825,8 @@ int crt_context_destroy(crt_context *ctx)
     dfree(ctx->st_tri);
     dfree(ctx->st_prim);
     dfree(ctx->st_geom);
+    dfree(ctx->st_inst);
+    dfree(ctx->st_inst_out);
     for (float *p : ctx->st_ng) dfree(p);
     dfree(ctx->st_occ);
     dfree(ctx->st_cnt);
@@ -941,8 +1000,16 @@ int crt_scene_set_instances(crt_scene *scene, uint32_t model_id, const float *tr
 {
     if (!scene || (!transforms && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
     if (model_id >= scene->models.size()) return fail(CRT_ERR_INVALID_ARG, "no such model");
-    if (count != 1 || !is_identity(transforms))
-        return fail(CRT_ERR_UNSUPPORTED, "only the single identity instance is supported");
+    for (uint32_t i = 0; i < count; ++i) {
+        const float *m = transforms + 16 * (size_t)i;
+        for (int k = 0; k < 16; ++k)
+            if (!std::isfinite(m[k])) return fail(CRT_ERR_INVALID_ARG, "non-finite instance transform");
+        // column-major affine: bottom row (m[3], m[7], m[11], m[15]) = (0, 0, 0, 1)
+        if (m[3] != 0.0f || m[7] != 0.0f || m[11] != 0.0f || m[15] != 1.0f)
+            return fail(CRT_ERR_INVALID_ARG, "instance transforms must be affine");
+        double inv[12];
+        if (!affine_inverse(m, inv)) return fail(CRT_ERR_INVALID_ARG, "singular instance transform");
+    }
     scene->models[model_id].instances.assign(transforms, transforms + 16 * (size_t)count);
     scene->dirty = true;
     return CRT_OK;
@@ -1018,6 +1085,159 @@ int crt_scene_set_camera(crt_scene *scene, const crt_camera *camera)
     scene->cam = *camera;
     return CRT_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Two-level BVH of a scene with instances (scene.cpp:83-95 x model.cpp:99-126): one BVH
+// per model in object space, one over the instances' world boxes, all nodes in one pool
+// (instance BVH first, root 0) and all triangles in one pool (model k at its first primID).
+int build_two_level(crt_scene *scene, const float *d_corners, const std::vector<uint32_t> &model_first, uint32_t width,
+                    uint32_t max_leaf, hipStream_t s, BuildResult &out, float *bmin, float *bmax, uint32_t &max_stack)
+{
+    const size_t nm = scene->models.size();
+    std::vector<BuildResult> blas(nm);
+    struct Guard {
+        std::vector<BuildResult> &b;
+        void *extra[2] = {nullptr, nullptr};
+        ~Guard()
+        {
+            for (auto &x : b) {
+                dfree(x.nodes);
+                dfree(x.tri);
+            }
+            for (void *p : extra) dfree(p);
+        }
+    } guard{blas};
+    uint32_t max_depth = 0, total_nodes = 0, bvh2 = 0;
+    float    ms = 0.0f;
+    std::vector<InstDev> insts;
+    std::vector<float>   boxes;
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = 3.4e38f;
+        bmax[k] = -3.4e38f;
+    }
+    for (size_t k = 0; k < nm; ++k) {
+        const ModelHost &m = scene->models[k];
+        const uint32_t   n = (uint32_t)(m.corners.size() / 9);
+        if (n == 0 || m.instances.empty()) continue;
+        hipError_t e = build_bvh(d_corners + 9 * (size_t)model_first[k], n, width, max_leaf, s, &blas[k]);
+        if (e == hipSuccess) e = tri_add_prim_base(blas[k].tri, n, model_first[k], s);
+        if (e != hipSuccess) return fail(CRT_ERR_HIP, std::string("model BVH build: ") + hipGetErrorString(e));
+        max_depth = std::max(max_depth, blas[k].depth);
+        ms += blas[k].ms;
+        bvh2 += blas[k].bvh2_nodes;
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) lo[a] = hi[a] = m.corners[a];
+        for (size_t i = 0; i < m.corners.size(); i += 3)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], m.corners[i + a]);
+                hi[a] = std::max(hi[a], m.corners[i + a]);
+            }
+        for (size_t j = 0; j < m.instances.size() / 16; ++j) {
+            const float *M = &m.instances[16 * j];
+            double inv[12];
+            if (!affine_inverse(M, inv)) return fail(CRT_ERR_INVALID_ARG, "singular instance transform");
+            InstDev d{};
+            for (int q = 0; q < 12; ++q) d.inv[q] = (float)inv[q];
+            d.model = (uint32_t)k;
+            d.local = (uint32_t)j;
+            insts.push_back(d);
+            // world box of the model box's 8 corners
+            float wl[3] = {3.4e38f, 3.4e38f, 3.4e38f}, wh[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+            for (int c = 0; c < 8; ++c) {
+                const double p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+                for (int a = 0; a < 3; ++a) {
+                    const double w = M[a] * p[0] + M[4 + a] * p[1] + M[8 + a] * p[2] + M[12 + a];
+                    wl[a] = std::min(wl[a], std::nextafter((float)w, -3.4e38f));
+                    wh[a] = std::max(wh[a], std::nextafter((float)w, 3.4e38f));
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                bmin[a] = std::min(bmin[a], wl[a]);
+                bmax[a] = std::max(bmax[a], wh[a]);
+            }
+            // a degenerate "triangle" whose bounds are the world box
+            boxes.insert(boxes.end(), {wl[0], wl[1], wl[2], wh[0], wh[1], wh[2], wl[0], wl[1], wl[2]});
+        }
+    }
+    const uint32_t ninst = (uint32_t)insts.size();
+    if (ninst == 0) {
+        for (int a = 0; a < 3; ++a) bmin[a] = 0.0f, bmax[a] = 1.0f;
+        out = BuildResult{};
+        max_stack = 0;
+        scene->ninst = 0;
+        return CRT_OK;
+    }
+    if (ninst >= (1u << 27)) return fail(CRT_ERR_UNSUPPORTED, "too many instances");
+    float *d_boxes = nullptr;
+    HIP_CHECK(dalloc(&d_boxes, boxes.size()));
+    guard.extra[0] = d_boxes;
+    HIP_CHECK(hipMemcpyAsync(d_boxes, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice, s));
+    BuildResult tl{};
+    hipError_t e = build_bvh(d_boxes, ninst, width, 1, s, &tl);
+    if (e != hipSuccess) return fail(CRT_ERR_HIP, std::string("instance BVH build: ") + hipGetErrorString(e));
+    guard.extra[1] = tl.nodes;
+    // instance of each TLAS leaf position
+    std::vector<float4> tl_tri((size_t)3 * ninst);
+    HIP_CHECK(hipMemcpyAsync(tl_tri.data(), tl.tri, sizeof(float4) * tl_tri.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    dfree(tl.tri);
+    std::vector<uint32_t> tlas_inst(ninst);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        uint32_t id;
+        std::memcpy(&id, &tl_tri[3 * (size_t)i].w, 4);
+        tlas_inst[i] = id;
+    }
+    // pools
+    total_nodes = tl.node_count;
+    std::vector<uint32_t> node_base(nm, 0);
+    for (size_t k = 0; k < nm; ++k)
+        if (blas[k].nodes) {
+            node_base[k] = total_nodes;
+            total_nodes += blas[k].node_count;
+        }
+    size_t tri_total = 0;
+    for (const auto &m : scene->models) tri_total += m.corners.size() / 9;
+    void   *pool = nullptr;
+    float4 *tpool = nullptr;
+    HIP_CHECK(hipMalloc(&pool, (size_t)128 * total_nodes));
+    scene->d_nodes = pool;
+    HIP_CHECK(dalloc(&tpool, 3 * std::max<size_t>(tri_total, 1)));
+    scene->d_tri = tpool;
+    HIP_CHECK(hipMemcpyAsync(pool, tl.nodes, (size_t)128 * tl.node_count, hipMemcpyDeviceToDevice, s));
+    for (size_t k = 0; k < nm; ++k) {
+        if (!blas[k].nodes) continue;
+        const uint32_t n = (uint32_t)(scene->models[k].corners.size() / 9);
+        void *dst = static_cast<char *>(pool) + (size_t)128 * node_base[k];
+        HIP_CHECK(hipMemcpyAsync(dst, blas[k].nodes, (size_t)128 * blas[k].node_count, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(bvh_rebase(dst, blas[k].node_count, width, node_base[k], model_first[k], s));
+        HIP_CHECK(hipMemcpyAsync(tpool + 3 * (size_t)model_first[k], blas[k].tri, sizeof(float4) * 3 * n,
+                                 hipMemcpyDeviceToDevice, s));
+    }
+    for (InstDev &d : insts) d.root = node_base[d.model];
+    HIP_CHECK(dalloc(&scene->d_inst, insts.size()));
+    HIP_CHECK(dalloc(&scene->d_tlas_inst, tlas_inst.size()));
+    HIP_CHECK(hipMemcpyAsync(scene->d_inst, insts.data(), sizeof(InstDev) * insts.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(scene->d_tlas_inst, tlas_inst.data(), sizeof(uint32_t) * tlas_inst.size(),
+                             hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    scene->ninst = ninst;
+    out            = BuildResult{};
+    out.node_count = total_nodes;
+    out.bvh2_nodes = bvh2 + tl.bvh2_nodes;
+    out.depth      = tl.depth + max_depth;
+    out.sah        = tl.sah;
+    out.ms         = ms + tl.ms;
+    // instance path: exit marker + leftover instance leaf + both levels' pushes
+    max_stack = (width - 1) * (tl.depth + max_depth) + 3;
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int crt_scene_commit(crt_scene *scene)
 {
@@ -1100,12 +1320,25 @@ int crt_scene_commit(crt_scene *scene)
     if (!corners.empty()) HIP_CHECK(hipMemcpyAsync(d_corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice, s));
     const uint32_t width = ctx->opt.bvh_width ? ctx->opt.bvh_width : 8u;
     const uint32_t max_leaf = ctx->opt.max_leaf ? ctx->opt.max_leaf : 4u;
+    const bool     two_level = scene_instanced(scene);
     BuildResult br{};
-    hipError_t  be = build_bvh(d_corners, ntri, width, max_leaf, s, &br);
-    dfree(d_corners);
-    if (be != hipSuccess) return fail(CRT_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(be));
-    scene->d_nodes = br.nodes;
-    scene->d_tri   = br.tri;
+    uint32_t    max_stack = 0;
+    if (!two_level) {
+        hipError_t be = build_bvh(d_corners, ntri, width, max_leaf, s, &br);
+        dfree(d_corners);
+        if (be != hipSuccess) return fail(CRT_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(be));
+        scene->d_nodes = br.nodes;
+        scene->d_tri   = br.tri;
+        max_stack      = ntri ? (width - 1) * br.depth + 1 : 0;
+    } else {
+        const int st2 = build_two_level(scene, d_corners, model_first, width, max_leaf, s, br, bmin, bmax, max_stack);
+        dfree(d_corners);
+        if (st2) {
+            free_scene_device(scene);
+            return st2;
+        }
+        for (auto &m : scene->models) h = fnv1a(h, m.instances.data(), m.instances.size() * sizeof(float));
+    }
     HIP_CHECK(dalloc(&scene->d_mat_idx, mat_idx.size()));
     HIP_CHECK(dalloc(&scene->d_mats, std::max<size_t>(mats.size(), 1)));
     if (!mat_idx.empty()) HIP_CHECK(hipMemcpyAsync(scene->d_mat_idx, mat_idx.data(), sizeof(uint32_t) * mat_idx.size(), hipMemcpyHostToDevice, s));
@@ -1125,7 +1358,6 @@ int crt_scene_commit(crt_scene *scene)
         HIP_CHECK(hipMemcpyAsync(scene->d_model_first, model_first.data(), sizeof(uint32_t) * model_first.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
 
-    const uint32_t max_stack = ntri ? (width - 1) * br.depth + 1 : 0;
     if (max_stack > kSpillDepth + 8) {   // capacity of the smallest LDS stack configuration
         free_scene_device(scene);
         return fail(CRT_ERR_UNSUPPORTED, "BVH too deep for the traversal stack");
@@ -1157,6 +1389,10 @@ int crt_scene_commit(crt_scene *scene)
     d.nmodels     = (uint32_t)model_first.size();
     d.bmin        = mk3(bmin[0], bmin[1], bmin[2]);
     d.bmax        = mk3(bmax[0], bmax[1], bmax[2]);
+    d.instanced   = two_level ? 1 : 0;
+    d.inst        = scene->d_inst;
+    d.tlas_inst   = scene->d_tlas_inst;
+    d.ninst       = scene->ninst;
     crt_bvh_info &bi = scene->info;
     bi.width           = width;
     bi.node_count      = br.node_count;

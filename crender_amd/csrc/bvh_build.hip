@@ -1003,4 +1003,56 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     return hipSuccess;
 }
 
+namespace {
+
+template <int W>
+__global__ void k_rebase(void *nodes, uint32_t count, uint32_t node_base, uint32_t tri_base)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t *child;
+    uint32_t  nc;
+    if constexpr (W == 4) {
+        Node4 *n = static_cast<Node4 *>(nodes) + i;
+        child    = n->child;
+        nc       = n->nchild;
+    } else {
+        Node8Q *n = static_cast<Node8Q *>(nodes) + i;
+        child     = n->child;
+        nc        = n->meta >> 24;
+    }
+    for (uint32_t j = 0; j < nc && j < (uint32_t)W; ++j) {
+        const uint32_t r = child[j];
+        child[j] = ref_is_leaf(r) ? ref_leaf(leaf_first(r) + tri_base, leaf_count(r)) : r + node_base;
+    }
+}
+
+__global__ void k_add_prim_base(float4 *tri, uint32_t ntri, uint32_t base)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntri) return;
+    float4 a = tri[3 * (size_t)i];
+    a.w      = __uint_as_float(__float_as_uint(a.w) + base);
+    tri[3 * (size_t)i] = a;
+}
+
+}  // namespace
+
+hipError_t bvh_rebase(void *nodes, uint32_t count, uint32_t width, uint32_t node_base, uint32_t tri_base,
+                      hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    const uint32_t b = (count + 255) / 256;
+    if (width == 4) k_rebase<4><<<b, 256, 0, s>>>(nodes, count, node_base, tri_base);
+    else k_rebase<8><<<b, 256, 0, s>>>(nodes, count, node_base, tri_base);
+    return hipGetLastError();
+}
+
+hipError_t tri_add_prim_base(float4 *tri, uint32_t ntri, uint32_t base, hipStream_t s)
+{
+    if (ntri == 0 || base == 0) return hipSuccess;
+    k_add_prim_base<<<(ntri + 255) / 256, 256, 0, s>>>(tri, ntri, base);
+    return hipGetLastError();
+}
+
 }  // namespace crt

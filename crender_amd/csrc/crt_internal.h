@@ -83,6 +83,17 @@ struct TextureDev {
     uint32_t w, h;
 };
 
+// Instance of a model (two-level scenes): world->object affine transform (row-major
+// 3x4) and the model BVH's root in the scene's node pool.
+struct InstDev {
+    float    inv[12];
+    uint32_t root;
+    uint32_t model;
+    uint32_t local;    // index among the model's instances
+    uint32_t pad;
+};
+static_assert(sizeof(InstDev) == 64, "InstDev is 64 bytes");
+
 struct SunDev {
     float size, intensity;
     f3    dir, colour;
@@ -110,6 +121,12 @@ struct SceneDev {
     const uint32_t    *model_first; // first primID of each model (for geomID)
     uint32_t           nmodels;
     f3                 bmin, bmax;  // scene bounding box
+    // two-level scenes (instances): `nodes` holds the instance BVH (root 0) and every model
+    // BVH; its leaves index tlas_inst, which names the instance; `tri` pools all models
+    int32_t            instanced;
+    const InstDev     *inst;
+    const uint32_t    *tlas_inst;
+    uint32_t           ninst;
 };
 
 // SoA ray buffer
@@ -119,6 +136,7 @@ struct RaySoA {
 struct HitSoA {
     float    *t, *u, *v;
     uint32_t *tri;   // leaf-order triangle index or kInvalidRef
+    uint32_t *inst;  // instance of the hit (two-level scenes; may be null)
 };
 
 // ---- builder ------------------------------------------------------------------------
@@ -135,6 +153,11 @@ struct BuildResult {
 // Builds the BVH of ntri triangles (9 floats each, device memory) on stream s.
 hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf,
                      hipStream_t s, BuildResult *out);
+// Moves a wide BVH into a pool: inner child refs += node_base, leaf first += tri_base.
+hipError_t bvh_rebase(void *nodes, uint32_t count, uint32_t width, uint32_t node_base, uint32_t tri_base,
+                      hipStream_t s);
+// Adds `base` to the primID stored in v0.w of every leaf-order triangle.
+hipError_t tri_add_prim_base(float4 *tri, uint32_t ntri, uint32_t base, hipStream_t s);
 
 // ---- traversal ----------------------------------------------------------------------
 struct TraceScene;
@@ -278,6 +301,6 @@ hipError_t launch_shadow_alpha_step_multi(const BatchDev &bd, const PathBuffers 
 hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uint32_t max_acc, hipStream_t s);
 hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s);
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
-                              float *ngx, float *ngy, float *ngz, hipStream_t s);
+                              float *ngx, float *ngy, float *ngz, uint32_t *inst, hipStream_t s);
 
 }  // namespace crt

@@ -186,8 +186,11 @@ struct HitRec {
 };
 
 // model.cpp:5-49 + model.cpp:99-126 for the identity instance
+// inst: instance of the hit in a two-level scene (kInvalidRef otherwise).  The normal is
+// the object-space one, as model.cpp:36-38 returns it for instances; world_normal maps it
+// to world space (inverse transpose), which the pbr estimator uses.
 __device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, float t, uint32_t tri, float bu,
-                                             float bv)
+                                             float bv, uint32_t inst = kInvalidRef, bool world_normal = false)
 {
     HitRec h;
     const float4 a = sc.tri[3 * (size_t)tri + 0];
@@ -200,6 +203,12 @@ __device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, 
     h.dist   = sqrtf(dot(pd, pd));
     const f3 v0 = mk3(a.x, a.y, a.z), v1 = mk3(b.x, b.y, b.z), v2 = mk3(c.x, c.y, c.z);
     h.normal = normalize(cross(v1 - v0, v2 - v0));
+    if (world_normal && inst != kInvalidRef) {
+        const float *m = sc.inst[inst].inv;
+        const f3     n = h.normal;
+        h.normal = normalize(mk3(m[0] * n.x + m[4] * n.y + m[8] * n.z, m[1] * n.x + m[5] * n.y + m[9] * n.z,
+                                 m[2] * n.x + m[6] * n.y + m[10] * n.z));
+    }
     if (sc.uv) {
         const float2 w0 = sc.uv[3 * (size_t)h.prim + 0], w1 = sc.uv[3 * (size_t)h.prim + 1],
                      w2 = sc.uv[3 * (size_t)h.prim + 2];
@@ -285,7 +294,8 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
         if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
         rad = rad + thr * s;
     } else {
-        const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k]);
+        const uint32_t     inst = sc.instanced && b.hits.inst ? b.hits.inst[k] : kInvalidRef;
+        const HitRec       h = hit_record(sc, o, d, b.hits.t[k], tri, b.hits.u[k], b.hits.v[k], inst, rp.mode == kModePbr);
         const MaterialDev &m = sc.mats[sc.mat_idx[h.prim]];
         const float4       col = mat_colour(sc, m, h.u, h.v);
         seg = 1;
@@ -677,7 +687,7 @@ __global__ void k_fill(float4 *p, uint64_t n, float v)
 }
 
 __global__ void k_hits_to_api(SceneDev sc, HitSoA h, uint32_t n, uint32_t *prim, uint32_t *geom, float *ngx,
-                              float *ngy, float *ngz)
+                              float *ngy, float *ngz, uint32_t *inst)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -698,6 +708,7 @@ __global__ void k_hits_to_api(SceneDev sc, HitSoA h, uint32_t n, uint32_t *prim,
     }
     prim[i] = pid;
     if (geom) geom[i] = gid;
+    if (inst) inst[i] = tri == kInvalidRef ? kInvalidRef : (sc.instanced && h.inst ? sc.inst[h.inst[i]].local : 0u);
     if (ngx) ngx[i] = ng.x;
     if (ngy) ngy[i] = ng.y;
     if (ngz) ngz[i] = ng.z;
@@ -891,9 +902,9 @@ hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t
     return hipGetLastError();
 }
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
-                              float *ngx, float *ngy, float *ngz, hipStream_t s)
+                              float *ngx, float *ngy, float *ngz, uint32_t *inst, hipStream_t s)
 {
-    if (n) k_hits_to_api<<<blocks(n), 256, 0, s>>>(sc, h, n, prim, geom, ngx, ngy, ngz);
+    if (n) k_hits_to_api<<<blocks(n), 256, 0, s>>>(sc, h, n, prim, geom, ngx, ngy, ngz, inst);
     return hipGetLastError();
 }
 

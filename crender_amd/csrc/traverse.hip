@@ -28,6 +28,7 @@ constexpr float kInf = __builtin_huge_valf();
 constexpr uint32_t kEmpty = kInvalidRef;
 constexpr uint32_t kMissKey = 0xFFFFFFFFu;
 constexpr uint32_t kRefillIdle = 16;
+constexpr uint32_t kInstExit   = 0xFFFFFFFEu;   // stack marker: leave the current instance
 // Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
 // 1 + 2*gamma(3) keeps FP32 box tests conservative.
 constexpr float kFarScale = 1.0000003576278687f;
@@ -242,7 +243,11 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // kMulti: rays of many scenes in one launch (batched rendering); each lane takes its
 // ray's scene (nodes, triangles, root) when it fetches the ray.
-template <int W, bool kAny, bool kMulti, uint32_t S>
+// kInst: two-level scene.  A leaf of the instance BVH (level 0) names instances; entering
+// one transforms the lane's ray into the instance's object space (direction renormalised,
+// distances scaled by |inv * d|, model.cpp:107-113) and pushes kInstExit, whose pop
+// restores the world ray; closest-hit distances are compared in world units.
+template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ, uint32_t *spill)
 {
     __shared__ uint32_t s_ref[S * kTraceBlock];
@@ -267,6 +272,50 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
     const void   *nodes = sc.nodes;
     const float4 *tris  = sc.tri;
     uint32_t rid = 0, cur = kEmpty;
+    // instance state (kInst): world ray while inside an instance, scale, instance ids
+    f3       wo = mk3(0, 0, 0), wd = mk3(0, 0, 0);
+    float    wtmax = 0.0f, ilen = 1.0f;
+    uint32_t level = 0, cur_inst = kInvalidRef, best_inst = kInvalidRef;
+    bool     inst_hit = false;
+    auto enter_instance = [&](uint32_t inst) {
+        const InstDev &I = sc.inst[inst];
+        wo    = r.o;
+        wd    = r.d;
+        wtmax = r.tmax;
+        const f3 o2 = mk3(I.inv[0] * r.o.x + I.inv[1] * r.o.y + I.inv[2] * r.o.z + I.inv[3],
+                          I.inv[4] * r.o.x + I.inv[5] * r.o.y + I.inv[6] * r.o.z + I.inv[7],
+                          I.inv[8] * r.o.x + I.inv[9] * r.o.y + I.inv[10] * r.o.z + I.inv[11]);
+        const f3 d2 = mk3(I.inv[0] * r.d.x + I.inv[1] * r.d.y + I.inv[2] * r.d.z,
+                          I.inv[4] * r.d.x + I.inv[5] * r.d.y + I.inv[6] * r.d.z,
+                          I.inv[8] * r.d.x + I.inv[9] * r.d.y + I.inv[10] * r.d.z);
+        ilen     = sqrtf(dot(d2, d2));
+        r.o      = o2;
+        r.d      = d2 * (1.0f / ilen);
+        r.inv    = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+        r.wt     = wt_setup(r.d);
+        r.tmax   = wtmax * ilen;
+        level    = 1;
+        cur_inst = inst;
+        inst_hit = false;
+        st.push(kInstExit, -kInf);
+        return I.root;
+    };
+    auto exit_instance = [&]() {
+        r.o    = wo;
+        r.d    = wd;
+        r.inv  = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));
+        r.wt   = wt_setup(r.d);
+        r.tmax = inst_hit ? r.tmax / ilen : wtmax;
+        level  = 0;
+    };
+    // next node to visit: pops, leaving instances as their exit markers come up
+    auto next = [&]() -> uint32_t {
+        for (;;) {
+            const uint32_t c = st.pop(r.tmax);
+            if (!kInst || c != kInstExit) return c;
+            exit_instance();
+        }
+    };
     bool     active = false, exhausted = false;
     uint32_t next_slice = 0;   // slices tried by this wave (wave-uniform)
 
@@ -310,6 +359,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                 r.best_prim = kInvalidRef;
                 r.bu = r.bv = 0.0f;
                 st.sp = st.gsp = 0;
+                level     = 0;
+                best_inst = kInvalidRef;
                 uint32_t root = sc.root, ntri = sc.ntri;
                 if constexpr (kMulti) {
                     const TraceScene ts = q.scenes[q.path_job[q.ray_path[rid]]];
@@ -338,7 +389,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
                 if (nh == 0) {
-                    cur = st.pop(r.tmax);
+                    cur = next();
                 } else {
                     sort_pairs<W>(key, ref);
                     cur = ref[0];
@@ -353,7 +404,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
         for (;;) {
             const bool leaf = active && cur != kEmpty && ref_is_leaf(cur);
             if (__ballot(leaf) == 0) break;
-            if (leaf) {
+            if (kInst && leaf && level == 0) {
+                // instance leaf: enter the first instance, keep the rest for later
+                const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+                if (cnt > 1) st.push(ref_leaf(first + 1, cnt - 1), -kInf);
+                cur = enter_instance(sc.tlas_inst[first]);
+            } else if (leaf) {
                 const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
                 bool any = false;
                 for (uint32_t k = 0; k < cnt; ++k) {
@@ -371,6 +427,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                             r.best_prim = prim;
                             r.bu        = u;
                             r.bv        = v;
+                            if (kInst) {
+                                best_inst = cur_inst;
+                                inst_hit  = true;
+                            }
                         }
                         if (kAny) {
                             any = true;
@@ -378,7 +438,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                         }
                     }
                 }
-                cur = (kAny && any) ? kEmpty : st.pop(r.tmax);
+                cur = (kAny && any) ? kEmpty : next();
             }
         }
         if (active && cur == kEmpty) {
@@ -390,6 +450,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q
                 q.hits.tri[rid] = r.best_tri;
                 if (q.hits.u) q.hits.u[rid] = r.bu;
                 if (q.hits.v) q.hits.v[rid] = r.bv;
+                if (kInst && q.hits.inst) q.hits.inst[rid] = h ? best_inst : kInvalidRef;
             }
             active = false;
         }
@@ -417,12 +478,12 @@ uint64_t trace_spill_bytes(uint32_t grid)
 namespace {
 
 // Instantiates k_trace for the BVH width and LDS stack depth of the launch.
-template <bool kAny, bool kMulti>
+template <bool kAny, bool kMulti, bool kInst>
 hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev &sc, const TraceQueue &q, uint8_t *occ,
                           uint32_t *spill, hipStream_t s)
 {
     const dim3 g(tc.grid), b(kTraceBlock);
-#define CRT_TRACE(Wd, Sd) k_trace<Wd, kAny, kMulti, Sd><<<g, b, 0, s>>>(sc, q, occ, spill)
+#define CRT_TRACE(Wd, Sd) k_trace<Wd, kAny, kMulti, Sd, kInst><<<g, b, 0, s>>>(sc, q, occ, spill)
     if (width == 4) {
         if (tc.stack == 8) CRT_TRACE(4, 8);
         else if (tc.stack == 32) CRT_TRACE(4, 32);
@@ -441,13 +502,15 @@ hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev 
 hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
                                 hipStream_t s)
 {
-    return dispatch_trace<false, false>(sc.width, tc, sc, q, nullptr, spill, s);
+    if (sc.instanced) return dispatch_trace<false, false, true>(sc.width, tc, sc, q, nullptr, spill, s);
+    return dispatch_trace<false, false, false>(sc.width, tc, sc, q, nullptr, spill, s);
 }
 
 hipError_t launch_trace_any(const SceneDev &sc, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
                             const TraceConfig &tc, hipStream_t s)
 {
-    return dispatch_trace<true, false>(sc.width, tc, sc, q, occ, spill, s);
+    if (sc.instanced) return dispatch_trace<true, false, true>(sc.width, tc, sc, q, occ, spill, s);
+    return dispatch_trace<true, false, false>(sc.width, tc, sc, q, occ, spill, s);
 }
 
 hipError_t launch_trace_closest_multi(uint32_t width, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
@@ -455,7 +518,7 @@ hipError_t launch_trace_closest_multi(uint32_t width, const TraceQueue &q, uint3
 {
     SceneDev none{};
     none.ntri = 1;   // per-ray scenes decide
-    return dispatch_trace<false, true>(width, tc, none, q, nullptr, spill, s);
+    return dispatch_trace<false, true, false>(width, tc, none, q, nullptr, spill, s);
 }
 
 hipError_t launch_trace_any_multi(uint32_t width, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
@@ -463,7 +526,7 @@ hipError_t launch_trace_any_multi(uint32_t width, const TraceQueue &q, uint8_t *
 {
     SceneDev none{};
     none.ntri = 1;
-    return dispatch_trace<true, true>(width, tc, none, q, occ, spill, s);
+    return dispatch_trace<true, true, false>(width, tc, none, q, occ, spill, s);
 }
 
 }  // namespace crt

@@ -205,12 +205,13 @@ class Scene:
         tf = np.ascontiguousarray(np.broadcast_to(np.float32(tfar), (n,)) if np.isscalar(tfar) else tfar, dtype=np.float32)
         rays = A.Rays(*[vptr(a) for a in comps + [tn, tf]])
         out = {"t": np.empty(n, np.float32), "u": np.empty(n, np.float32), "v": np.empty(n, np.float32),
-               "prim_id": np.empty(n, np.uint32), "geom_id": np.empty(n, np.uint32)}
+               "prim_id": np.empty(n, np.uint32), "geom_id": np.empty(n, np.uint32),
+               "inst_id": np.empty(n, np.uint32)}
         if normals:
             for k in ("ng_x", "ng_y", "ng_z"):
                 out[k] = np.empty(n, np.float32)
         hits = A.Hits(*[vptr(out[k]) if k in out else None for k in
-                        ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z")])
+                        ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z", "inst_id")])
         check(self._L.crt_intersect(self._h, C.byref(rays), n, C.byref(hits), A.CRT_MEM_HOST))
         return out
 

@@ -172,7 +172,8 @@ typedef struct crt_hits {     /* SoA; any pointer except t/prim_id may be NULL *
     float    *u, *v;          /* barycentrics of v1 and v2 (Embree convention) */
     uint32_t *prim_id;        /* 0xFFFFFFFF on miss (RTC_INVALID_GEOMETRY_ID) */
     uint32_t *geom_id;        /* model index, 0xFFFFFFFF on miss */
-    float    *ng_x, *ng_y, *ng_z; /* unnormalised (v1-v0)x(v2-v0) */
+    float    *ng_x, *ng_y, *ng_z; /* unnormalised (v1-v0)x(v2-v0), object space for instances */
+    uint32_t *inst_id;        /* instance of the model (crt_scene_set_instances order), 0xFFFFFFFF on miss */
 } crt_hits;
 
 typedef struct crt_context  crt_context;
@@ -194,8 +195,11 @@ int crt_scene_create(crt_context *ctx, crt_scene **out);                      /*
 int crt_scene_destroy(crt_scene *scene);
 /* scene::add_model (scene.cpp:31-34) -> registry::register_model (registry.cpp:51-97) */
 int crt_scene_add_model(crt_scene *scene, const crt_model_data *model, uint32_t *model_id);
-/* entity::instances (components.h:18-21); transforms are column-major 4x4.  Identity by
- * default (registry.cpp:71-72).  Non-identity instances are traced as model.cpp:99-126. */
+/* entity::instances (components.h:18-21); transforms are column-major affine 4x4
+ * (object -> world).  Identity by default (registry.cpp:71-72); count 0 hides the model.
+ * A scene with any other instancing is committed as a two-level BVH and traced as
+ * model.cpp:99-126 does (ray mapped into object space with its direction renormalised,
+ * world distances compared; hit normals stay in object space in compat mode). */
 int crt_scene_set_instances(crt_scene *scene, uint32_t model_id, const float *transforms, uint32_t count);
 /* replace a model's material table (ui.h:924-929 edits) */
 int crt_scene_set_materials(crt_scene *scene, uint32_t model_id, const crt_material *mats, uint32_t count);

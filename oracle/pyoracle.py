@@ -257,3 +257,45 @@ def display(raw: np.ndarray, n: int) -> np.ndarray:
     """renderer.cpp:371-383 display transform."""
     m = np.clip(raw.astype(np.float32) / np.float32(n), 0.0, 1.0)
     return np.power(m, np.float32(1.0 / 2.2))
+
+
+def intersect_instanced(models, instances, org, dirs, mode=BRUTE64, nthreads=0):
+    """Closest hit over models x instances as the reference computes it
+    (scene.cpp:79-98 -> model.cpp:99-126): per instance, inverse(T) maps the ray into object
+    space with the direction renormalised; the object-space hit (tnear 1e-5) is mapped back
+    by T and its distance recomputed in world space as |T p - o|; the strictly nearer hit
+    wins, so the first model / instance wins ties.
+
+    models: list of (ntri, 9) float32 corner arrays; instances: list (per model) of lists of
+    column-major 4x4 object->world transforms.  Returns (model, inst, prim, dist, u, v) with
+    model = -1 on a miss; dist in float64."""
+    org = np.asarray(org, np.float64).reshape(-1, 3)
+    dirs = np.asarray(dirs, np.float64).reshape(-1, 3)
+    n = org.shape[0]
+    best = np.full(n, np.inf)
+    bm = np.full(n, -1, np.int64)
+    bi = np.full(n, -1, np.int64)
+    bp = np.full(n, -1, np.int64)
+    bu = np.zeros(n)
+    bv = np.zeros(n)
+    for k, (tri, insts) in enumerate(zip(models, instances)):
+        for j, T in enumerate(insts):
+            M = np.asarray(T, np.float64).reshape(4, 4).T        # column-major -> row-major
+            inv = np.linalg.inv(M)
+            o2 = (org @ inv[:3, :3].T) + inv[:3, 3]
+            d2 = dirs @ inv[:3, :3].T
+            d2 = d2 / np.linalg.norm(d2, axis=1, keepdims=True)
+            prim, t, u, v = intersect(tri, o2.astype(np.float32), d2.astype(np.float32), mode=mode, nthreads=nthreads)
+            hit = prim >= 0
+            th = np.where(hit, t, 0.0)
+            p_obj = o2.astype(np.float32).astype(np.float64) + d2.astype(np.float32).astype(np.float64) * th[:, None]
+            p_w = p_obj @ M[:3, :3].T + M[:3, 3]
+            dist = np.where(hit, np.linalg.norm(p_w - org, axis=1), np.inf)
+            win = dist < best
+            best = np.where(win, dist, best)
+            bm = np.where(win, k, bm)
+            bi = np.where(win, j, bi)
+            bp = np.where(win, prim, bp)
+            bu = np.where(win, u, bu)
+            bv = np.where(win, v, bv)
+    return bm, bi, bp, best, bu, bv
