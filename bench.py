@@ -12,6 +12,13 @@ traversal): algorithmic bytes = 48 B per ray (32 B ray in, 16 B hit out) over th
 kernel's mean launch time measured with HIP events on the library's stream.
 ``cpu_baseline`` times the CPU oracle's restatement of the reference loop (FP32
 binned-SAH BVH, one task per row, barrier per pass) on a band of rows of the same frame.
+
+Two more measurements ride on the same line (skip with --no-nrc / --no-batch):
+``nrc`` -- the metric's second half: image MSE against a 4096-spp plain render of the same
+scene at 1920x1080 (pbr, depth 16), plain tracer vs neural radiance cache, as MSE-vs-time
+curves; ``time_ratio`` = the cache's time to reach the plain tracer's MSE at T over T.
+``batch`` -- 64 independent scenes rendered by one batched call after the autotuner:
+aggregate Mrays/s and per-scene latency.
 """
 from __future__ import annotations
 
@@ -66,6 +73,124 @@ def cpu_baseline(spec, res, bounces, seconds_target: float, threads: int) -> dic
     }
 
 
+def _mse(img, ref) -> float:
+    d = img.astype(np.float64) - ref
+    return float(np.mean(d * d))
+
+
+def _curve(r, checkpoints, ref, t_start=0.0):
+    """Renders up to each checkpoint spp; returns [(spp, seconds of rendering, MSE)]."""
+    out, done, t = [], 0, t_start
+    for c in checkpoints:
+        t0 = time.perf_counter()
+        r.render(c - done)
+        t += time.perf_counter() - t0
+        done = c
+        out.append((c, t, _mse(r.raw() / c, ref)))
+    return out
+
+
+def _time_to_reach(curve, target):
+    """Seconds at which a (spp, seconds, MSE) curve first reaches `target` MSE (log-log
+    interpolation between checkpoints); None if it never does."""
+    if curve and curve[0][2] <= target:
+        return curve[0][1]
+    for (_, t0, m0), (_, t1, m1) in zip(curve, curve[1:]):
+        if m0 > target >= m1:
+            f = (np.log(m0) - np.log(target)) / (np.log(m0) - np.log(m1))
+            return float(np.exp(np.log(t0) + f * (np.log(t1) - np.log(t0))))
+    return None
+
+
+def nrc_equal_time(ctx, sc, ref_spp: int) -> dict:
+    """Equal-time MSE against a ref_spp plain render at 1920x1080: plain tracer vs the neural
+    radiance cache (warm-up passes counted), pbr estimator, depth 16 with Russian roulette
+    from bounce 3, the soup scene of the Mrays/s workload."""
+    from crender_amd import Nrc, Renderer
+
+    res, bounces, rr = (1920, 1080), 16, 3
+
+    def mk(seed):
+        r = Renderer(ctx, sc, res[0], res[1], bounces)
+        r.set_resolution(*res)
+        r.set_options(mode=1, rr_start=rr, seed=seed)
+        return r
+
+    t0 = time.perf_counter()
+    ref = mk(1000)
+    done = 0
+    while done < ref_spp:
+        k = min(512, ref_spp - done)
+        ref.render(k)
+        done += k
+        print(f"# nrc reference {done}/{ref_spp} spp, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    refimg = ref.raw() / ref_spp
+    ref.close()
+    plain = mk(1)
+    pc = _curve(plain, [4, 8, 16, 32, 64], refimg)
+    plain.close()
+    nrc = Nrc(ctx, seed=5)
+    r = mk(2)
+    warm = 32
+    r.set_nrc(nrc, warmup_passes=warm)
+    tw0 = time.perf_counter()
+    r.render(warm)
+    t_warm = time.perf_counter() - tw0
+    nc = _curve(r, [1, 2, 4, 8, 16, 32, 64], refimg, t_start=t_warm)
+    st = r.current_stats()
+    r.close()
+    nrc.close()
+    T = pc[-1][1]
+    target = pc[-1][2]
+    t_match = _time_to_reach(nc, target)
+    return {
+        "workload": "soup 1M tris, 1920x1080, pbr, depth 16, RR from bounce 3; cache: term bounce 1 (footprint c=0.01), "
+                    "65536 training paths/pass, 4 bounces past the query, 4 Adam steps/pass, 32 warm-up passes",
+        "reference_spp": ref_spp,
+        "plain": [{"spp": a, "seconds": b, "mse": c} for a, b, c in pc],
+        "cache": [{"spp": a, "seconds": b, "mse": c} for a, b, c in nc],
+        "cache_warmup_seconds": t_warm,
+        "cache_train_ms_per_pass": st["ms_nrc_train"] / max(1, warm + 64),
+        "equal_time_seconds": T,
+        "plain_mse_at_T": target,
+        "cache_seconds_to_plain_mse": t_match,
+        "time_ratio": (t_match / T) if t_match else None,
+    }
+
+
+def batch_bench(seed: int = 7) -> dict:
+    """64 independent scenes (soup / terrain, 1k-500k tris, 128^2-1024^2, 1-64 spp, depth 6)
+    rendered by crt_render_batch after crt_autotune."""
+    from crender_amd import Context, autotune, render_batch, scene_from_spec, scenes
+
+    suite = scenes.batch_suite(64, seed)
+    ctx = Context(0)
+    scs = [scene_from_spec(ctx, spec) for spec, _, _, _ in suite]
+    jobs = [dict(scene=sc, width=w, height=h, spp=spp, max_bounces=6, mode=spec.mode, rr_start=0, seed=k)
+            for k, (sc, (spec, w, h, spp)) in enumerate(zip(scs, suite))]
+    tune = autotune(ctx, jobs)
+    print(f"# batch autotune {tune}", file=sys.stderr, flush=True)
+    _, ms, st = render_batch(ctx, jobs)
+    for sc in scs:
+        sc.close()
+    ctx.close()
+    lat = np.sort(np.asarray(ms))
+    return {
+        "workload": "64 scenes (soup / terrain), 1k-500k tris log-uniform, 128^2-1024^2, spp 1/4/16/64, depth 6",
+        "triangles": int(sum(spec.ntri for spec, _, _, _ in suite)),
+        "paths": int(sum(w * h * spp for _, w, h, spp in suite)),
+        "mrays_per_s": st["rays_per_second"] / 1e6,
+        "ms_total": st["ms_total"],
+        "extension_rays": st["extension_rays"],
+        "shadow_rays": st["shadow_rays"],
+        "replays": st["replays"],
+        "used_graph": bool(st["used_graph"]),
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
+                       "max": float(lat[-1])},
+        "tuned": tune,
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,8 +199,13 @@ def main(argv=None) -> int:
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=0, help="BVH width (0 = library default)")
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent traversal occupancy (0 = default)")
+    ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per traversal lane (0 = default)")
+    ap.add_argument("--sort-rays", action="store_true", help="reorder secondary rays before tracing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
+    ap.add_argument("--no-nrc", action="store_true", help="skip the neural radiance cache equal-time MSE")
+    ap.add_argument("--nrc-ref-spp", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) 5 times, for scaling "
                          "profiler byte counters; prints the bytes moved per launch")
@@ -100,7 +230,8 @@ def main(argv=None) -> int:
 
     res, bounces = (1920, 1080), 4
     spec = scenes.soup(args.tris, seed=2, res=res)
-    ctx = Context(0, bvh_width=args.width, phase_timing=True, trace_waves_per_cu=args.waves_per_cu)
+    ctx = Context(0, bvh_width=args.width, phase_timing=True, trace_waves_per_cu=args.waves_per_cu,
+                  trace_stack=args.stack, sort_rays=args.sort_rays)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
@@ -144,6 +275,8 @@ def main(argv=None) -> int:
                         f"(depth 4), compat estimator, sun off, constant sky",
             "bvh_width": info["width"],
             "waves_per_cu": args.waves_per_cu,
+            "trace_stack": args.stack or 16,
+            "sort_rays": args.sort_rays,
             "device": dev,
         },
         "roofline": {
@@ -166,8 +299,12 @@ def main(argv=None) -> int:
                 "bytes": info["bytes"]},
     }
     r.close()
+    if not args.no_nrc:
+        out["nrc"] = nrc_equal_time(ctx, sc, args.nrc_ref_spp)
     sc.close()
     ctx.close()
+    if not args.no_batch:
+        out["batch"] = batch_bench()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(spec, res, bounces, args.cpu_seconds, threads)

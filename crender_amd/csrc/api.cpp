@@ -156,6 +156,7 @@ struct crt_renderer {
     crt_render_options opt{};
     std::atomic<uint64_t> samples{0};
     PathBuffers  b{};
+    RaySortScratch sort{};
     float4      *display = nullptr;
     std::vector<void *> allocs;
     // background passes
@@ -179,8 +180,6 @@ struct crt_renderer {
     // neural radiance cache
     crt_nrc               *nrc = nullptr;
     crt_nrc_render_options nopt{};
-    PathBuffers            tb{};            // training paths
-    std::vector<void *>    nrc_allocs;
     double                 ms_nrc_train = 0.0;
     float                  last_train_loss = 0.0f;
     uint32_t               nrc_warm_done = 0;   // warm-up passes run since set_nrc
@@ -352,10 +351,14 @@ int alloc_renderer_buffers(crt_renderer *r)
 {
     free_renderer_buffers(r);
     PathBuffers &b = r->b;
-    b.W = r->W;
-    b.H = r->H;
-    b.P = r->W * r->H;
-    const size_t P = b.P;
+    b.W    = r->W;
+    b.H    = r->H;
+    b.npix = r->W * r->H;
+    // cache-training paths share the wavefront: slots npix .. npix + train_paths - 1
+    const size_t T = r->nrc ? r->nopt.train_paths : 0u;
+    const size_t S = r->nrc ? r->nopt.train_suffix : 0u;
+    b.P = b.npix + (uint32_t)T;
+    const size_t P = b.P, NP = b.npix;
     int st;
 #define RA(ptr, n)                          \
     if ((st = ralloc(r, ptr, n)) != CRT_OK) return st
@@ -379,23 +382,40 @@ int alloc_renderer_buffers(crt_renderer *r)
     RA(&b.occ, P);
     RA(&b.thr, P);
     RA(&b.rad, P);
-    RA(&b.raw, 3 * P);
-    RA(&b.aov_alb, P);
-    RA(&b.aov_nrm, P);
-    RA(&b.aov_dep, P);
-    RA(&r->display, P);
+    RA(&b.raw, 3 * NP);
+    RA(&b.aov_alb, NP);
+    RA(&b.aov_nrm, NP);
+    RA(&b.aov_dep, NP);
+    RA(&r->display, NP);
     RA(&b.cnt, (size_t)kCntWords);
     RA(&b.stats, (size_t)4 * kStatSlots);
+    if (T) {
+        RA(&b.rq, T * S * 13);
+        RA(&b.r_thr, T * S);
+        RA(&b.r_rad, T * S);
+        RA(&b.tq, T * S * 13);
+        RA(&b.tt, T * S * 3);
+    }
+    r->sort = RaySortScratch{};
+    if (r->ctx->opt.flags & CRT_FLAG_SORT_RAYS) {
+        RA(&r->sort.keys, P);
+        RA(&r->sort.keys_out, P);
+        RA(&r->sort.vals, P);
+        RA(&r->sort.vals_out, P);
+        r->sort.temp_bytes = ray_sort_temp_bytes((uint32_t)P);
+        RA(reinterpret_cast<uint8_t **>(&r->sort.temp), std::max<size_t>(r->sort.temp_bytes, 1));
+        r->sort.capacity = (uint32_t)P;
+    }
 #undef RA
     if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
     hipStream_t s = r->ctx->stream;
-    HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * P, s));
+    HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * NP, s));
     HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
     // image.h:30-33: new images are filled with FLT_MAX
-    HIP_CHECK(launch_fill(b.aov_alb, P, 3.402823466e+38f, s));
-    HIP_CHECK(launch_fill(b.aov_nrm, P, 3.402823466e+38f, s));
-    HIP_CHECK(launch_fill(b.aov_dep, P, 3.402823466e+38f, s));
+    HIP_CHECK(launch_fill(b.aov_alb, NP, 3.402823466e+38f, s));
+    HIP_CHECK(launch_fill(b.aov_nrm, NP, 3.402823466e+38f, s));
+    HIP_CHECK(launch_fill(b.aov_dep, NP, 3.402823466e+38f, s));
     HIP_CHECK(hipStreamSynchronize(s));
     return CRT_OK;
 }
@@ -444,11 +464,34 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
     HIP_CHECK(launch_generate(sc, b, rp, s));
     HIP_CHECK(mark(1));
     uint32_t cur = 0;
-    const uint32_t timed = std::min<uint32_t>(rp.max_bounces, crt_renderer::kTimedBounces);
-    for (uint32_t i = 0; i < rp.max_bounces; ++i) {
+    // bounces to run: camera paths stop at max_bounces, or at term + 1 at the latest when
+    // they end in the cache (later only through alpha pass-through); training paths at
+    // train_term + 1
+    uint32_t nb = rp.max_bounces;
+    if (rp.nrc.enabled) {
+        uint32_t cam = sc.any_alpha ? rp.max_bounces : std::min(rp.max_bounces, rp.nrc.term + 2);
+        if (rp.nrc.gen_first >= b.npix) cam = 0;
+        nb = std::max(cam, rp.nrc.train_term + 1);
+    }
+    const uint32_t timed = std::min<uint32_t>(nb, crt_renderer::kTimedBounces);
+    const bool sort = (ctx->opt.flags & CRT_FLAG_SORT_RAYS) && r->sort.keys;
+    for (uint32_t i = 0; i < nb; ++i) {
         const bool t = timing && i < timed;
-        HIP_CHECK(launch_bounce_prep(b, cur, s));
-        if (t) HIP_CHECK(mark(2 + 4 * (int)i));
+        if (sort && i > 0) {
+            // reorder the secondary rays (counted in the extend phase)
+            if (t) HIP_CHECK(mark(2 + 4 * (int)i));
+            uint32_t n = 0;
+            HIP_CHECK(hipMemcpyAsync(&n, b.cnt + cur, sizeof n, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            n = std::min(n, b.P);
+            HIP_CHECK(launch_ray_sort(b.q[cur], b.qpath[cur], b.cnt + cur, n, b.q[cur ^ 1u], b.qpath[cur ^ 1u],
+                                      b.cnt + (cur ^ 1u), sc.bmin, sc.bmax, r->sort, s));
+            cur ^= 1u;
+            HIP_CHECK(launch_bounce_prep(b, cur, s));
+        } else {
+            HIP_CHECK(launch_bounce_prep(b, cur, s));
+            if (t) HIP_CHECK(mark(2 + 4 * (int)i));
+        }
         TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
         HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
         if (t) HIP_CHECK(mark(3 + 4 * (int)i));
@@ -490,15 +533,18 @@ NrcParams nrc_params(const crt_renderer *r)
     np.bmin = sc.bmin;
     const f3 ext = sc.bmax - sc.bmin;
     np.binv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
-    if (r->nrc) {
+    if (r->nrc && r->bounces > 0) {
         np.enabled     = 1;
         np.term        = r->nopt.term_bounce;
         np.footprint_c = r->nopt.footprint_c;
+        np.rec_first   = r->nopt.term_bounce;
+        np.rec_slots   = r->nopt.train_suffix;
+        np.train_term  = r->nopt.term_bounce + r->nopt.train_suffix;
     }
     return np;
 }
 
-int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render);
+int nrc_train_step(crt_renderer *r);
 
 // One sample pass: renderer.cpp:121-125 (one barrier per pass) as a kernel sequence.
 int render_pass(crt_renderer *r)
@@ -516,30 +562,28 @@ int render_pass(crt_renderer *r)
     rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
     rp.nrc         = nrc_params(r);
     // cache warm-up pass: only the training paths (their own Philox stream), no image
-    if (r->nrc && r->nrc_warm_done < r->nopt.warmup_passes) {
-        rp.sample = 0x40000000u + r->nrc_warm_done;
-        const auto t0 = std::chrono::steady_clock::now();
-        if (int st = nrc_training_pass(r, rp)) return st;
-        r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        r->nrc_warm_done++;
-        return CRT_OK;
+    const bool warmup = rp.nrc.enabled && r->nrc_warm_done < r->nopt.warmup_passes;
+    if (warmup) {
+        rp.sample        = 0x40000000u + r->nrc_warm_done;
+        rp.nrc.gen_first = b.npix;
     }
-    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0;
+    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !warmup;
     HIP_CHECK(hipEventRecord(r->ev[0], s));
     r->ms_phase_pending = 0;
     int st = run_paths(r, b, rp, timing);
     if (st) return st;
-    if (r->nrc) {
+    if (rp.nrc.enabled) {
+        // one inference for the camera and the training paths' queries
         HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
         HIP_CHECK(launch_nrc_apply(b, s));
     }
     if (timing) HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvAcc], s));
-    HIP_CHECK(launch_accumulate(b, s));
+    if (!warmup) HIP_CHECK(launch_accumulate(b, s));
     HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvEnd], s));
     HIP_CHECK(hipEventSynchronize(r->ev[crt_renderer::kEvEnd]));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[crt_renderer::kEvEnd]));
-    r->ms_phase[5] += ms;
+    if (!warmup) r->ms_phase[5] += ms;
     if (timing) {
         HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
         r->ms_phase[0] += ms;
@@ -558,41 +602,31 @@ int render_pass(crt_renderer *r)
     uint32_t err = 0;
     HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
     if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
-    if (r->nrc && r->nopt.train) {
+    if (rp.nrc.enabled && (r->nopt.train || warmup)) {
         const auto t0 = std::chrono::steady_clock::now();
-        if ((st = nrc_training_pass(r, rp))) return st;
+        if ((st = nrc_train_step(r))) return st;
         r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (warmup) {
+        r->nrc_warm_done++;
+        return CRT_OK;
     }
     r->samples.fetch_add(1);
     return CRT_OK;
 }
 
-// Training paths of the cache: random pixels, traced train_suffix bounces past the query
-// vertex with every vertex from term_bounce on recorded, then terminated into the cache
-// (self-training).  Targets L_v = (L_path - L_before_v) / T_v train the cache with
-// train_steps Adam steps.
-int nrc_training_pass(crt_renderer *r, const RenderParams &rp_render)
+// Training of the cache after a pass: the training paths (slots npix ..) traced
+// train_suffix bounces past the query vertex, recording every vertex from term_bounce on,
+// and terminated into the cache (self-training) give targets L_v = (L_path - L_before_v) /
+// T_v; train_steps Adam steps follow.
+int nrc_train_step(crt_renderer *r)
 {
     crt_nrc           *n = r->nrc;
-    const PathBuffers &t = r->tb;
+    const PathBuffers &t = r->b;
     hipStream_t        s = r->ctx->stream;
-    RenderParams       rp = rp_render;
-    const uint32_t     S  = r->nopt.train_suffix;
-    rp.nrc.enabled     = 1;
-    rp.nrc.train       = 1;
-    rp.nrc.record      = 1;
-    rp.nrc.rec_first   = r->nopt.term_bounce;
-    rp.nrc.rec_slots   = S;
-    rp.nrc.term        = r->nopt.term_bounce + S;
-    rp.nrc.footprint_c = 0.0f;
-    rp.max_bounces     = rp.nrc.term + 1;
-    rp.sample          = rp_render.sample | 0x80000000u;
-    int st = run_paths(r, t, rp, false);
-    if (st) return st;
-    HIP_CHECK(nrc_launch_infer_dev(t.nq, t.nq_out, t.cnt + kCntNrcQ, t.P, n->WT, s));
-    HIP_CHECK(launch_nrc_apply(t, s));
+    const uint32_t     S = r->nopt.train_suffix;
     HIP_CHECK(hipMemsetAsync(t.cnt + kCntTrain, 0, sizeof(uint32_t), s));
-    const uint32_t max_out = t.P * S;
+    const uint32_t max_out = (t.P - t.npix) * S;
     HIP_CHECK(launch_nrc_targets(t, S, max_out, s));
     uint32_t count = 0;
     HIP_CHECK(hipMemcpyAsync(&count, t.cnt + kCntTrain, sizeof count, hipMemcpyDeviceToHost, s));
@@ -653,7 +687,7 @@ int join_worker(crt_renderer *r)
 int reset_accum(crt_renderer *r)
 {
     hipStream_t s = r->ctx->stream;
-    HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.P, s));
+    HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.npix, s));
     HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
     HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -1491,8 +1525,6 @@ int crt_renderer_destroy(crt_renderer *r)
     join_worker(r);
     (void)hipSetDevice(r->ctx->device);
     free_renderer_buffers(r);
-    for (void *p : r->nrc_allocs) dfree(p);
-    r->nrc_allocs.clear();
     for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
     r->scene->users--;
     delete r;
@@ -1572,8 +1604,6 @@ int crt_renderer_set_resolution(crt_renderer *r, uint32_t x, uint32_t y)
     r->W      = x;
     r->H      = y;
     r->aspect = (float)x / (float)y;   // renderer.cpp:199
-    r->tb.W   = x;
-    r->tb.H   = y;
     if ((st = alloc_renderer_buffers(r))) return st;
     r->samples.store(0);
     return CRT_OK;
@@ -1651,11 +1681,11 @@ int crt_renderer_current_resolution(crt_renderer *r, uint32_t *x, uint32_t *y)
 
 static int buffer_ptr(crt_renderer *r, int buffer, void **ptr, size_t *bytes, bool refresh)
 {
-    const size_t P = r->b.P;
+    const size_t P = r->b.npix;
     switch (buffer) {
     case CRT_BUFFER_PROGRESS:
         if (refresh) {
-            HIP_CHECK(launch_display(r->b.raw, r->samples.load(), r->b.P, r->display, r->ctx->stream));
+            HIP_CHECK(launch_display(r->b.raw, r->samples.load(), r->b.npix, r->display, r->ctx->stream));
         }
         *ptr   = r->display;
         *bytes = P * 16;
@@ -1729,7 +1759,7 @@ struct CkptHeader {
 int crt_renderer_checkpoint_size(crt_renderer *r, uint64_t *bytes)
 {
     if (!r || !bytes) return fail(CRT_ERR_INVALID_ARG, "null argument");
-    *bytes = sizeof(CkptHeader) + (uint64_t)r->b.P * 3 * sizeof(float);
+    *bytes = sizeof(CkptHeader) + (uint64_t)r->b.npix * 3 * sizeof(float);
     return CRT_OK;
 }
 
@@ -1753,7 +1783,7 @@ int crt_renderer_checkpoint_save(crt_renderer *r, void *buf, uint64_t bytes)
     h.sample_offset = r->opt.sample_offset;
     h.scene_hash    = r->scene->hash;
     std::memcpy(buf, &h, sizeof h);
-    HIP_CHECK(hipMemcpy(static_cast<char *>(buf) + sizeof h, r->b.raw, (size_t)r->b.P * 12, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(static_cast<char *>(buf) + sizeof h, r->b.raw, (size_t)r->b.npix * 12, hipMemcpyDeviceToHost));
     return CRT_OK;
 }
 
@@ -1767,9 +1797,9 @@ int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t byte
     if (std::memcmp(h.magic, "CRTCKPT1", 8) != 0 || h.version != 1) return fail(CRT_ERR_CHECKPOINT, "not a crt checkpoint");
     if (h.W != r->W || h.H != r->H) return fail(CRT_ERR_CHECKPOINT, "checkpoint resolution differs");
     if (h.scene_hash != r->scene->hash) return fail(CRT_ERR_CHECKPOINT, "checkpoint scene differs");
-    if (bytes < sizeof h + (uint64_t)r->b.P * 12) return fail(CRT_ERR_CHECKPOINT, "checkpoint truncated");
+    if (bytes < sizeof h + (uint64_t)r->b.npix * 12) return fail(CRT_ERR_CHECKPOINT, "checkpoint truncated");
     if ((st = reset_accum(r))) return st;
-    HIP_CHECK(hipMemcpy(r->b.raw, static_cast<const char *>(buf) + sizeof h, (size_t)r->b.P * 12, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(r->b.raw, static_cast<const char *>(buf) + sizeof h, (size_t)r->b.npix * 12, hipMemcpyHostToDevice));
     r->bounces           = h.bounces;
     r->opt.mode          = h.mode;
     r->opt.rr_start      = h.rr_start;
@@ -1783,15 +1813,6 @@ int crt_renderer_checkpoint_load(crt_renderer *r, const void *buf, uint64_t byte
 
 // ---- neural radiance cache --------------------------------------------------------------
 
-namespace {
-void free_nrc_paths(crt_renderer *r)
-{
-    for (void *p : r->nrc_allocs) dfree(p);
-    r->nrc_allocs.clear();
-    r->tb = PathBuffers{};
-}
-}  // namespace
-
 extern "C" void crt_nrc_render_options_default(crt_nrc_render_options *o)
 {
     if (!o) return;
@@ -1804,37 +1825,26 @@ extern "C" void crt_nrc_render_options_default(crt_nrc_render_options *o)
     o->warmup_passes = 0;
 }
 
+// Attaching or detaching the cache re-allocates the path buffers (the training paths share
+// the wavefront) and so clears the accumulation, like renderer::update (renderer.cpp:185-192).
 extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc_render_options *opt)
 {
     int st = require_paused(r);
     if (st) return st;
     if (nrc && nrc->ctx != r->ctx) return fail(CRT_ERR_INVALID_ARG, "cache belongs to another context");
-    free_nrc_paths(r);
-    r->nrc = nrc;
-    if (!nrc) return CRT_OK;
     crt_nrc_render_options o;
     crt_nrc_render_options_default(&o);
     if (opt) o = *opt;
-    if (o.term_bounce < 1 || o.train_suffix < 1 || o.train_paths < 64 || o.train_paths > (1u << 22))
+    if (nrc && (o.term_bounce < 1 || o.train_suffix < 1 || o.train_suffix > 16 || o.train_paths < 64 ||
+                o.train_paths > (1u << 22)))
         return fail(CRT_ERR_INVALID_ARG, "bad cache render options");
+    join_worker(r);
+    r->nrc           = nrc;
     r->nopt          = o;
     r->nrc_warm_done = 0;
-    PathBuffers &t = r->tb;
-    t.W = r->W;
-    t.H = r->H;
-    t.P = o.train_paths;
-    const size_t P = t.P, S = o.train_suffix;
-    auto &L = r->nrc_allocs;
-    if ((st = alloc_path_core(L, t, P))) return st;
-    if ((st = alloc_nrc_queries(L, t, P))) return st;
-    if ((st = lalloc(L, &t.rq, P * S * 13))) return st;
-    if ((st = lalloc(L, &t.r_thr, P * S))) return st;
-    if ((st = lalloc(L, &t.r_rad, P * S))) return st;
-    if ((st = lalloc(L, &t.tq, P * S * 13))) return st;
-    if ((st = lalloc(L, &t.tt, P * S * 3))) return st;
-    HIP_CHECK(hipMemset(t.cnt, 0, sizeof(uint32_t) * kCntWords));
-    HIP_CHECK(hipMemset(t.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots));
-    return CRT_OK;
+    if ((st = alloc_renderer_buffers(r))) return st;
+    r->samples.store(0);
+    return reset_accum(r);
 }
 
 
@@ -2241,7 +2251,8 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
     auto &L = run.allocs;
     int st;
     PathBuffers b{};
-    b.P = P;
+    b.P    = P;
+    b.npix = P;   // no cache-training paths in batches
     if ((st = alloc_path_core(L, b, P))) return st;
     if ((st = lalloc(L, &b.pjob, P))) return st;
     if ((st = lalloc(L, &b.ppix, P))) return st;

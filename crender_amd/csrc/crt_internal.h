@@ -188,6 +188,19 @@ hipError_t launch_trace_closest_multi(uint32_t width, const TraceQueue &q, uint3
                                       hipStream_t s);
 hipError_t launch_trace_any_multi(uint32_t width, const TraceQueue &q, uint8_t *occ, uint32_t *spill,
                                   const TraceConfig &tc, hipStream_t s);
+// Ray-queue reordering (raysort.hip): keys, values and radix-sort temporary storage.
+struct RaySortScratch {
+    uint32_t *keys, *keys_out, *vals, *vals_out;
+    void     *temp;
+    size_t    temp_bytes;
+    uint32_t  capacity;
+};
+size_t     ray_sort_temp_bytes(uint32_t capacity);
+// Sorts the n rays of src by (origin Morton code in [bmin, bmax], direction octant) into
+// dst (rays, path ids, count).
+hipError_t launch_ray_sort(const RaySoA &src, const uint32_t *src_path, const uint32_t *src_count, uint32_t n,
+                           const RaySoA &dst, uint32_t *dst_path, uint32_t *dst_count, f3 bmin, f3 bmax,
+                           const RaySortScratch &w, hipStream_t s);
 uint32_t trace_grid(int device, uint32_t waves_per_cu);
 uint64_t trace_spill_bytes(uint32_t grid);
 constexpr uint32_t kTraceBlock = 256;
@@ -199,7 +212,8 @@ constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multip
 enum : uint32_t { kModeCompat = 0, kModePbr = 1 };
 
 struct PathBuffers {
-    uint32_t  W, H, P;          // image size, path slots (= W*H)
+    uint32_t  W, H, P;          // image size, path slots (W*H camera paths, then cache-training paths)
+    uint32_t  npix;             // W*H: slots below are pixels, slots from here on train the cache
     RaySoA    q[2];             // extension queues (ping-pong)
     uint32_t *qpath[2];         // path slot of each queued ray
     HitSoA    hits;
@@ -232,14 +246,16 @@ enum : uint32_t {
     kCntWorkExt = 8, kCntWorkShadow = 16, kCntWords = 32
 };
 
-// Neural radiance cache hooks of the integrator.
+// Neural radiance cache hooks of the integrator.  Camera paths (slots < npix) query the
+// cache at `term` (footprint rule); training paths (slots >= npix) start at random pixels,
+// record vertices rec_first .. rec_first + rec_slots - 1 and query at train_term.
 struct NrcParams {
     int32_t  enabled;       // terminate paths into the cache
-    uint32_t term;          // bounce at which paths query the cache
+    uint32_t term;          // bounce at which camera paths query the cache
     float    footprint_c;   // > 0: query at `term` only if the path spread exceeds c * a0, else at term + 1
-    int32_t  record;        // training paths: record vertices rec_first .. rec_first + rec_slots - 1
+    uint32_t train_term;    // bounce at which training paths query the cache
     uint32_t rec_first, rec_slots;
-    int32_t  train;         // training paths: random pixels, no AOVs, no accumulation
+    uint32_t gen_first;     // first path slot generated this pass (npix: training paths only)
     f3       bmin, binv;    // scene box, for normalising query positions to [0,1]
 };
 

@@ -122,34 +122,40 @@ __device__ __forceinline__ void put_ray(const RaySoA &q, uint32_t k, f3 o, f3 d)
     q.tf[k] = kInf;               // model.cpp:23
 }
 
+// Camera rays of the pass: one per pixel (slot p = pixel), then the cache-training paths,
+// which start at random pixels (slots >= npix); slots below rp.nrc.gen_first are skipped.
 __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, RenderParams rp)
 {
-    const uint32_t p    = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool     live = p < b.P;
+    const uint32_t first = rp.nrc.gen_first;
+    const uint32_t end   = rp.nrc.enabled ? b.P : b.npix;   // training slots idle without the cache
+    const uint32_t p     = first + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     live  = p < end;
     if (live) {
-        // training paths of the radiance cache start at random pixels
-        const uint32_t pix = rp.nrc.train ? philox4x32_10(u4{p, rp.sample, 0xFFFFFFFEu, 0u}, rp.seed, kKeyHi).x % (b.W * b.H)
-                                          : p;
+        const bool     trainp = p >= b.npix;
+        const uint32_t sample = trainp ? (rp.sample | 0x80000000u) : rp.sample;
+        const uint32_t pix    = trainp ? philox4x32_10(u4{p, sample, 0xFFFFFFFEu, 0u}, rp.seed, kKeyHi).x % b.npix : p;
         const uint32_t x = pix % b.W, y = pix / b.W;
-        const rnd4     j = rng4(rp.seed, p, rp.sample, kJitterBounce, 0);
-        if (rp.nrc.record)
-            for (uint32_t r = 0; r < rp.nrc.rec_slots; ++r) b.r_thr[(size_t)p * rp.nrc.rec_slots + r] = make_float4(0, 0, 0, 0);
+        const rnd4     j = rng4(rp.seed, p, sample, kJitterBounce, 0);
+        if (trainp)
+            for (uint32_t r = 0; r < rp.nrc.rec_slots; ++r)
+                b.r_thr[(size_t)(p - b.npix) * rp.nrc.rec_slots + r] = make_float4(0, 0, 0, 0);
         if (rp.nrc.enabled) b.nstate[p] = make_float4(0, 0, 0, 0);
         f3 o, d;
         // renderer.cpp:260-263
         camera_ray(rp.cam, ((float)x + j.a) / (float)b.W, ((float)y + j.b) / (float)b.H, rp.aspect, o, d);
-        put_ray(b.q[0], p, o, d);
-        b.qpath[0][p] = p;
+        const uint32_t slot = p - first;
+        put_ray(b.q[0], slot, o, d);
+        b.qpath[0][slot] = p;
         b.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         b.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (rp.max_bounces == 0 && !rp.nrc.train) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
-        if (p == 0) {
-            b.cnt[kCntQ0]   = rp.max_bounces > 0 ? b.P : 0u;
+        if (rp.max_bounces == 0 && !trainp) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+        if (p == first) {
+            b.cnt[kCntQ0]   = rp.max_bounces > 0 ? end - first : 0u;
             b.cnt[kCntNrcQ] = 0;
         }
     }
     // every path starts at total_bounces = 1 (renderer.cpp:271)
-    block_add_stat(b.stats, 2, live ? 1u : 0u);
+    block_add_stat(b.stats, 2, live && p < b.npix ? 1u : 0u);
 }
 
 // Before bounce i: clear the next queue, the shadow queue and the traversal work counters;
@@ -228,6 +234,7 @@ __device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, 
 __device__ __forceinline__ bool nrc_terminates(const NrcParams &np, const PathBuffers &b, uint32_t p, uint32_t bounce,
                                                const struct HitRec &h, f3 d)
 {
+    if (p >= b.npix) return bounce >= np.train_term;   // training path
     if (bounce < np.term) return false;
     if (np.footprint_c <= 0.0f || bounce > np.term) return true;
     const float4 st  = b.nstate[p];
@@ -287,11 +294,11 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
     f3             thr  = mk3(thr4.x, thr4.y, thr4.z);
     float4         rad4 = b.rad[p];
     f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
-    const bool     can_continue = bounce + 1 < rp.max_bounces;
+    const bool     can_continue = bounce + 1 < (p >= b.npix ? rp.nrc.train_term + 1 : rp.max_bounces);
     qthr = thr;
     if (tri == kInvalidRef) {
         const f3 s = sky(sc, d);
-        if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
+        if (bounce == 0 && p < b.npix) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
         rad = rad + thr * s;
     } else {
         const uint32_t     inst = sc.instanced && b.hits.inst ? b.hits.inst[k] : kInvalidRef;
@@ -301,22 +308,22 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
         seg = 1;
         if (col.w == 0.0f) {
             // alpha pass-through (renderer.cpp:294-301)
-            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+            if (bounce == 0 && p < b.npix) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
             no = h.point + d * 0.1f;
             nd = d;
             push_next = can_continue;
         } else if (rp.nrc.enabled && nrc_terminates(rp.nrc, b, p, bounce, h, d)) {
             // the cache stands in for this vertex's emission, direct light and everything after it
             const f3 cc = mk3(col.x, col.y, col.z);
-            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, cc, h.normal, h.dist);
+            if (bounce == 0 && p < b.npix) aov_write(b, pix, cc, h.normal, h.dist);
             nrc_query(rp.nrc, h.point, d, h.normal, cc, m, nq);
             do_query = true;
         } else {
             const rnd4 r  = rng4(rp.seed, pix, sample, bounce, 0);
             const f3   n  = h.normal;
             const f3   cc = mk3(col.x, col.y, col.z);
-            if (rp.nrc.record && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
-                const size_t slot = (size_t)p * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
+            if (p >= b.npix && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
+                const size_t slot = (size_t)(p - b.npix) * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
                 nrc_query(rp.nrc, h.point, d, n, cc, m, b.rq + slot * 13);
                 b.r_thr[slot] = make_float4(thr.x, thr.y, thr.z, 1.0f);
                 b.r_rad[slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
@@ -433,7 +440,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
                     nee_n = nf;
                 }
             }
-            if (bounce == 0 && !rp.nrc.train) aov_write(b, pix, rp.mode == kModeCompat ? alb : cc, n, h.dist);
+            if (bounce == 0 && p < b.npix) aov_write(b, pix, rp.mode == kModeCompat ? alb : cc, n, h.dist);
             const f3 thr_prev = thr;
             if (rp.mode == kModeCompat) {
                 thr = thr * alb;
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
             const JobDev &J = jobs[b.pjob[p]];
             shade_path(J.sc, b, J.rp, k, p, b.ppix[p], b.psamp[p], bounce, cur, o);
         } else {
-            shade_path(sc, b, rp, k, p, p, rp.sample, bounce, cur, o);
+            shade_path(sc, b, rp, k, p, p, p >= b.npix ? (rp.sample | 0x80000000u) : rp.sample, bounce, cur, o);
         }
     }
     const uint32_t nqs = (!kMulti && rp.nrc.enabled) ? block_append(&b.cnt[kCntNrcQ], o.do_query) : 0u;
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
         put_ray(b.q[cur ^ 1u], nn, o.no, o.nd);
         b.qpath[cur ^ 1u][nn] = p;
     }
-    block_add_stat(b.stats, 2, o.seg);
+    block_add_stat(b.stats, 2, p < b.npix ? o.seg : 0u);
 }
 
 __global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
@@ -655,7 +662,7 @@ __global__ void k_shadow_swap(PathBuffers b)
 __global__ __launch_bounds__(256) void k_accumulate(PathBuffers b)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= b.P) return;
+    if (p >= b.npix) return;
     const uint32_t x = p % b.W, y = p / b.W;
     const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
     const float4   r = b.rad[p];
@@ -733,7 +740,7 @@ __global__ __launch_bounds__(256) void k_nrc_apply(PathBuffers b)
 __global__ __launch_bounds__(256) void k_nrc_targets(PathBuffers b, uint32_t rec_slots, uint32_t max_out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t p = i / rec_slots;
+    const uint32_t p = b.npix + i / rec_slots;   // training path slot
     bool  ok = false;
     float tg[3] = {0, 0, 0};
     if (p < b.P) {
@@ -844,7 +851,7 @@ static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); 
 
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s)
 {
-    k_generate<<<blocks(b.P), 256, 0, s>>>(sc, b, rp);
+    k_generate<<<blocks(b.P - std::min(rp.nrc.gen_first, b.P)), 256, 0, s>>>(sc, b, rp);
     return hipGetLastError();
 }
 hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)
@@ -877,7 +884,7 @@ hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hi
 }
 hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s)
 {
-    k_accumulate<<<blocks(b.P), 256, 0, s>>>(b);
+    k_accumulate<<<blocks(b.npix), 256, 0, s>>>(b);
     return hipGetLastError();
 }
 hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s)
@@ -898,7 +905,7 @@ hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s)
 }
 hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s)
 {
-    k_nrc_targets<<<blocks((uint64_t)b.P * rec_slots), 256, 0, s>>>(b, rec_slots, max_out);
+    k_nrc_targets<<<blocks((uint64_t)(b.P - b.npix) * rec_slots), 256, 0, s>>>(b, rec_slots, max_out);
     return hipGetLastError();
 }
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,

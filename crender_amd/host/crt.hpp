@@ -70,6 +70,7 @@ struct hit {
     float    distance = __builtin_huge_valf();   // intersection_record::distance (ray.h:12-20)
     uint32_t prim_id  = 0xFFFFFFFFu;
     uint32_t geom_id  = 0xFFFFFFFFu;
+    uint32_t inst_id  = 0xFFFFFFFFu;
     float    u = 0.0f, v = 0.0f;
     float    ng[3] = {0, 0, 0};
 };
@@ -103,6 +104,28 @@ public:
         check(crt_scene_add_model(h_, &md, &id));
         return id;
     }
+    // asset_loader::load_model + scene::add_model (asset_loader.cpp:182-303, scene.cpp:31-34)
+    uint32_t load_model(const std::string &file, const std::string &folder = "")
+    {
+        crt_obj *o = nullptr;
+        check(crt_obj_load(file.c_str(), folder.empty() ? nullptr : folder.c_str(), &o));
+        crt_model_data md{};
+        uint32_t       id = 0;
+        int            rc = crt_obj_model(o, &md);
+        if (rc == CRT_OK) rc = crt_scene_add_model(h_, &md, &id);
+        crt_obj_free(o);
+        check(rc);
+        return id;
+    }
+    // entity::instances (components.h:18-21): column-major 4x4 object->world transforms
+    void set_instances(uint32_t model_id, const std::vector<float> &transforms)
+    {
+        check(crt_scene_set_instances(h_, model_id, transforms.data(), (uint32_t)(transforms.size() / 16)));
+    }
+    void set_materials(uint32_t model_id, const std::vector<crt_material> &m)
+    {
+        check(crt_scene_set_materials(h_, model_id, m.data(), (uint32_t)m.size()));
+    }
     // scene::set_skybox (scene.cpp:36-60); rgba is copied
     void set_skybox(const float *rgba, uint32_t w, uint32_t h)
     {
@@ -127,7 +150,8 @@ public:
         const float tn = 1e-5f, tf = __builtin_huge_valf();
         crt_rays r{&origin[0], &origin[1], &origin[2], &direction[0], &direction[1], &direction[2], &tn, &tf};
         hit      out;
-        crt_hits h{&out.distance, &out.u, &out.v, &out.prim_id, &out.geom_id, &out.ng[0], &out.ng[1], &out.ng[2]};
+        crt_hits h{&out.distance, &out.u,     &out.v,     &out.prim_id, &out.geom_id,
+                   &out.ng[0],    &out.ng[1], &out.ng[2], &out.inst_id};
         check(crt_intersect(h_, &r, 1, &h, CRT_MEM_HOST));
         return out;
     }
@@ -247,5 +271,21 @@ private:
     }
     crt_renderer *h_ = nullptr;
 };
+
+// asset_loader::export_framebuffer (asset_loader.cpp:348-377): the caller names the file
+inline void export_framebuffer(const image &im, const std::string &path, int format)
+{
+    check(crt_image_write(path.c_str(), format, im.data.data(), im.width, im.height));
+}
+
+// Renders many jobs together (crt_render_batch); returns each job's latency in ms.
+inline std::vector<double> render_batch(context &ctx, std::vector<crt_job> &jobs, crt_batch_stats *stats = nullptr)
+{
+    std::vector<double> ms(jobs.size());
+    crt_batch_options   o;
+    crt_batch_options_default(&o);
+    check(crt_render_batch(ctx.get(), jobs.data(), (uint32_t)jobs.size(), &o, ms.data(), stats));
+    return ms;
+}
 
 }  // namespace crt

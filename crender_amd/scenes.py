@@ -388,3 +388,24 @@ def write_obj(spec: SceneSpec, path) -> None:
                 f.write(f"usemtl m{m}\n")
                 cur = m
             f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+def batch_suite(n: int = 64, seed: int = 7):
+    """Jobs of the batched-rendering workload: `n` independent scenes, alternately triangle
+    soup and terrain (with icospheres), triangle counts log-uniform in [1k, 500k],
+    resolutions uniform in {128^2, 256^2, 512^2, 1024^2}, spp uniform in {1, 4, 16, 64}.
+    Returns a list of (SceneSpec, width, height, spp)."""
+    u = Stream(seed, stream=20).uniform(n, dims=4).astype(np.float64)
+    out = []
+    for k in range(n):
+        tris = int(np.exp(np.log(1e3) + u[k, 0] * (np.log(5e5) - np.log(1e3))))
+        res = (128, 256, 512, 1024)[min(3, int(u[k, 1] * 4))]
+        spp = (1, 4, 16, 64)[min(3, int(u[k, 2] * 4))]
+        if k % 2 == 0:
+            spec = soup(tris, seed=seed * 1000 + k, res=(res, res))
+        else:
+            spheres = max(1, tris // 3200)                  # ~10% of the triangles in spheres
+            grid = max(2, int(np.sqrt(max(tris - 320 * spheres, 2) / 2)) + 1)
+            spec = terrain(grid=grid, spheres=spheres, seed=seed * 1000 + k, res=(res, res))
+        out.append((spec, res, res, spp))
+    return out

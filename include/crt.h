@@ -125,6 +125,7 @@ typedef struct crt_options {
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
+#define CRT_FLAG_SORT_RAYS    2u /* reorder secondary ray queues by origin / direction before tracing */
 
 typedef struct crt_render_options {
     uint32_t mode;          /* crt_mode */
