@@ -358,7 +358,7 @@ def terrain(grid: int = 2048, spheres: int = 20_000, seed: int = 3, res=(1920, 1
     mi = np.concatenate([np.zeros(len(tfaces), np.uint32),
                          np.repeat(np.arange(1, spheres + 1, dtype=np.uint32), len(sf))])
     sky = np.array([[[0.45, 0.55, 0.75, 1.0]]], np.float32)
-    cam = camera_dict((0.5, 0.32, -0.25), rotation=(0.0, -18.0, 0.0), fov=55.0)
+    cam = camera_dict((0.5, 0.32, -0.25), rotation=(0.0, 22.0, 0.0), fov=55.0)   # pitched down: horizon in frame
     return SceneSpec(name=f"terrain-{grid}-{spheres}-{seed}", vertices=verts, vertex_indices=faces.reshape(-1),
                      material_indices=mi, materials=mats, sky=sky, sun_enabled=True, sun=None, camera=cam,
                      res=tuple(res), spp=64, max_bounces=8, rr_start=3, mode=1)

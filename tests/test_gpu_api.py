@@ -157,9 +157,15 @@ def test_error_paths():
         sc.add_model(bad)
     with pytest.raises(CrtError, match="STATE"):
         sc.cast_rays(np.zeros((1, 3)), np.ones((1, 3)))        # not committed
-    with pytest.raises(CrtError, match="UNSUPPORTED"):
-        sc.add_model(scenes.cornell(1))
-        sc.set_instances(0, np.eye(4).reshape(-1) * 2)
+    sc.add_model(scenes.cornell(1))
+    with pytest.raises(CrtError, match="INVALID_ARG"):
+        sc.set_instances(0, np.eye(4).reshape(-1) * 2)          # bottom row (0, 0, 0, 2): not affine
+    singular = np.eye(4)
+    singular[1, 1] = 0.0
+    with pytest.raises(CrtError, match="INVALID_ARG"):
+        sc.set_instances(0, singular.reshape(-1))
+    with pytest.raises(CrtError, match="INVALID_ARG"):
+        sc.set_instances(0, np.full(16, np.nan))
     sc.close()
     ctx.close()
 

@@ -91,3 +91,17 @@ def test_sunlit_materials_match_oracle(oracle, mode):
     assert abs(st["extension_rays"] - rs["ext_rays"]) <= 1e-2 * rs["ext_rays"]
     if mode == 0:
         assert abs(st["shadow_rays"] - rs["shadow_rays"]) <= 1e-2 * rs["shadow_rays"]
+
+
+def test_terrain_pbr_matches_oracle(oracle):
+    # the terrain-with-objects workload at a size the FP32 CPU BVH oracle renders in seconds:
+    # fBm terrain, icospheres of every material, sun NEE + sky, pbr estimator, RR from 3
+    spec = scenes.terrain(grid=96, spheres=150, seed=3, res=(64, 36))
+    res, spp, seed = (64, 36), 4, 13
+    gpu = _render(spec, res, spp, seed=seed)
+    bvh = oracle.Bvh(spec.corners())
+    ref = oracle.render(spec, res=res, spp=spp, seed=seed, trav=oracle.BVH32, bvh=bvh)
+    assert _agree(gpu["raw"], ref["raw"], rtol=2e-3, atol=1e-3) >= 0.98
+    assert abs(gpu["raw"].mean() - ref["raw"].mean()) <= 1e-2 * ref["raw"].mean()
+    st, rs = gpu["stats"], ref["stats"]
+    assert abs(st["extension_rays"] - rs["ext_rays"]) <= 1e-2 * rs["ext_rays"]
