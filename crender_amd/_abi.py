@@ -123,6 +123,17 @@ class TuneResult(C.Structure):
                 ("max_paths", C.c_uint32), ("rays_per_second", C.c_double), ("trials", C.c_uint32)]
 
 
+class PostOptions(C.Structure):
+    _fields_ = [("bloom", C.c_int32), ("bloom_threshold", C.c_float), ("bloom_strength", C.c_float),
+                ("gray_scale", C.c_int32), ("tonemapping", C.c_int32), ("tonemapping_type", C.c_int32),
+                ("exposure", C.c_float), ("gamma", C.c_float), ("full_coverage", C.c_int32)]
+
+
+class DenoiseOptions(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("sigma_colour", C.c_float), ("sigma_normal", C.c_float),
+                ("sigma_albedo", C.c_float)]
+
+
 class NrcRenderOptions(C.Structure):
     _fields_ = [("term_bounce", C.c_uint32), ("footprint_c", C.c_float), ("train_paths", C.c_uint32),
                 ("train_suffix", C.c_uint32), ("train_steps", C.c_uint32), ("train", C.c_uint32),
@@ -196,6 +207,10 @@ SIGNATURES = {
     "crt_nrc_render_options_default": (None, [C.POINTER(NrcRenderOptions)]),
     "crt_renderer_set_nrc": (C.c_int, [P, P, C.POINTER(NrcRenderOptions)]),
     "crt_image_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
+    "crt_post_options_default": (None, [C.POINTER(PostOptions)]),
+    "crt_post_process": (C.c_int, [P, C.POINTER(PostOptions), P, P, C.c_uint32, C.c_uint32, C.c_int]),
+    "crt_denoise_options_default": (None, [C.POINTER(DenoiseOptions)]),
+    "crt_denoise": (C.c_int, [P, C.POINTER(DenoiseOptions), P, P, P, P, C.c_uint32, C.c_uint32, C.c_int]),
     "crt_obj_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "crt_obj_model": (C.c_int, [P, C.POINTER(ModelData)]),
     "crt_obj_warnings": (C.c_char_p, [P]),

@@ -2485,3 +2485,137 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
 }
 
 }  // extern "C"
+
+// ---- post-processing and denoising ------------------------------------------------------
+
+namespace {
+
+// Device copies of caller images (host memory) or the caller's device pointers; scratch.
+struct ImageScratch {
+    std::vector<void *> allocs;
+    ~ImageScratch()
+    {
+        for (void *p : allocs) dfree(p);
+    }
+};
+
+int image_in(ImageScratch &sc, const float *p, size_t n4, int memory, hipStream_t s, const float4 **out)
+{
+    if (memory == CRT_MEM_DEVICE) {
+        *out = reinterpret_cast<const float4 *>(p);
+        return CRT_OK;
+    }
+    float4 *d = nullptr;
+    int     st = lalloc(sc.allocs, &d, n4);
+    if (st) return st;
+    HIP_CHECK(hipMemcpyAsync(d, p, sizeof(float4) * n4, hipMemcpyHostToDevice, s));
+    *out = d;
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void crt_post_options_default(crt_post_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->bloom_threshold = 0.7f;   // post_processor.h:22-23
+    o->bloom_strength  = 1.0f;
+    o->exposure        = 1.0f;   // post_processor.h:36-37
+    o->gamma           = 2.2f;
+}
+
+int crt_post_process(crt_context *ctx, const crt_post_options *opt, const float *rgba_in, float *rgba_out,
+                     uint32_t width, uint32_t height, int memory)
+{
+    if (!ctx || !rgba_in || !rgba_out || width == 0 || height == 0) return fail(CRT_ERR_INVALID_ARG, "null or empty image");
+    if (memory != CRT_MEM_HOST && memory != CRT_MEM_DEVICE) return fail(CRT_ERR_INVALID_ARG, "bad memory kind");
+    crt_post_options o;
+    crt_post_options_default(&o);
+    if (opt) o = *opt;
+    if (o.tonemapping && (o.tonemapping_type < 0 || o.tonemapping_type > 3))
+        return fail(CRT_ERR_INVALID_ARG, "tonemapping_type must be 0..3");
+    const size_t n4 = (size_t)width * height;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    if (!o.bloom && !o.gray_scale && !o.tonemapping) {   // post_processor.cpp:112-114
+        const hipMemcpyKind k = memory == CRT_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToHost;
+        HIP_CHECK(hipMemcpyAsync(rgba_out, rgba_in, sizeof(float4) * n4, k, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        return CRT_OK;
+    }
+    ImageScratch sc;
+    const float4 *in = nullptr;
+    int st = image_in(sc, rgba_in, n4, memory, s, &in);
+    if (st) return st;
+    float4 *out = nullptr, *t0 = nullptr, *t1 = nullptr;
+    if (memory == CRT_MEM_DEVICE) out = reinterpret_cast<float4 *>(rgba_out);
+    else if ((st = lalloc(sc.allocs, &out, n4))) return st;
+    if ((st = lalloc(sc.allocs, &t0, n4))) return st;
+    if (o.bloom && (st = lalloc(sc.allocs, &t1, n4))) return st;
+    PostDev p{};
+    p.w = width;
+    p.h = height;
+    p.cover_w = o.full_coverage ? width : width / 8u * 8u;
+    p.cover_h = o.full_coverage ? height : height / 8u * 8u;
+    p.bloom = o.bloom;
+    p.threshold = o.bloom_threshold;
+    p.strength = o.bloom_strength;
+    p.gray = o.gray_scale;
+    p.tonemap = o.tonemapping;
+    p.tonemap_type = o.tonemapping_type;
+    p.exposure = o.exposure;
+    p.gamma = o.gamma;
+    HIP_CHECK(launch_post(p, in, out, t0, t1 ? t1 : t0, s));
+    if (memory != CRT_MEM_DEVICE) HIP_CHECK(hipMemcpyAsync(rgba_out, out, sizeof(float4) * n4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return CRT_OK;
+}
+
+void crt_denoise_options_default(crt_denoise_options *o)
+{
+    if (!o) return;
+    o->levels       = 5;
+    o->sigma_colour = 0.6f;
+    o->sigma_normal = 0.2f;
+    o->sigma_albedo = 0.2f;
+}
+
+int crt_denoise(crt_context *ctx, const crt_denoise_options *opt, const float *colour, const float *normal,
+                const float *albedo, float *rgba_out, uint32_t width, uint32_t height, int memory)
+{
+    if (!ctx || !colour || !normal || !albedo || !rgba_out || width == 0 || height == 0)
+        return fail(CRT_ERR_INVALID_ARG, "null or empty image");
+    if (memory != CRT_MEM_HOST && memory != CRT_MEM_DEVICE) return fail(CRT_ERR_INVALID_ARG, "bad memory kind");
+    crt_denoise_options o;
+    crt_denoise_options_default(&o);
+    if (opt) o = *opt;
+    if (o.levels < 1 || o.levels > 10 || !(o.sigma_colour > 0.0f) || !(o.sigma_normal > 0.0f) ||
+        !(o.sigma_albedo > 0.0f))
+        return fail(CRT_ERR_INVALID_ARG, "bad denoise options");
+    const size_t n4 = (size_t)width * height;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    ImageScratch sc;
+    const float4 *c = nullptr, *n = nullptr, *a = nullptr;
+    int st = image_in(sc, colour, n4, memory, s, &c);
+    if (!st) st = image_in(sc, normal, n4, memory, s, &n);
+    if (!st) st = image_in(sc, albedo, n4, memory, s, &a);
+    if (st) return st;
+    float4 *out = nullptr, *t0 = nullptr, *t1 = nullptr;
+    if (memory == CRT_MEM_DEVICE) out = reinterpret_cast<float4 *>(rgba_out);
+    else if ((st = lalloc(sc.allocs, &out, n4))) return st;
+    if ((st = lalloc(sc.allocs, &t0, n4))) return st;
+    if ((st = lalloc(sc.allocs, &t1, n4))) return st;
+    HIP_CHECK(launch_atrous(c, n, a, out, t0, t1, width, height, (int)o.levels, o.sigma_colour, o.sigma_normal,
+                            o.sigma_albedo, s));
+    if (memory != CRT_MEM_DEVICE) HIP_CHECK(hipMemcpyAsync(rgba_out, out, sizeof(float4) * n4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return CRT_OK;
+}
+
+}  // extern "C"

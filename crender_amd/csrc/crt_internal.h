@@ -201,6 +201,17 @@ size_t     ray_sort_temp_bytes(uint32_t capacity);
 hipError_t launch_ray_sort(const RaySoA &src, const uint32_t *src_path, const uint32_t *src_count, uint32_t n,
                            const RaySoA &dst, uint32_t *dst_path, uint32_t *dst_count, f3 bmin, f3 bmax,
                            const RaySortScratch &w, hipStream_t s);
+// Post-processing (post.hip): image size, the reference's dispatch coverage and settings.
+struct PostDev {
+    uint32_t w, h, cover_w, cover_h;
+    int32_t  bloom;
+    float    threshold, strength;
+    int32_t  gray, tonemap, tonemap_type;
+    float    exposure, gamma;
+};
+hipError_t launch_post(const PostDev &p, const float4 *in, float4 *out, float4 *t0, float4 *t1, hipStream_t s);
+hipError_t launch_atrous(const float4 *colour, const float4 *normal, const float4 *albedo, float4 *out, float4 *t0,
+                         float4 *t1, uint32_t w, uint32_t h, int levels, float sc, float sn, float sa, hipStream_t s);
 uint32_t trace_grid(int device, uint32_t waves_per_cu);
 uint64_t trace_spill_bytes(uint32_t grid);
 constexpr uint32_t kTraceBlock = 256;

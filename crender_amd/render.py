@@ -251,6 +251,33 @@ def write_image(path, rgba, fmt=None) -> None:
     check(A.lib().crt_image_write(str(path).encode(), IMAGE_FORMATS[fmt], fptr(img), img.shape[1], img.shape[0]))
 
 
+def post_process(ctx: Context, rgba, bloom=False, threshold=0.7, strength=1.0, gray=False, tonemap=False,
+                 tonemap_type=0, exposure=1.0, gamma=2.2, full_coverage=False) -> np.ndarray:
+    """crt_post_process of an (H, W, 4) image (cr::post_processor::process)."""
+    L = A.lib()
+    img = np.ascontiguousarray(rgba, dtype=np.float32)
+    o = A.PostOptions()
+    L.crt_post_options_default(C.byref(o))
+    o.bloom, o.bloom_threshold, o.bloom_strength = int(bloom), threshold, strength
+    o.gray_scale, o.tonemapping, o.tonemapping_type = int(gray), int(tonemap), tonemap_type
+    o.exposure, o.gamma, o.full_coverage = exposure, gamma, int(full_coverage)
+    out = np.empty_like(img)
+    check(L.crt_post_process(ctx.handle, C.byref(o), vptr(img), vptr(out), img.shape[1], img.shape[0], A.CRT_MEM_HOST))
+    return out
+
+
+def denoise(ctx: Context, colour, normal, albedo, levels=5, sigma_colour=0.6, sigma_normal=0.2,
+            sigma_albedo=0.2) -> np.ndarray:
+    """crt_denoise: a-trous filter guided by the normal and albedo images (all (H, W, 4))."""
+    L = A.lib()
+    c, n, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (colour, normal, albedo))
+    o = A.DenoiseOptions(levels, sigma_colour, sigma_normal, sigma_albedo)
+    out = np.empty_like(c)
+    check(L.crt_denoise(ctx.handle, C.byref(o), vptr(c), vptr(n), vptr(a), vptr(out), c.shape[1], c.shape[0],
+                        A.CRT_MEM_HOST))
+    return out
+
+
 class ObjModel:
     """crt_obj_load (asset_loader::load_model, asset_loader.cpp:182-303): a parsed OBJ/MTL
     model.  Arrays are copies; add_to() hands the library's own view to a scene."""

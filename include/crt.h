@@ -322,6 +322,38 @@ int         crt_obj_free(crt_obj *obj);
 enum crt_image_format { CRT_IMAGE_PNG = 0, CRT_IMAGE_JPG = 1, CRT_IMAGE_EXR = 2, CRT_IMAGE_HDR = 3, CRT_IMAGE_PFM = 4 };
 int crt_image_write(const char *path, int format, const float *rgba, uint32_t width, uint32_t height);
 
+/* ---- post-processing: cr::post_processor (post_processor.h:19-42, post_processor.cpp,
+ * post_process.comp, blur.comp) on the GPU.  In and out are W*H RGBA FP32 images. */
+typedef struct crt_post_options {
+    int32_t bloom;            /* default 0 */
+    float   bloom_threshold;  /* default 0.7 */
+    float   bloom_strength;   /* default 1 */
+    int32_t gray_scale;       /* default 0 */
+    int32_t tonemapping;      /* default 0 */
+    int32_t tonemapping_type; /* 0 linear, 1 Reinhard, 2 Jim and Richard, 3 Uncharted 2 */
+    float   exposure;         /* default 1 */
+    float   gamma;            /* default 2.2 */
+    int32_t full_coverage;    /* 0: like the reference's floor(W/8) x floor(H/8) dispatch, pixels
+                                 beyond it are 0; 1: every pixel */
+} crt_post_options;
+void crt_post_options_default(crt_post_options *opt);
+int  crt_post_process(crt_context *ctx, const crt_post_options *opt, const float *rgba_in, float *rgba_out,
+                      uint32_t width, uint32_t height, int memory);
+
+/* Denoising of the colour image guided by the normal and albedo images, the inputs
+ * cr::denoise passes to Open Image Denoise (denoise.h:10-42).  OIDN's pretrained network is
+ * not part of this build: this is an edge-avoiding a-trous wavelet filter (Dammertz et al.
+ * 2010) on the GPU.  Images are W*H RGBA FP32; the output alpha is 1. */
+typedef struct crt_denoise_options {
+    uint32_t levels;         /* default 5 (footprint 2^levels * 4 + 1 pixels) */
+    float    sigma_colour;   /* default 0.6, halved per level */
+    float    sigma_normal;   /* default 0.2 (normals stored n * 0.5 + 0.5) */
+    float    sigma_albedo;   /* default 0.2 */
+} crt_denoise_options;
+void crt_denoise_options_default(crt_denoise_options *opt);
+int  crt_denoise(crt_context *ctx, const crt_denoise_options *opt, const float *colour, const float *normal,
+                 const float *albedo, float *out, uint32_t width, uint32_t height, int memory);
+
 /* ---- batched rendering (no counterpart in the reference, whose renderer owns one scene) --
  * Many independent jobs -- a committed scene at its own resolution, sample count and depth
  * -- render together: every traversal launch carries rays of all jobs, and one pass of
