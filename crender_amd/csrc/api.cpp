@@ -2442,7 +2442,7 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
             return st;
         }
     }
-    for (uint32_t cand : {2u, 3u, 4u, 5u}) {
+    for (uint32_t cand : {2u, 3u, 4u, 5u, 6u, 8u}) {
         if (cand == blocks) continue;
         const uint32_t keep = blocks;
         set_grid(cand);

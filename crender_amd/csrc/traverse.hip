@@ -247,8 +247,11 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // one transforms the lane's ray into the instance's object space (direction renormalised,
 // distances scaled by |inv * d|, model.cpp:107-113) and pushes kInstExit, whose pop
 // restores the world ray; closest-hit distances are compared in world units.
+// The 8-entry stack variant is also the high-occupancy one: its 16 KB of LDS per block
+// leaves room for 8 waves per SIMD, so the register budget is capped to match (64 VGPRs).
 template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ, uint32_t *spill)
+__global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ,
+                                                                      uint32_t *spill)
 {
     __shared__ uint32_t s_ref[S * kTraceBlock];
     __shared__ float    s_dist[S * kTraceBlock];

@@ -389,7 +389,7 @@ typedef struct crt_batch_stats {
 
 /* Autotuner of the traversal and wavefront shape for a workload: times short batches of
  * `jobs` (every job at 1 spp, outputs discarded) over BVH width {4, 8} (CRT_TUNE_WIDTH:
- * re-commits the jobs' scenes), traversal blocks per CU {2, 3, 4, 5}, LDS stack depth
+ * re-commits the jobs' scenes), traversal blocks per CU {2, 3, 4, 5, 6, 8}, LDS stack depth
  * {8, 16, 32} and wavefront size {1, 2, 4, 8} M paths, one parameter at a time, and keeps
  * the fastest on the context (used by later batches and renderers).  The context must be
  * idle. */
