@@ -241,9 +241,11 @@ def lib() -> C.CDLL:
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not LIBPATH.exists():
-        raise ImportError(f"{LIBPATH} is missing: run __graft_entry__.build() first")
-    L = C.CDLL(str(LIBPATH))
+    # CRT_LIBRARY selects another build of the same ABI (e.g. a variant under test)
+    path = Path(os.environ.get("CRT_LIBRARY", str(LIBPATH)))
+    if not path.exists():
+        raise ImportError(f"{path} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(str(path))
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)
         f.restype = res
