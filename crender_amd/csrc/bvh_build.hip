@@ -546,6 +546,25 @@ struct LocalTask {
     float    cblo[3], cbhi[3];
 };
 
+constexpr uint32_t kWaveTask   = 128;   // subtrees of at most this many primitives: one wave each
+constexpr uint32_t kMaxWTasks  = 512;   // wave-task list capacity (larger lists fall back to the block)
+constexpr int      kWaveStack  = 8;     // per-wave stack: depth <= log2(kWaveTask)
+
+struct WTask {
+    uint32_t b, e, slot;
+};
+
+// Orders this wave's LDS accesses (lanes of one wave read what others wrote).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One workgroup builds a whole small segment: nodes above kWaveTask primitives block-wide
+// (one node at a time), then every wave takes whole subtrees of at most kWaveTask
+// primitives from a shared list and builds them without block barriers.
 __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t ntasks, uint32_t *idx,
                                                const float4 *plo, const float4 *phi, const float4 *cen,
                                                Bvh2Node *nodes, uint32_t max_leaf)
@@ -560,6 +579,11 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
     __shared__ uint32_t  wsum[4];
     __shared__ int       dec_leaf, dec_axis;
     __shared__ uint32_t  dec_nleft;
+    __shared__ WTask     wtasks[kMaxWTasks];
+    __shared__ uint32_t  wcount, wnext;
+    __shared__ int       wbins[4][kBinWords];
+    __shared__ SahOut    wsah[4];
+    __shared__ LocalTask wstack[4][kWaveStack];
 
     const uint32_t ti = blockIdx.x;
     if (ti >= ntasks) return;
@@ -569,19 +593,30 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
     const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) ids[i] = idx[gb + i];
     if (threadIdx.x == 0) {
-        LocalTask r;
-        r.b    = 0;
-        r.e    = n;
-        r.slot = task.slot;
-        r.pad  = 0;
-        for (int k = 0; k < 3; ++k) {
-            r.cblo[k] = task.cblo[k];
-            r.cbhi[k] = task.cbhi[k];
+        wcount = 0;
+        wnext  = 0;
+        if (n <= kWaveTask) {
+            wtasks[0] = WTask{0, n, task.slot};
+            wcount    = 1;
+            sp        = 0;
+            cur.e     = 0xFFFFFFFFu;
+        } else {
+            LocalTask r;
+            r.b    = 0;
+            r.e    = n;
+            r.slot = task.slot;
+            r.pad  = 0;
+            for (int k = 0; k < 3; ++k) {
+                r.cblo[k] = task.cblo[k];
+                r.cbhi[k] = task.cbhi[k];
+            }
+            stack[0] = r;
+            sp       = 1;
+            cur.e    = 0;
         }
-        stack[0] = r;
-        sp       = 1;
     }
     __syncthreads();
+    // ---- phase A: nodes above kWaveTask primitives, whole block per node ----
     for (;;) {
         if (threadIdx.x == 0 && sp > 0) cur = stack[--sp];
         __syncthreads();
@@ -656,18 +691,124 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                     r.cblo[k] = dec_axis >= 0 ? s.rcblo[k] : cur.cblo[k];
                     r.cbhi[k] = dec_axis >= 0 ? s.rcbhi[k] : cur.cbhi[k];
                 }
-                // larger child first so the smaller is processed next: stack depth <= log2(n)+1
-                if (l.e - l.b >= r.e - r.b) {
-                    stack[sp++] = l;
-                    stack[sp++] = r;
-                } else {
-                    stack[sp++] = r;
-                    stack[sp++] = l;
+                // larger child first so the smaller is processed next: stack depth <= log2(n)+1;
+                // children of at most kWaveTask primitives go to the wave list while it has room
+                const LocalTask ch[2] = {l.e - l.b >= r.e - r.b ? l : r, l.e - l.b >= r.e - r.b ? r : l};
+                for (int j = 0; j < 2; ++j) {
+                    if (ch[j].e - ch[j].b <= kWaveTask && wcount < kMaxWTasks) wtasks[wcount++] = WTask{ch[j].b, ch[j].e, ch[j].slot};
+                    else stack[sp++] = ch[j];
                 }
             }
             if (sp == 0) cur.e = 0xFFFFFFFFu;
         }
         __syncthreads();
+    }
+    __syncthreads();
+    // ---- phase B: subtrees of at most kWaveTask primitives, one wave each, no block barriers ----
+    int       *wb  = wbins[w];
+    SahOut    *ws  = &wsah[w];
+    LocalTask *wst = wstack[w];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&wnext, 1u);
+        t = __shfl(t, 0);
+        if (t >= wcount) break;
+        const WTask wt = wtasks[t];
+        // centroid bounds of the subtree
+        float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+        for (uint32_t i = wt.b + lane; i < wt.e; i += 64) {
+            const float4 c = cen[ids[i]];
+            clo[0] = fminf(clo[0], c.x), clo[1] = fminf(clo[1], c.y), clo[2] = fminf(clo[2], c.z);
+            chi[0] = fmaxf(chi[0], c.x), chi[1] = fmaxf(chi[1], c.y), chi[2] = fmaxf(chi[2], c.z);
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            for (int k = 0; k < 3; ++k) {
+                clo[k] = fminf(clo[k], __shfl_xor(clo[k], o));
+                chi[k] = fmaxf(chi[k], __shfl_xor(chi[k], o));
+            }
+        LocalTask c;
+        c.b    = wt.b;
+        c.e    = wt.e;
+        c.slot = wt.slot;
+        c.pad  = 0;
+        for (int k = 0; k < 3; ++k) {
+            c.cblo[k] = clo[k];
+            c.cbhi[k] = chi[k];
+        }
+        int wsp = 0;   // wave-uniform
+        for (;;) {
+            bins_init(wb, lane, 64);
+            wave_sync();
+            for (uint32_t i = c.b + lane; i < c.e; i += 64) {
+                const uint32_t id = ids[i];
+                bins_add(wb, cen[id], plo[id], phi[id], c.cblo, c.cbhi);
+            }
+            wave_sync();
+            sah_wave(wb, ws);
+            wave_sync();
+            uint32_t nl = 0;
+            int      ax = -1;
+            const int leaf = decide(*ws, max_leaf, nl, ax);   // same result in every lane
+            if (lane == 0) {
+                if (leaf) write_leaf(nodes, c.slot, *ws, gb + c.b, c.e - c.b);
+                else write_inner(nodes, c.slot, *ws, nl);
+            }
+            if (!leaf && ax >= 0) {
+                const int      bin = ws->bin;
+                uint32_t       lcount = 0, rcount = 0;
+                for (uint32_t base = c.b; base < c.e; base += 64) {
+                    const uint32_t i  = base + lane;
+                    const bool     in = i < c.e;
+                    bool           f  = false;
+                    uint32_t       id = 0;
+                    if (in) {
+                        id = ids[i];
+                        const float4 cc4 = cen[id];
+                        const float  cc  = ax == 0 ? cc4.x : (ax == 1 ? cc4.y : cc4.z);
+                        f = bin_of(cc, c.cblo[ax], c.cbhi[ax]) <= bin;
+                    }
+                    const uint64_t bal  = __ballot(f);
+                    const uint32_t rank = __popcll(bal & lt);
+                    if (in) tmp[f ? c.b + lcount + rank : c.b + nl + rcount + ((uint32_t)lane - rank)] = id;
+                    const uint32_t tot = __popcll(bal);
+                    lcount += tot;
+                    rcount += min(64u, c.e - base) - tot;
+                }
+                wave_sync();
+                for (uint32_t i = c.b + lane; i < c.e; i += 64) ids[i] = tmp[i];
+                wave_sync();
+            }
+            if (!leaf) {
+                LocalTask l, r;
+                l.b    = c.b;
+                l.e    = c.b + nl;
+                l.slot = c.slot + 1;
+                l.pad  = 0;
+                r.b    = c.b + nl;
+                r.e    = c.e;
+                r.slot = c.slot + 2 * nl;
+                r.pad  = 0;
+                for (int k = 0; k < 3; ++k) {
+                    l.cblo[k] = ax >= 0 ? ws->lcblo[k] : c.cblo[k];
+                    l.cbhi[k] = ax >= 0 ? ws->lcbhi[k] : c.cbhi[k];
+                    r.cblo[k] = ax >= 0 ? ws->rcblo[k] : c.cblo[k];
+                    r.cbhi[k] = ax >= 0 ? ws->rcbhi[k] : c.cbhi[k];
+                }
+                // the larger child waits, the smaller (at most half of c) is next: every push
+                // halves the size, so at most log2(kWaveTask) = 7 entries are ever held
+                const bool lbig = l.e - l.b >= r.e - r.b;
+                if (lane == 0) wst[wsp] = lbig ? l : r;
+                ++wsp;
+                c = lbig ? r : l;
+                wave_sync();
+                continue;
+            }
+            if (wsp == 0) break;
+            --wsp;
+            wave_sync();
+            c = wst[wsp];
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) idx[gb + i] = ids[i];
