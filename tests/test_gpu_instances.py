@@ -74,13 +74,13 @@ def test_instanced_hits_match_oracle(width):
         a, b, Ta, Tb = _models()
         sc = _scene(ctx, a, b, Ta, Tb)
         print(f"instances width={width}", sc.bvh_info(), flush=True)
-        org, dirs = scenes.diffuse_rays(20000, seed=9, lo=-1.5, hi=2.0)
+        org, dirs = scenes.diffuse_rays(60000, seed=9, lo=-1.5, hi=2.0)
         g = sc.cast_rays(org, dirs)
         m, i, p, d, u, v = O.intersect_instanced([a.corners().reshape(-1, 9), b.corners().reshape(-1, 9)], [Ta, Tb],
                                                  org, dirs)
         hit = m >= 0
         ghit = g["prim_id"] != 0xFFFFFFFF
-        assert hit.sum() > 2000
+        assert hit.sum() > 1500
         assert np.mean(hit == ghit) > 0.999
         both = hit & ghit
         same = (g["geom_id"][both] == m[both]) & (g["inst_id"][both] == i[both]) & (g["prim_id"][both] == p[both])
