@@ -181,7 +181,7 @@ struct crt_renderer {
     crt_nrc               *nrc = nullptr;
     crt_nrc_render_options nopt{};
     double                 ms_nrc_train = 0.0;
-    float                  last_train_loss = 0.0f;
+    uint32_t               loss_tiles = 0;   // tiles of the last training step (loss partials)
     uint32_t               nrc_warm_done = 0;   // warm-up passes run since set_nrc
 };
 
@@ -644,14 +644,8 @@ int nrc_train_step(crt_renderer *r)
         HIP_CHECK(nrc_launch_adam(n->partial, (batch + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                                   n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
     }
-    // loss of the last step: sum of its per-tile partials
-    const uint32_t tiles = (batch + 63) / 64;
-    std::vector<float> pl(tiles);
-    HIP_CHECK(hipMemcpyAsync(pl.data(), n->partial_loss, sizeof(float) * tiles, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    double acc = 0.0;
-    for (float x : pl) acc += x;
-    r->last_train_loss = (float)acc;
+    // the loss of the last step (sum of its per-tile partials) is read when stats ask for it
+    r->loss_tiles = (batch + 63) / 64;
     return CRT_OK;
 }
 
@@ -1746,7 +1740,15 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->ms_render     = r->ms_phase[5];
     stats->extend_launches = r->extend_launches;
     stats->ms_nrc_train    = r->ms_nrc_train;
-    stats->nrc_loss        = r->last_train_loss;
+    if (r->nrc && r->loss_tiles) {
+        std::lock_guard<std::mutex> g(r->nrc->mu);
+        std::vector<float> pl(r->loss_tiles);
+        HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
+        HIP_CHECK(hipMemcpy(pl.data(), r->nrc->partial_loss, sizeof(float) * pl.size(), hipMemcpyDeviceToHost));
+        double acc = 0.0;
+        for (float x : pl) acc += x;
+        stats->nrc_loss = (float)acc;
+    }
     return CRT_OK;
 }
 
@@ -1842,6 +1844,7 @@ extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc
     r->nrc           = nrc;
     r->nopt          = o;
     r->nrc_warm_done = 0;
+    r->loss_tiles    = 0;
     if ((st = alloc_renderer_buffers(r))) return st;
     r->samples.store(0);
     return reset_accum(r);
