@@ -69,22 +69,23 @@ def test_denoise_matches_restatement(ctx):
 
 
 def test_denoise_lowers_render_error(ctx):
+    # cr::denoise is applied to current_progress() (ui.h:607-611): the clamped, gamma-encoded
+    # display image, guided by the last pass's normals and albedos
     spec = scenes.cornell(1)
     sc = scene_from_spec(ctx, spec)
     res = (96, 96)
     imgs = {}
-    for spp in (4, 512):
+    for spp in (16, 1024):
         r = Renderer(ctx, sc, res[0], res[1], 8)
         r.set_resolution(*res)
         r.set_options(mode=1, rr_start=3, seed=spp)
         r.render(spp)
-        imgs[spp] = (r.raw() / spp, r.current_normals(), r.current_albedos())
+        imgs[spp] = (r.current_progress(), r.current_normals(), r.current_albedos())
         r.close()
-    noisy, nrm, alb = imgs[4]
-    ref = imgs[512][0]
-    col = np.concatenate([noisy, np.ones(noisy.shape[:2] + (1,), np.float32)], -1)
-    out = denoise(ctx, col, nrm, alb, sigma_colour=2.0)
-    e_noisy = np.mean((noisy - ref) ** 2)
+    noisy, nrm, alb = imgs[16]
+    ref = imgs[1024][0][..., :3]
+    out = denoise(ctx, noisy, nrm, alb)
+    e_noisy = np.mean((noisy[..., :3] - ref) ** 2)
     e_den = np.mean((out[..., :3] - ref) ** 2)
     print(f"denoise: noisy MSE {e_noisy:.4e}, denoised {e_den:.4e}", flush=True)
     assert e_den < 0.5 * e_noisy
