@@ -2582,7 +2582,7 @@ void crt_denoise_options_default(crt_denoise_options *o)
 {
     if (!o) return;
     o->levels       = 5;
-    o->sigma_colour = 0.6f;
+    o->sigma_colour = 2.0f;
     o->sigma_normal = 0.2f;
     o->sigma_albedo = 0.2f;
 }

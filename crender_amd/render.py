@@ -266,7 +266,7 @@ def post_process(ctx: Context, rgba, bloom=False, threshold=0.7, strength=1.0, g
     return out
 
 
-def denoise(ctx: Context, colour, normal, albedo, levels=5, sigma_colour=0.6, sigma_normal=0.2,
+def denoise(ctx: Context, colour, normal, albedo, levels=5, sigma_colour=2.0, sigma_normal=0.2,
             sigma_albedo=0.2) -> np.ndarray:
     """crt_denoise: a-trous filter guided by the normal and albedo images (all (H, W, 4))."""
     L = A.lib()

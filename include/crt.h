@@ -346,7 +346,7 @@ int  crt_post_process(crt_context *ctx, const crt_post_options *opt, const float
  * 2010) on the GPU.  Images are W*H RGBA FP32; the output alpha is 1. */
 typedef struct crt_denoise_options {
     uint32_t levels;         /* default 5 (footprint 2^levels * 4 + 1 pixels) */
-    float    sigma_colour;   /* default 0.6, halved per level */
+    float    sigma_colour;   /* default 2.0 (display-range colour), halved per level */
     float    sigma_normal;   /* default 0.2 (normals stored n * 0.5 + 0.5) */
     float    sigma_albedo;   /* default 0.2 */
 } crt_denoise_options;

@@ -118,7 +118,7 @@ def post_process(img, bloom=False, threshold=0.7, strength=1.0, gray=False, tone
     return out
 
 
-def atrous(colour, normal, albedo, levels=5, sc=0.6, sn=0.2, sa=0.2):
+def atrous(colour, normal, albedo, levels=5, sc=2.0, sn=0.2, sa=0.2):
     """Edge-avoiding a-trous wavelet filter: per level, a 5x5 B3-spline kernel with holes of
     2^level pixels, each tap weighted by exp(-(|dc|^2/sc_l^2 + |dn|^2/sn^2 + |da|^2/sa^2)),
     sc_l = sc * 2^-level; float64."""

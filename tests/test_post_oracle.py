@@ -88,3 +88,19 @@ def test_atrous_keeps_constants_and_smooths_noise():
     noisy[..., :3] += rng.normal(0, 0.05, (24, 32, 3))
     out = P.atrous(noisy, n, n)
     assert np.mean((out[..., :3] - 0.4) ** 2) < 0.1 * np.mean((noisy[..., :3] - 0.4) ** 2)
+
+
+def test_atrous_defaults_halve_display_error_of_a_16spp_render():
+    """The default tolerances on the renderer's own output: a 16 spp Cornell box display
+    image, guided by that render's normals and albedos, against a 1024 spp one."""
+    import pyoracle as po
+    from crender_amd import scenes
+
+    spec = scenes.cornell(1)
+    bvh = po.Bvh(np.ascontiguousarray(spec.corners(), dtype=np.float32))
+    kw = dict(res=(96, 96), mode=1, rr_start=3, max_bounces=8, trav=po.BVH32, bvh=bvh, nthreads=8)
+    a = po.render(spec, spp=16, seed=16, **kw)
+    b = po.render(spec, spp=1024, seed=1024, aovs=False, **kw)
+    noisy, ref = po.display(a["raw"], 16), po.display(b["raw"], 1024)
+    out = P.atrous(noisy, a["normal"], a["albedo"])[..., :3]
+    assert np.mean((out - ref) ** 2) < 0.5 * np.mean((noisy - ref) ** 2)
