@@ -1152,6 +1152,7 @@ int build_two_level(crt_scene *scene, const float *d_corners, const std::vector<
         if (n == 0 || m.instances.empty()) continue;
         hipError_t e = build_bvh(d_corners + 9 * (size_t)model_first[k], n, width, max_leaf, s, &blas[k]);
         if (e == hipSuccess) e = tri_add_prim_base(blas[k].tri, n, model_first[k], s);
+        if (blas[k].invalid) return fail(CRT_ERR_HIP, "model BVH build: the builder's consistency checks failed");
         if (e != hipSuccess) return fail(CRT_ERR_HIP, std::string("model BVH build: ") + hipGetErrorString(e));
         max_depth = std::max(max_depth, blas[k].depth);
         ms += blas[k].ms;
@@ -1354,6 +1355,7 @@ int crt_scene_commit(crt_scene *scene)
     if (!two_level) {
         hipError_t be = build_bvh(d_corners, ntri, width, max_leaf, s, &br);
         dfree(d_corners);
+        if (br.invalid) return fail(CRT_ERR_HIP, "BVH build: the builder's consistency checks failed");
         if (be != hipSuccess) return fail(CRT_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(be));
         scene->d_nodes = br.nodes;
         scene->d_tri   = br.tri;

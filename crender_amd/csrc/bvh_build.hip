@@ -85,14 +85,14 @@ __device__ __forceinline__ void bins_add(int *bins, float4 c, float4 lo, float4 
 }
 
 struct BinVal {
-    float n;
+    uint32_t n;   // exact count (a float sum loses integers above 2^24)
     float lo[3], hi[3], clo[3], chi[3];
 };
 
 __device__ __forceinline__ BinVal bv_identity()
 {
     BinVal v;
-    v.n = 0.0f;
+    v.n = 0u;
     for (int k = 0; k < 3; ++k) {
         v.lo[k] = kInf;
         v.hi[k] = -kInf;
@@ -158,7 +158,7 @@ __device__ void sah_wave(const int *bins, SahOut *out)
     BinVal    v = bv_identity();
     if (lane < 48) {
         const int *p = bins + (axis * kNumBins + b) * 13;
-        v.n = (float)p[0];
+        v.n = (uint32_t)p[0];
         if (p[0] > 0) {
             for (int k = 0; k < 3; ++k) {
                 v.lo[k]  = ord2f(p[1 + k]);
@@ -179,9 +179,9 @@ __device__ void sah_wave(const int *bins, SahOut *out)
     }
     const BinVal rs = bv_shfl<1>(suf, 1);
     float cost = kInf;
-    if (lane < 48 && b < 15 && pre.n > 0.0f && rs.n > 0.0f) {
-        cost = area3(pre.lo[0], pre.lo[1], pre.lo[2], pre.hi[0], pre.hi[1], pre.hi[2]) * pre.n +
-               area3(rs.lo[0], rs.lo[1], rs.lo[2], rs.hi[0], rs.hi[1], rs.hi[2]) * rs.n;
+    if (lane < 48 && b < 15 && pre.n > 0u && rs.n > 0u) {
+        cost = area3(pre.lo[0], pre.lo[1], pre.lo[2], pre.hi[0], pre.hi[1], pre.hi[2]) * (float)pre.n +
+               area3(rs.lo[0], rs.lo[1], rs.lo[2], rs.hi[0], rs.hi[1], rs.hi[2]) * (float)rs.n;
     }
     // argmin (cost, lane)
     float bc = cost;
@@ -195,7 +195,7 @@ __device__ void sah_wave(const int *bins, SahOut *out)
         }
     }
     // node box and count: full prefix of axis 0 (lane 15)
-    const float tn = __shfl(pre.n, 15);
+    const uint32_t tn = __shfl(pre.n, 15);
     float       nb[6];
     for (int k = 0; k < 3; ++k) {
         nb[k]     = __shfl(pre.lo[k], 15);
@@ -206,7 +206,7 @@ __device__ void sah_wave(const int *bins, SahOut *out)
             out->lo[k] = nb[k];
             out->hi[k] = nb[3 + k];
         }
-        out->n = (uint32_t)tn;
+        out->n = tn;
         out->cost = bc;
         if (bc == kInf) {
             out->axis = -1;
@@ -217,7 +217,7 @@ __device__ void sah_wave(const int *bins, SahOut *out)
         }
     }
     if (bc != kInf && lane == bl) {
-        out->nleft = (uint32_t)pre.n;
+        out->nleft = pre.n;
         for (int k = 0; k < 3; ++k) {
             out->lcblo[k] = pre.clo[k];
             out->lcbhi[k] = pre.chi[k];
@@ -247,8 +247,15 @@ __device__ __forceinline__ int decide(const SahOut &s, uint32_t max_leaf, uint32
     return 0;
 }
 
-__device__ __forceinline__ void write_inner(Bvh2Node *nodes, uint32_t slot, const SahOut &s, uint32_t nleft)
+// Node writers.  A slot or child range outside the tree (nslots = 2 * ntri slots) flags the
+// build as failed instead of writing out of bounds.
+__device__ __forceinline__ void write_inner(Bvh2Node *nodes, uint32_t nslots, uint32_t *err, uint32_t slot,
+                                            const SahOut &s, uint32_t nleft)
 {
+    if (slot >= nslots || nleft == 0u || nleft >= nslots || slot + 2u * nleft >= nslots) {
+        atomicOr(err, 1u);
+        return;
+    }
     Bvh2Node nd;
     for (int k = 0; k < 3; ++k) {
         nd.lo[k] = s.lo[k];
@@ -259,8 +266,13 @@ __device__ __forceinline__ void write_inner(Bvh2Node *nodes, uint32_t slot, cons
     nodes[slot] = nd;
 }
 
-__device__ __forceinline__ void write_leaf(Bvh2Node *nodes, uint32_t slot, const SahOut &s, uint32_t first, uint32_t n)
+__device__ __forceinline__ void write_leaf(Bvh2Node *nodes, uint32_t nslots, uint32_t *err, uint32_t slot,
+                                           const SahOut &s, uint32_t first, uint32_t n)
 {
+    if (slot >= nslots || n == 0u || n > 16u || first > nslots / 2u || n > nslots / 2u - first) {
+        atomicOr(err, 1u);
+        return;
+    }
     Bvh2Node nd;
     for (int k = 0; k < 3; ++k) {
         nd.lo[k] = s.lo[k];
@@ -400,8 +412,9 @@ __global__ __launch_bounds__(256) void k_bin_large(const BuildTask *tasks, const
 
 // one wave per large segment: choose the split, write the node, emit the children
 __global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, uint32_t ntasks, const int *gbins,
-                                                    SplitInfo *split, Bvh2Node *nodes, BuildTask *next_large,
-                                                    BuildTask *small, uint32_t *counts, uint32_t max_leaf)
+                                                    SplitInfo *split, Bvh2Node *nodes, uint32_t nslots,
+                                                    BuildTask *next_large, BuildTask *small, uint32_t *counts,
+                                                    uint32_t max_leaf)
 {
     __shared__ SahOut s;
     const uint32_t t = blockIdx.x;
@@ -423,7 +436,7 @@ __global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, uint
     si.nleft = nleft;
     si.leaf  = 0;
     split[t] = si;
-    write_inner(nodes, tk.slot, s, nleft);
+    write_inner(nodes, nslots, counts + 5, tk.slot, s, nleft);
     BuildTask l, r;
     l.begin = tk.begin;
     l.end   = tk.begin + nleft;
@@ -567,7 +580,7 @@ __device__ __forceinline__ void wave_sync()
 // primitives from a shared list and builds them without block barriers.
 __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t ntasks, uint32_t *idx,
                                                const float4 *plo, const float4 *phi, const float4 *cen,
-                                               Bvh2Node *nodes, uint32_t max_leaf)
+                                               Bvh2Node *nodes, uint32_t nslots, uint32_t *err, uint32_t max_leaf)
 {
     __shared__ uint32_t  ids[kChunk];
     __shared__ uint32_t  tmp[kChunk];
@@ -590,6 +603,10 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
     const BuildTask task = tasks[ti];
     const uint32_t  gb   = task.begin;
     const uint32_t  n    = task.end - task.begin;
+    if (task.end <= task.begin || n > kChunk) {
+        if (threadIdx.x == 0) atomicOr(err, 1u);
+        return;
+    }
     const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) ids[i] = idx[gb + i];
     if (threadIdx.x == 0) {
@@ -636,8 +653,8 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
             dec_leaf    = decide(s, max_leaf, nl, ax);
             dec_nleft   = nl;
             dec_axis    = ax;
-            if (dec_leaf) write_leaf(nodes, cur.slot, s, gb + cur.b, cur.e - cur.b);
-            else write_inner(nodes, cur.slot, s, nl);
+            if (dec_leaf) write_leaf(nodes, nslots, err, cur.slot, s, gb + cur.b, cur.e - cur.b);
+            else write_inner(nodes, nslots, err, cur.slot, s, nl);
         }
         __syncthreads();
         if (!dec_leaf && dec_axis >= 0) {
@@ -674,6 +691,8 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
             }
             for (uint32_t i = cur.b + threadIdx.x; i < cur.e; i += blockDim.x) ids[i] = tmp[i];
         }
+        // every wave has finished reading cur before thread 0 replaces it below
+        __syncthreads();
         if (threadIdx.x == 0) {
             if (!dec_leaf) {
                 LocalTask l, r;
@@ -715,6 +734,10 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
         t = __shfl(t, 0);
         if (t >= wcount) break;
         const WTask wt = wtasks[t];
+        if (wt.b >= wt.e || wt.e > n) {
+            if (lane == 0) atomicOr(err, 1u);
+            continue;
+        }
         // centroid bounds of the subtree
         float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
         for (uint32_t i = wt.b + lane; i < wt.e; i += 64) {
@@ -751,8 +774,8 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
             int      ax = -1;
             const int leaf = decide(*ws, max_leaf, nl, ax);   // same result in every lane
             if (lane == 0) {
-                if (leaf) write_leaf(nodes, c.slot, *ws, gb + c.b, c.e - c.b);
-                else write_inner(nodes, c.slot, *ws, nl);
+                if (leaf) write_leaf(nodes, nslots, err, c.slot, *ws, gb + c.b, c.e - c.b);
+                else write_inner(nodes, nslots, err, c.slot, *ws, nl);
             }
             if (!leaf && ax >= 0) {
                 const int      bin = ws->bin;
@@ -945,6 +968,45 @@ __global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Fron
     }
 }
 
+// Structural check of the finished wide BVH, so that a bad tree fails the build instead of
+// the traversal: child counts in [1, W]; inner children after their parent (which also rules
+// out cycles) and inside the pool; leaf ranges inside the triangle array; quantised child
+// boxes with lo <= hi on every axis (the traversal picks entry planes by ray direction).
+template <int W>
+__global__ void k_check_wide(const void *wide, uint32_t nwide, uint32_t ntri, uint32_t *err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nwide) return;
+    uint32_t nc;
+    uint32_t ch[W];
+    bool     ok = true;
+    if constexpr (W == 4) {
+        const Node4 &n = static_cast<const Node4 *>(wide)[i];
+        nc = n.nchild;
+        for (int j = 0; j < W; ++j) ch[j] = n.child[j];
+    } else {
+        const Node8Q &n = static_cast<const Node8Q *>(wide)[i];
+        nc = n.meta >> 24;
+        for (int j = 0; j < W; ++j) {
+            ch[j] = n.child[j];
+            if ((uint32_t)j < nc)
+                ok = ok && n.qlox[j] <= n.qhix[j] && n.qloy[j] <= n.qhiy[j] && n.qloz[j] <= n.qhiz[j];
+        }
+    }
+    ok = ok && nc >= 1u && nc <= (uint32_t)W;
+    for (int j = 0; j < W; ++j) {
+        if ((uint32_t)j >= nc) break;
+        const uint32_t r = ch[j];
+        if (ref_is_leaf(r)) {
+            const uint32_t first = leaf_first(r), cnt = leaf_count(r);
+            ok = ok && first < ntri && cnt <= ntri - first;
+        } else {
+            ok = ok && r > i && r < nwide;
+        }
+    }
+    if (!ok) atomicOr(err, 1u);
+}
+
 __global__ void k_reorder(const uint32_t *idx, uint32_t n, const float *tri, float4 *out)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1053,7 +1115,8 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         const uint32_t nchunks = hc[2];
         k_bin_large<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, idx, plo, phi, cen, gbins);
         HIP_TRY(hipMemsetAsync(counts + 0, 0, sizeof(uint32_t), s));
-        k_split_large<<<nlarge, 64, 0, s>>>(large_a, nlarge, gbins, split, nodes, large_b, small, counts, max_leaf);
+        k_split_large<<<nlarge, 64, 0, s>>>(large_a, nlarge, gbins, split, nodes, nslots, large_b, small, counts,
+                                            max_leaf);
         k_part_count<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, idx, cen, chunk_left);
         k_part_offsets<<<(nlarge + 63) / 64, 64, 0, s>>>(nlarge, task_chunk0, chunk_left, chunk_loff);
         k_part_scatter<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, chunk_loff, idx, cen, tmp);
@@ -1065,8 +1128,21 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         std::swap(large_a, large_b);
     }
     const uint32_t nsmall = hc[1];
-    if (nsmall) k_small<<<nsmall, 256, 0, s>>>(small, nsmall, idx, plo, phi, cen, nodes, max_leaf);
+    if (nsmall) k_small<<<nsmall, 256, 0, s>>>(small, nsmall, idx, plo, phi, cen, nodes, nslots, counts + 5, max_leaf);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (hc[5] != 0u) {
+        // a node slot or child range fell outside the tree: nothing past this point is safe
+        void *early[] = {plo, phi, cen, idx, tmp, root, nodes, large_a, large_b, small, counts, chunk_task,
+                         chunk_begin, task_chunk0, chunk_left, chunk_loff, split, gbins};
+        for (void *p : early) HIP_TRY(hipFreeAsync(p, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        hipEventDestroy(ev0);
+        hipEventDestroy(ev1);
+        out->invalid = 1;
+        return hipErrorUnknown;
+    }
 
     // SAH cost + used slots
     double   *sah_acc;
@@ -1104,6 +1180,12 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         std::swap(fa, fb);
     }
     const uint32_t nwide = hc[4];
+    if (width == 4) k_check_wide<4><<<(nwide + 255) / 256, 256, 0, s>>>(wide_tmp, nwide, ntri, counts + 5);
+    else k_check_wide<8><<<(nwide + 255) / 256, 256, 0, s>>>(wide_tmp, nwide, ntri, counts + 5);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const bool invalid = hc[5] != 0u;
     void *wide;
     HIP_TRY(hipMalloc(&wide, node_bytes * nwide));
     HIP_TRY(hipMemcpyAsync(wide, wide_tmp, node_bytes * nwide, hipMemcpyDeviceToDevice, s));
@@ -1132,6 +1214,12 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
                      task_chunk0, chunk_left, chunk_loff, split, gbins, sah_acc, used, wide_tmp, fa, fb};
     for (void *p : frees) HIP_TRY(hipFreeAsync(p, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (invalid) {
+        hipFree(wide);
+        hipFree(tri_out);
+        out->invalid = 1;
+        return hipErrorUnknown;
+    }
 
     out->nodes      = wide;
     out->tri        = tri_out;

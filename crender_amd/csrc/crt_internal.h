@@ -149,6 +149,7 @@ struct BuildResult {
     uint32_t root;
     double   sah;
     float    ms;
+    uint32_t invalid = 0;   // 1: the builder's own consistency checks failed (hipErrorUnknown)
 };
 // Builds the BVH of ntri triangles (9 floats each, device memory) on stream s.
 hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf,

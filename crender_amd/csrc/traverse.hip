@@ -12,7 +12,7 @@
 //     an XCD that drains its slice steals from the others.
 //   * refill: when 16 or more lanes of a wave are idle, one lane reserves a run of rays
 //     for all idle lanes with a single atomic; lanes take their ray by ballot rank.
-//   * per-lane short stack in LDS (entry-major, conflict-free; 8, 16 or 32 entries), spilled
+//   * per-lane short stack in LDS (entry-major (ref, distance) pairs; 8, 16 or 32 entries), spilled
 //     a full stack at a time to a per-lane global region when full.  Entries carry the child's entry
 //     distance so popped nodes beyond the current hit are skipped.
 //   * children are ordered by entry distance with a sorting network; the nearest is
@@ -139,8 +139,11 @@ struct NodeOps<8> {
     // Plane distances in FMA form: t = q * (2^e * inv) + (origin - o) * inv, one rounding per
     // plane instead of decoding the plane first.  The rounding of the per-node base and of
     // the FMA is bounded by delta = 2^-20 * (|base| + 255 |2^e inv|) per axis; subtracting
-    // it from the base of the near plane and adding it to the far plane keeps every child
-    // interval a superset of the exact one, at no per-child cost.
+    // it from the base of the entry plane and adding it to the base of the exit plane keeps
+    // every child interval a superset of the exact one, at no per-child cost.
+    // The entry plane of an axis is the low one for a positive direction and the high one for
+    // a negative direction, so the byte words are picked once per node (lo <= hi holds for
+    // every child slot below the child count) and a child costs no min/max per axis.
     static __device__ __forceinline__ int expand(const void *nodes, uint32_t idx, const RayState &r, uint32_t (&key)[8],
                                                  uint32_t (&ref)[8])
     {
@@ -159,27 +162,26 @@ struct NodeOps<8> {
         const float dx = 9.5367431640625e-07f * (fabsf(bx) + 255.0f * fabsf(sx));
         const float dy = 9.5367431640625e-07f * (fabsf(by) + 255.0f * fabsf(sy));
         const float dz = 9.5367431640625e-07f * (fabsf(bz) + 255.0f * fabsf(sz));
-        // lo plane is the near one when inv > 0
-        const float blx = r.inv.x >= 0.0f ? bx - dx : bx + dx, bhx = r.inv.x >= 0.0f ? bx + dx : bx - dx;
-        const float bly = r.inv.y >= 0.0f ? by - dy : by + dy, bhy = r.inv.y >= 0.0f ? by + dy : by - dy;
-        const float blz = r.inv.z >= 0.0f ? bz - dz : bz + dz, bhz = r.inv.z >= 0.0f ? bz + dz : bz - dz;
+        const float bnx = bx - dx, bfx = bx + dx, bny = by - dy, bfy = by + dy, bnz = bz - dz, bfz = bz + dz;
+        // words [0..1] hold the low bytes of children 0-3 / 4-7, words [2..3] the high bytes
+        const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+        const uint32_t nx0 = px ? qx.x : qx.z, nx1 = px ? qx.y : qx.w, fx0 = px ? qx.z : qx.x, fx1 = px ? qx.w : qx.y;
+        const uint32_t ny0 = py ? qy.x : qy.z, ny1 = py ? qy.y : qy.w, fy0 = py ? qy.z : qy.x, fy1 = py ? qy.w : qy.y;
+        const uint32_t nz0 = pz ? qz.x : qz.z, nz1 = pz ? qz.y : qz.w, fz0 = pz ? qz.z : qz.x, fz1 = pz ? qz.w : qz.y;
         const float tn0 = fmaxf(r.tn, 0.0f);
         const uint32_t nc = h.w >> 24;
         int nh = 0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int sh = 8 * (j & 3);
-            const uint32_t wlo_x = j < 4 ? qx.x : qx.y, whi_x = j < 4 ? qx.z : qx.w;
-            const uint32_t wlo_y = j < 4 ? qy.x : qy.y, whi_y = j < 4 ? qy.z : qy.w;
-            const uint32_t wlo_z = j < 4 ? qz.x : qz.y, whi_z = j < 4 ? qz.z : qz.w;
-            const float t0x = fmaf((float)((wlo_x >> sh) & 0xFFu), sx, blx);
-            const float t1x = fmaf((float)((whi_x >> sh) & 0xFFu), sx, bhx);
-            const float t0y = fmaf((float)((wlo_y >> sh) & 0xFFu), sy, bly);
-            const float t1y = fmaf((float)((whi_y >> sh) & 0xFFu), sy, bhy);
-            const float t0z = fmaf((float)((wlo_z >> sh) & 0xFFu), sz, blz);
-            const float t1z = fmaf((float)((whi_z >> sh) & 0xFFu), sz, bhz);
-            const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tn0));
-            const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), r.tmax));
+            const float tnx = fmaf((float)(((j < 4 ? nx0 : nx1) >> sh) & 0xFFu), sx, bnx);
+            const float tfx = fmaf((float)(((j < 4 ? fx0 : fx1) >> sh) & 0xFFu), sx, bfx);
+            const float tny = fmaf((float)(((j < 4 ? ny0 : ny1) >> sh) & 0xFFu), sy, bny);
+            const float tfy = fmaf((float)(((j < 4 ? fy0 : fy1) >> sh) & 0xFFu), sy, bfy);
+            const float tnz = fmaf((float)(((j < 4 ? nz0 : nz1) >> sh) & 0xFFu), sz, bnz);
+            const float tfz = fmaf((float)(((j < 4 ? fz0 : fz1) >> sh) & 0xFFu), sz, bfz);
+            const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tn0));
+            const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, r.tmax));
             const bool hit = (uint32_t)j < nc && tmin <= tmax;
             key[j] = hit ? make_key(tmin, j) : kMissKey;
             nh += hit ? 1 : 0;
@@ -190,8 +192,8 @@ struct NodeOps<8> {
 
 template <uint32_t S>
 struct Stack {
-    uint32_t *s_ref;    // LDS [S][kTraceBlock]
-    float    *s_dist;
+    uint2    *s;        // LDS [S + 1][kTraceBlock] of (ref, entry distance bits); row S absorbs
+                        // the writes of children that missed
     uint32_t *g;        // global spill, entry-major over all lanes of the grid: [k][2][nlanes]
     uint32_t  nlanes, gl, tid;
     uint32_t  sp, gsp;
@@ -208,15 +210,37 @@ struct Stack {
             }
             for (uint32_t k = 0; k < S; ++k) {
                 const size_t e = (size_t)(gsp + k);
-                g[(e * 2 + 0) * nlanes + gl] = s_ref[k * kTraceBlock + tid];
-                g[(e * 2 + 1) * nlanes + gl] = __float_as_uint(s_dist[k * kTraceBlock + tid]);
+                const uint2  v = s[k * kTraceBlock + tid];
+                g[(e * 2 + 0) * nlanes + gl] = v.x;
+                g[(e * 2 + 1) * nlanes + gl] = v.y;
             }
             gsp += S;
             sp = 0;
         }
-        s_ref[sp * kTraceBlock + tid]  = ref;
-        s_dist[sp * kTraceBlock + tid] = dist;
+        s[sp * kTraceBlock + tid] = make_uint2(ref, __float_as_uint(dist));
         ++sp;
+    }
+    // Pushes the hit children of a node, keys sorted ascending with the nh - 1 hits behind
+    // the nearest in slots 1..nh-1, so that slot 1 ends on top.  When they fit, every slot
+    // is written unconditionally (misses into row S), which keeps the node step free of
+    // per-child branches.
+    template <int W>
+    __device__ __forceinline__ void push_children(const uint32_t (&key)[W], const uint32_t (&ref)[W], int nh)
+    {
+        const uint32_t m = (uint32_t)nh - 1u;
+        if (sp + m <= S) {
+#pragma unroll
+            for (int k = 1; k < W; ++k) {
+                const uint32_t pos = k < nh ? sp + m - (uint32_t)k : S;
+                s[pos * kTraceBlock + tid] = make_uint2(ref[k], key[k] & ~7u);
+            }
+            sp += m;
+        } else {
+#pragma unroll
+            for (int k = W - 1; k >= 1; --k) {
+                if (key[k] != kMissKey) push(ref[k], __uint_as_float(key[k] & ~7u));
+            }
+        }
     }
     // pops the next entry closer than tmax; kEmpty when the stack is exhausted
     __device__ __forceinline__ uint32_t pop(float tmax)
@@ -227,14 +251,13 @@ struct Stack {
                 gsp -= S;
                 for (uint32_t k = 0; k < S; ++k) {
                     const size_t e = (size_t)(gsp + k);
-                    s_ref[k * kTraceBlock + tid]  = g[(e * 2 + 0) * nlanes + gl];
-                    s_dist[k * kTraceBlock + tid] = __uint_as_float(g[(e * 2 + 1) * nlanes + gl]);
+                    s[k * kTraceBlock + tid] = make_uint2(g[(e * 2 + 0) * nlanes + gl], g[(e * 2 + 1) * nlanes + gl]);
                 }
                 sp = S;
             }
             --sp;
-            const float d = s_dist[sp * kTraceBlock + tid];
-            if (d <= tmax) return s_ref[sp * kTraceBlock + tid];
+            const uint2 v = s[sp * kTraceBlock + tid];
+            if (__uint_as_float(v.y) <= tmax) return v.x;
         }
     }
 };
@@ -247,14 +270,13 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // one transforms the lane's ray into the instance's object space (direction renormalised,
 // distances scaled by |inv * d|, model.cpp:107-113) and pushes kInstExit, whose pop
 // restores the world ray; closest-hit distances are compared in world units.
-// The 8-entry stack variant is also the high-occupancy one: its 16 KB of LDS per block
+// The 8-entry stack variant is also the high-occupancy one: its 18 KB of LDS per block
 // leaves room for 8 waves per SIMD, so the register budget is capped to match (64 VGPRs).
 template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst>
 __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ,
                                                                       uint32_t *spill)
 {
-    __shared__ uint32_t s_ref[S * kTraceBlock];
-    __shared__ float    s_dist[S * kTraceBlock];
+    __shared__ uint2 s_stack[(S + 1) * kTraceBlock];
     const uint32_t n = *q.count;
     if (n == 0) return;
     const uint32_t lane  = lane_id();
@@ -262,8 +284,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
     const uint32_t slice = (n + kXcds - 1) / kXcds;
 
     Stack<S> st;
-    st.s_ref  = s_ref;
-    st.s_dist = s_dist;
+    st.s      = s_stack;
     st.g      = spill;
     st.nlanes = gridDim.x * blockDim.x;
     st.gl     = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,10 +417,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 } else {
                     sort_pairs<W>(key, ref);
                     cur = ref[0];
-#pragma unroll
-                    for (int k = W - 1; k >= 1; --k) {
-                        if (key[k] != kMissKey) st.push(ref[k], __uint_as_float(key[k] & ~7u));
-                    }
+                    st.template push_children<W>(key, ref, nh);
                 }
             }
         }
