@@ -16,7 +16,10 @@
 //     a full stack at a time to a per-lane global region when full.  Entries carry the child's entry
 //     distance so popped nodes beyond the current hit are skipped.
 //   * children are ordered by entry distance with a sorting network; the nearest is
-//     visited next and the others pushed far-to-near.
+//     visited next and the others pushed far-to-near, with unconditional LDS writes.
+//   * leaf parking: a lane that reaches a leaf keeps it aside and continues down the tree
+//     until every lane of the wave holds a leaf, so the triangle phase runs with more lanes
+//     busy (Aila and Laine 2009, "speculative traversal").
 #include <hip/hip_runtime.h>
 #include "crt_internal.h"
 
@@ -340,6 +343,10 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             exit_instance();
         }
     };
+    // leaf parked by the inner-node phase (not for two-level scenes, whose instance leaves
+    // change the lane's ray)
+    constexpr bool kSpec = !kInst;
+    uint32_t pend = kEmpty;
     bool     active = false, exhausted = false;
     uint32_t next_slice = 0;   // slices tried by this wave (wave-uniform)
 
@@ -383,6 +390,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 r.best_prim = kInvalidRef;
                 r.bu = r.bv = 0.0f;
                 st.sp = st.gsp = 0;
+                pend      = kEmpty;
                 level     = 0;
                 best_inst = kInvalidRef;
                 uint32_t root = sc.root, ntri = sc.ntri;
@@ -405,10 +413,16 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             if (exhausted) break;
             continue;
         }
-        // ---- inner-node phase: lanes expand nodes until every lane holds a leaf or is done ----
+        // ---- inner-node phase: lanes expand nodes until every lane holds a leaf or is done.
+        // kSpec: a lane whose next node is a leaf parks it and keeps traversing until every
+        // lane has a leaf, so that the leaf phase runs with more lanes busy ----
         for (;;) {
+            if (kSpec && active && pend == kEmpty && cur != kEmpty && ref_is_leaf(cur)) {
+                pend = cur;
+                cur  = next();
+            }
             const bool inner = active && cur != kEmpty && !ref_is_leaf(cur);
-            if (__ballot(inner) == 0) break;
+            if (__ballot(inner && (!kSpec || pend == kEmpty)) == 0) break;
             if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
@@ -421,9 +435,10 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 }
             }
         }
-        // ---- leaf phase ----
+        // ---- leaf phase: the parked leaf first, then leaves popped after it ----
         for (;;) {
-            const bool leaf = active && cur != kEmpty && ref_is_leaf(cur);
+            const bool parked = kSpec && active && pend != kEmpty;
+            const bool leaf   = active && (parked || (cur != kEmpty && ref_is_leaf(cur)));
             if (__ballot(leaf) == 0) break;
             if (kInst && leaf && level == 0) {
                 // instance leaf: enter the first instance, keep the rest for later
@@ -431,7 +446,8 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 if (cnt > 1) st.push(ref_leaf(first + 1, cnt - 1), -kInf);
                 cur = enter_instance(sc.tlas_inst[first]);
             } else if (leaf) {
-                const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+                const uint32_t lf = parked ? pend : cur;
+                const uint32_t first = leaf_first(lf), cnt = leaf_count(lf);
                 bool any = false;
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const size_t  ti = (size_t)(first + k);
@@ -459,7 +475,12 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                         }
                     }
                 }
-                cur = (kAny && any) ? kEmpty : next();
+                if (parked) {
+                    pend = kEmpty;
+                    if (kAny && any) cur = kEmpty;
+                } else {
+                    cur = (kAny && any) ? kEmpty : next();
+                }
             }
         }
         if (active && cur == kEmpty) {
