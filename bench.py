@@ -275,8 +275,8 @@ def main(argv=None) -> int:
             "workload": f"triangle soup {args.tris} tris, 1920x1080, 1 spp, primary + 3 diffuse bounces "
                         f"(depth 4), compat estimator, sun off, constant sky",
             "bvh_width": info["width"],
-            "waves_per_cu": args.waves_per_cu,
-            "trace_stack": args.stack or 16,
+            "waves_per_cu": args.waves_per_cu or 8,
+            "trace_stack": args.stack or 32,
             "sort_rays": args.sort_rays,
             "device": dev,
         },

@@ -2428,7 +2428,7 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
         return CRT_OK;
     };
     if (ctx->batch_paths == 0) ctx->batch_paths = 4u << 20;
-    uint32_t blocks = ctx->opt.trace_waves_per_cu ? (ctx->opt.trace_waves_per_cu + 3) / 4 : 4;
+    uint32_t blocks = ctx->opt.trace_waves_per_cu ? (ctx->opt.trace_waves_per_cu + 3) / 4 : 2;
     uint32_t width  = scenes[0]->dev.width;
     int st = recommit(width);   // one width for the batch
     if (st) return st;

@@ -217,7 +217,7 @@ uint32_t trace_grid(int device, uint32_t waves_per_cu);
 uint64_t trace_spill_bytes(uint32_t grid);
 constexpr uint32_t kTraceBlock = 256;
 constexpr uint32_t kXcds       = 8;
-constexpr uint32_t kStackLds   = 16;    // default LDS stack entries per lane
+constexpr uint32_t kStackLds   = 32;    // default LDS stack entries per lane
 constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multiple of kStackLds)
 
 // ---- integrator ---------------------------------------------------------------------

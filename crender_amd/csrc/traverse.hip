@@ -505,7 +505,8 @@ uint32_t trace_grid(int device, uint32_t blocks_per_cu)
 {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) != hipSuccess) return 1024;
-    if (blocks_per_cu == 0) blocks_per_cu = 4;
+    // two 256-lane blocks per CU: fewer rays in flight keep more of the BVH in cache
+    if (blocks_per_cu == 0) blocks_per_cu = 2;
     uint32_t g = (uint32_t)p.multiProcessorCount * blocks_per_cu;
     // whole multiple of the XCD count so every XCD gets the same number of blocks
     g = (g + kXcds - 1) / kXcds * kXcds;

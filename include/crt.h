@@ -119,9 +119,9 @@ typedef struct crt_options {
     int32_t  device;        /* HIP device ordinal */
     uint32_t bvh_width;     /* 0 = autotune, 2/4/8 */
     uint32_t max_leaf;      /* 0 = default (4) */
-    uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy */
+    uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy (8 waves per CU) */
     uint32_t flags;         /* CRT_FLAG_* */
-    uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 16), 8, 16 or 32 */
+    uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 32), 8, 16 or 32 */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
