@@ -48,15 +48,17 @@ def cpu_baseline(spec, res, bounces, seconds_target: float, threads: int) -> dic
     bvh = O.Bvh(spec.corners())
     build_s = time.perf_counter() - t0
     W, H = res
-    # calibrate on a few rows, then size the band for the target CPU time
-    cal_rows = 8
-    r = O.render(spec, res=res, spp=1, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh,
-                 rows=(H // 2, H // 2 + cal_rows), nthreads=threads, aovs=False)
-    per_row = max(r["stats"]["seconds"], 1e-6) / cal_rows
-    nrows = int(min(H, max(cal_rows, seconds_target / per_row)))
-    # a full frame that finishes early is repeated over several sample passes
-    spp = 1 if nrows < H else int(min(64, max(1, round(seconds_target / (per_row * H)))))
-    r0 = max(0, H // 2 - nrows // 2)
+    # calibrate on one full-frame pass, then size the sample for the target CPU time: whole
+    # frames over several sample passes, or a band of rows when one frame takes longer
+    r = O.render(spec, res=res, spp=1, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh, nthreads=threads,
+                 aovs=False)
+    per_frame = max(r["stats"]["seconds"], 1e-6)
+    if per_frame < seconds_target:
+        nrows, r0 = H, 0
+        spp = int(min(256, max(1, round(seconds_target / per_frame))))
+    else:
+        nrows = int(min(H, max(8, H * seconds_target / per_frame)))
+        r0, spp = max(0, H // 2 - nrows // 2), 1
     r = O.render(spec, res=res, spp=spp, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh,
                  rows=(r0, r0 + nrows), nthreads=threads, aovs=False)
     st = r["stats"]
