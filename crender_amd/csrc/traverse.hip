@@ -18,8 +18,8 @@
 //   * children are ordered by entry distance with a sorting network; the nearest is
 //     visited next and the others pushed far-to-near, with unconditional LDS writes.
 //   * leaf parking: a lane that reaches a leaf keeps it aside and continues down the tree
-//     until every lane of the wave holds a leaf, so the triangle phase runs with more lanes
-//     busy (Aila and Laine 2009, "speculative traversal").
+//     until (nearly) every lane of the wave holds a leaf, so the triangle phase runs with
+//     more lanes busy (Aila and Laine 2009, "speculative traversal").
 #include <hip/hip_runtime.h>
 #include "crt_internal.h"
 
@@ -31,6 +31,7 @@ constexpr float kInf = __builtin_huge_valf();
 constexpr uint32_t kEmpty = kInvalidRef;
 constexpr uint32_t kMissKey = 0xFFFFFFFFu;
 constexpr uint32_t kRefillIdle = 16;
+constexpr uint32_t kParkMin    = 8;            // leaf parking ends when fewer lanes search
 constexpr uint32_t kInstExit   = 0xFFFFFFFEu;   // stack marker: leave the current instance
 // Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
 // 1 + 2*gamma(3) keeps FP32 box tests conservative.
@@ -422,7 +423,11 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 cur  = next();
             }
             const bool inner = active && cur != kEmpty && !ref_is_leaf(cur);
-            if (__ballot(inner && (!kSpec || pend == kEmpty)) == 0) break;
+            const uint64_t srch = __ballot(inner && (!kSpec || pend == kEmpty));
+            if (srch == 0) break;
+            // fewer than kParkMin lanes still searching and some holding a leaf: run the leaf
+            // phase now rather than keep most of the wave waiting
+            if (kSpec && __popcll(srch) < kParkMin && __ballot(active && pend != kEmpty) != 0) break;
             if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
