@@ -641,7 +641,7 @@ int nrc_train_step(crt_renderer *r)
         const float *tt = t.tt + (size_t)k * batch * 3;
         HIP_CHECK(nrc_launch_train(q, tt, batch, n->W, n->WT, n->act, n->partial, n->partial_loss, s));
         n->step++;
-        HIP_CHECK(nrc_launch_adam(n->partial, (batch + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
+        HIP_CHECK(nrc_launch_adam(n->partial, nrc_partial_rows(batch), n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                                   n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
     }
     // the loss of the last step (sum of its per-tile partials) is read when stats ask for it
@@ -2042,7 +2042,7 @@ int crt_nrc_gradient(crt_nrc *n, const float *queries, const float *targets, uin
     std::lock_guard<std::mutex> g(n->mu);
     int st = nrc_fwd_bwd(n, queries, targets, count, memory, loss);
     if (st) return st;
-    HIP_CHECK(nrc_launch_sum_partials(n->partial, (count + 63) / 64, n->grad, n->ctx->stream));
+    HIP_CHECK(nrc_launch_sum_partials(n->partial, nrc_partial_rows(count), n->grad, n->ctx->stream));
     HIP_CHECK(hipMemcpyAsync(grad, n->grad, sizeof(float) * kNrcParams, hipMemcpyDeviceToHost, n->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(n->ctx->stream));
     return CRT_OK;
@@ -2056,7 +2056,7 @@ int crt_nrc_train_step(crt_nrc *n, const float *queries, const float *targets, u
     int st = nrc_fwd_bwd(n, queries, targets, count, memory, loss);
     if (st) return st;
     n->step++;
-    HIP_CHECK(nrc_launch_adam(n->partial, (count + 63) / 64, n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
+    HIP_CHECK(nrc_launch_adam(n->partial, nrc_partial_rows(count), n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                               n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, n->ctx->stream));
     HIP_CHECK(hipStreamSynchronize(n->ctx->stream));
     return CRT_OK;

@@ -157,8 +157,9 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const float *__restrict__ q, 
     }
 }
 
-// One wave = one tile of 64 training samples.  Writes the tile's gradient partial
-// (all parameters) and loss partial.
+// One wave = one tile of 64 training samples; a workgroup's four tiles add their weight
+// gradients in LDS (wave 0 + 1 + 2 + 3, always in that order) and write one partial row
+// (all parameters) per workgroup; the loss partial stays per tile.
 __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, const float *__restrict__ target,
                                                    uint32_t n, float inv_count, const bf16 *__restrict__ W,
                                                    const bf16 *__restrict__ WT, bf16 *__restrict__ act,
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
     __shared__ bf16 xs[4 * 64 * kXStride];
     __shared__ bf16 gs[4 * 64 * kGStride];    // G sample-major [s][n]
     __shared__ bf16 gts[4 * 64 * kGStride];   // G feature-major [n][s]
+    __shared__ float red[4 * 16 * kNrcHid];     // per wave: 16 rows of one layer's dW
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r16 = lane & 15, g = lane >> 4;
     const uint32_t tile = blockIdx.x * 4 + wv;
     const uint32_t s0 = tile * 64;
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
     if (lane == 0) partial_loss[tile] = lsum;
     __syncthreads();
-    float *P = partial + (size_t)tile * kNrcParams;
+    float *P = partial + (size_t)blockIdx.x * kNrcParams;
     // ---- backward ----
 #define BWD(L)                                                                                              \
     {                                                                                                       \
@@ -246,8 +248,12 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
             for (int ct = 0; ct < N / 16; ++ct) {                                                           \
                 f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};                                                    \
                 for (int ss = 0; ss < 2; ++ss) d = mma(a[ss], lds8(Gt + (ct * 16 + r16) * kGStride + ss * 32 + 8 * g), d); \
-                for (int i = 0; i < 4; ++i) P[nrc_w_off(L) + (kt * 16 + 4 * g + i) * N + ct * 16 + r16] = d[i]; \
+                for (int i = 0; i < 4; ++i) red[(wv * 16 + 4 * g + i) * N + ct * 16 + r16] = d[i];           \
             }                                                                                               \
+            __syncthreads();                                                                                \
+            for (int e = threadIdx.x; e < 16 * N; e += 256)                                                 \
+                P[nrc_w_off(L) + kt * 16 * N + e] = ((red[e] + red[16 * N + e]) + red[32 * N + e]) + red[48 * N + e]; \
+            __syncthreads();                                                                                \
         }                                                                                                   \
         if (L > 0) {                                                                                        \
             /* dX = (G W_L^T) * relu'(X_L) : rows s, columns k; reduction over n */                          \
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
 // Adam on the FP32 master weights from the summed per-tile partials; refreshes the bf16
 // copies.  Padded outputs (columns >= 3 of the last layer) and padded input features
 // (rows >= 83 of the first layer) stay zero.
-__global__ void k_nrc_adam(const float *__restrict__ partial, uint32_t ntiles, float *params, float *m, float *v,
+__global__ void k_nrc_adam(const float *__restrict__ partial, uint32_t nrows, float *params, float *m, float *v,
                            bf16 *W, bf16 *WT, float lr, float b1, float b2, float eps, float bc1, float bc2, int update)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,7 +306,7 @@ __global__ void k_nrc_adam(const float *__restrict__ partial, uint32_t ntiles, f
     float w = params[p];
     if (update && !frozen) {
         float gsum = 0.0f;
-        for (uint32_t t = 0; t < ntiles; ++t) gsum += partial[(size_t)t * kNrcParams + p];
+        for (uint32_t t = 0; t < nrows; ++t) gsum += partial[(size_t)t * kNrcParams + p];
         const float mm = b1 * m[p] + (1.0f - b1) * gsum;
         const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
         m[p] = mm;
@@ -316,12 +322,12 @@ __global__ void k_nrc_adam(const float *__restrict__ partial, uint32_t ntiles, f
     WT[nrc_w_off(l) + c * K + k] = to_bf16(w);
 }
 
-__global__ void k_nrc_sum_partials(const float *__restrict__ partial, uint32_t ntiles, float *out)
+__global__ void k_nrc_sum_partials(const float *__restrict__ partial, uint32_t nrows, float *out)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (uint32_t)kNrcParams) return;
     float s = 0.0f;
-    for (uint32_t t = 0; t < ntiles; ++t) s += partial[(size_t)t * kNrcParams + p];
+    for (uint32_t t = 0; t < nrows; ++t) s += partial[(size_t)t * kNrcParams + p];
     out[p] = s;
 }
 
@@ -354,19 +360,19 @@ hipError_t nrc_launch_train(const float *q, const float *target, uint32_t n, con
     return hipGetLastError();
 }
 
-hipError_t nrc_launch_adam(const float *partial, uint32_t ntiles, float *params, float *m, float *v, __bf16 *W,
+hipError_t nrc_launch_adam(const float *partial, uint32_t nrows, float *params, float *m, float *v, __bf16 *W,
                            __bf16 *WT, float lr, float b1, float b2, float eps, uint64_t step, int update,
                            hipStream_t s)
 {
     const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
-    k_nrc_adam<<<(kNrcParams + 255) / 256, 256, 0, s>>>(partial, ntiles, params, m, v, W, WT, lr, b1, b2, eps,
+    k_nrc_adam<<<(kNrcParams + 255) / 256, 256, 0, s>>>(partial, nrows, params, m, v, W, WT, lr, b1, b2, eps,
                                                          update ? bc1 : 1.0f, update ? bc2 : 1.0f, update);
     return hipGetLastError();
 }
 
-hipError_t nrc_launch_sum_partials(const float *partial, uint32_t ntiles, float *out, hipStream_t s)
+hipError_t nrc_launch_sum_partials(const float *partial, uint32_t nrows, float *out, hipStream_t s)
 {
-    k_nrc_sum_partials<<<(kNrcParams + 255) / 256, 256, 0, s>>>(partial, ntiles, out);
+    k_nrc_sum_partials<<<(kNrcParams + 255) / 256, 256, 0, s>>>(partial, nrows, out);
     return hipGetLastError();
 }
 
