@@ -20,9 +20,8 @@ __host__ __device__ constexpr int nrc_w_off(int l)
     return l == 0 ? 0 : kNrcIn * kNrcHid + (l - 1) * kNrcHid * kNrcHid;
 }
 
-// Gradient partial rows written by nrc_launch_train for n samples: one per workgroup of
-// four 64-sample tiles (the workgroup sums its tiles in LDS, in a fixed order).
-__host__ __device__ constexpr uint32_t nrc_partial_rows(uint32_t n) { return ((n + 63u) / 64u + 3u) / 4u; }
+// Gradient partial rows written by nrc_launch_train for n samples: one per 64-sample tile.
+__host__ __device__ constexpr uint32_t nrc_partial_rows(uint32_t n) { return (n + 63u) / 64u; }
 
 hipError_t nrc_launch_infer(const float *q, float *out, uint32_t n, const __bf16 *WT, hipStream_t s);
 hipError_t nrc_launch_train(const float *q, const float *target, uint32_t n, const __bf16 *W, const __bf16 *WT,

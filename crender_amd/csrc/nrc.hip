@@ -15,10 +15,12 @@
 // A and B use the same element mapping, so the reduction is exact whatever order the
 // hardware sums the 32 products in.
 //
-// Each wave owns 64 samples (4 row tiles of 16); activations between layers go through
-// the wave's slice of LDS.  Training stores every layer's input activations (sample-major,
-// bf16) in a global scratch so the backward pass can form dW = X^T G, writes per-wave
-// gradient partials (no atomics), and the Adam kernel sums the partials per parameter.
+// Inference: each wave owns 64 samples (4 row tiles of 16); activations between layers go
+// through the wave's slice of LDS.  Training: a workgroup of four waves owns 64 samples and
+// splits every layer's output columns over its waves; it stores every layer's input
+// activations (sample-major, bf16) in a global scratch so the backward pass can form
+// dW = X^T G, writes per-tile gradient partials (no atomics), and the Adam kernel sums the
+// partials per parameter.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "crt_internal.h"
@@ -157,44 +159,86 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const float *__restrict__ q, 
     }
 }
 
-// One wave = one tile of 64 training samples; a workgroup's four tiles add their weight
-// gradients in LDS (wave 0 + 1 + 2 + 3, always in that order) and write one partial row
-// (all parameters) per workgroup; the loss partial stays per tile.
+// One part of a query's features, for the four lane groups of the training workgroup:
+// part c < 3 = the 24 sin/cos features of position component c; part 3 = direction, normal,
+// albedo, roughness, the constant 1 and the zero padding.  Same values as encode_row.
+__device__ __forceinline__ void encode_part(const float *q, bf16 *row, int part)
+{
+    if (part < 3) {
+        float f = 3.14159265358979323846f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const float arg = q[part] * f;
+            float s, co;
+            sincosf(arg, &s, &co);
+            row[part * 24 + 2 * k]     = to_bf16(s);
+            row[part * 24 + 2 * k + 1] = to_bf16(co);
+            f *= 2.0f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) row[72 + k] = to_bf16(q[3 + k]);
+        row[82] = to_bf16(1.0f);
+#pragma unroll
+        for (int k = 83; k < kNrcIn; ++k) row[k] = to_bf16(0.0f);
+    }
+}
+
+// Copies the tile's 64 activation rows (columns [0, C)) from LDS to the global scratch.
+template <int C>
+__device__ __forceinline__ void save_acts(const bf16 *X, bf16 *A, int off)
+{
+    for (int c = threadIdx.x; c < 64 * (C / 8); c += 256) {
+        const int row = c / (C / 8), col = (c % (C / 8)) * 8;
+        *reinterpret_cast<bf16x8 *>(A + row * kNrcActs + off + col) = lds8(X + row * kXStride + col);
+    }
+}
+
+// Training: one workgroup (4 waves) = one tile of 64 samples.  The output columns of every
+// layer product are split over the waves (forward and dX: column tile = wave; dW: row tile
+// kt = wave, wave + 4, ...), with the tile's activations and output gradients shared in LDS,
+// so a step's serial chain of layer products is a quarter as long as with one wave per
+// tile.  Writes the tile's gradient partial (every parameter by exactly one wave, no
+// atomics) and its loss partial.
 __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, const float *__restrict__ target,
                                                    uint32_t n, float inv_count, const bf16 *__restrict__ W,
                                                    const bf16 *__restrict__ WT, bf16 *__restrict__ act,
                                                    float *__restrict__ partial, float *__restrict__ partial_loss)
 {
-    __shared__ bf16 xs[4 * 64 * kXStride];
-    __shared__ bf16 gs[4 * 64 * kGStride];    // G sample-major [s][n]
-    __shared__ bf16 gts[4 * 64 * kGStride];   // G feature-major [n][s]
-    __shared__ float red[4 * 16 * kNrcHid];     // per wave: 16 rows of one layer's dW
+    __shared__ bf16 X[64 * kXStride];
+    __shared__ bf16 Gs[64 * kGStride];    // G sample-major [s][n]
+    __shared__ bf16 Gt[64 * kGStride];    // G feature-major [n][s]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r16 = lane & 15, g = lane >> 4;
-    const uint32_t tile = blockIdx.x * 4 + wv;
+    const uint32_t tile = blockIdx.x;
     const uint32_t s0 = tile * 64;
-    bf16 *X  = xs + wv * 64 * kXStride;
-    bf16 *Gs = gs + wv * 64 * kGStride;
-    bf16 *Gt = gts + wv * 64 * kGStride;
-    bf16 *A  = act + (size_t)tile * 64 * kNrcActs;    // [64][kNrcActs] sample-major
-    const uint32_t s = s0 + lane;
+    bf16 *A = act + (size_t)tile * 64 * kNrcActs;    // [64][kNrcActs] sample-major
     {
+        const uint32_t s = s0 + lane;
         float qq[kNrcQuery];
 #pragma unroll
         for (int k = 0; k < kNrcQuery; ++k) qq[k] = s < n ? q[(size_t)s * kNrcQuery + k] : 0.0f;
-        encode_row(qq, X + lane * kXStride);
-        for (int k = 0; k < kNrcIn; k += 8) *reinterpret_cast<bf16x8 *>(A + lane * kNrcActs + k) = lds8(X + lane * kXStride + k);
+        encode_part(qq, X + lane * kXStride, wv);
     }
     __syncthreads();
-    // ---- forward, saving each hidden layer's output as the next layer's input ----
+    save_acts<kNrcIn>(X, A, 0);
+    // ---- forward: wave w computes output columns [16w, 16w + 16) of each hidden layer ----
 #define FWD_HIDDEN(L)                                                                             \
     {                                                                                             \
-        f32x4 acc[4][4];                                                                          \
-        layer_fwd<L>(X, WT, acc);                                                                 \
+        constexpr int K = layer_k(L);                                                             \
+        f32x4 acc[4];                                                                             \
+        for (int rt = 0; rt < 4; ++rt) acc[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};                   \
+        const bf16 *w = WT + nrc_w_off(L);                                                        \
+        for (int kk = 0; kk < K / 32; ++kk) {                                                     \
+            const bf16x8 b = glb8(w + (wv * 16 + r16) * K + kk * 32 + 8 * g);                     \
+            for (int rt = 0; rt < 4; ++rt)                                                        \
+                acc[rt] = mma(lds8(X + (rt * 16 + r16) * kXStride + kk * 32 + 8 * g), b, acc[rt]); \
+        }                                                                                         \
         __syncthreads();                                                                          \
-        store_relu<kNrcHid>(X, acc);                                                              \
+        for (int rt = 0; rt < 4; ++rt)                                                            \
+            for (int i = 0; i < 4; ++i)                                                           \
+                X[(rt * 16 + 4 * g + i) * kXStride + wv * 16 + r16] = to_bf16(fmaxf(acc[rt][i], 0.0f)); \
         __syncthreads();                                                                          \
-        for (int k = 0; k < kNrcHid; k += 8)                                                      \
-            *reinterpret_cast<bf16x8 *>(A + lane * kNrcActs + act_off(L + 1) + k) = lds8(X + lane * kXStride + k); \
+        save_acts<kNrcHid>(X, A, act_off(L + 1));                                                 \
     }
     FWD_HIDDEN(0)
     FWD_HIDDEN(1)
@@ -202,43 +246,47 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
     FWD_HIDDEN(3)
     FWD_HIDDEN(4)
 #undef FWD_HIDDEN
-    // ---- output layer, loss and dL/dy ----
-    float lsum = 0.0f;
-    {
-        f32x4 acc[4][4];
-        layer_fwd<5>(X, WT, acc);
+    // ---- output layer (waves 0, 1: column tiles 0, 1), loss and dL/dy ----
+    if (wv < kNrcOut / 16) {
+        constexpr int K = layer_k(5);
+        f32x4 acc[4];
+        for (int rt = 0; rt < 4; ++rt) acc[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const bf16 *w = WT + nrc_w_off(5);
+        for (int kk = 0; kk < K / 32; ++kk) {
+            const bf16x8 b = glb8(w + (wv * 16 + r16) * K + kk * 32 + 8 * g);
+            for (int rt = 0; rt < 4; ++rt) acc[rt] = mma(lds8(X + (rt * 16 + r16) * kXStride + kk * 32 + 8 * g), b, acc[rt]);
+        }
+        float lsum = 0.0f;
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < kNrcOut / 16; ++ct)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = rt * 16 + 4 * g + i, col = ct * 16 + r16;
-                    const uint32_t srow = s0 + row;
-                    float gr = 0.0f;
-                    if (col < 3 && srow < n) {
-                        const float y = acc[rt][ct][i];
-                        const float t = target[(size_t)srow * 3 + col];
-                        const float e = y - t;
-                        const float den = y * y + kNrcEpsRel;
-                        lsum += e * e / den * inv_count;
-                        gr = 2.0f * e / den * inv_count;
-                    }
-                    const bf16 gb = to_bf16(gr);
-                    Gs[row * kGStride + col] = gb;
-                    Gt[col * kGStride + row] = gb;
+            for (int i = 0; i < 4; ++i) {
+                const int row = rt * 16 + 4 * g + i, col = wv * 16 + r16;
+                const uint32_t srow = s0 + row;
+                float gr = 0.0f;
+                if (col < 3 && srow < n) {
+                    const float y = acc[rt][i];
+                    const float t = target[(size_t)srow * 3 + col];
+                    const float e = y - t;
+                    const float den = y * y + kNrcEpsRel;
+                    lsum += e * e / den * inv_count;
+                    gr = 2.0f * e / den * inv_count;
                 }
+                const bf16 gb = to_bf16(gr);
+                Gs[row * kGStride + col] = gb;
+                Gt[col * kGStride + row] = gb;
+            }
+        for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+        if (wv == 0 && lane == 0) partial_loss[tile] = lsum;
     }
-    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
-    if (lane == 0) partial_loss[tile] = lsum;
     __syncthreads();
-    float *P = partial + (size_t)blockIdx.x * kNrcParams;
+    float *P = partial + (size_t)tile * kNrcParams;
     // ---- backward ----
 #define BWD(L)                                                                                              \
     {                                                                                                       \
         constexpr int K = layer_k(L), N = layer_n(L);                                                       \
-        /* dW = X_L^T G : rows k (K/16 tiles), columns n (N/16 tiles), reduction over the 64 samples */     \
-        for (int kt = 0; kt < K / 16; ++kt) {                                                               \
+        /* dW = X_L^T G: row tiles kt = wave, wave + 4, ...; all N / 16 column tiles */                     \
+        for (int kt = wv; kt < K / 16; kt += 4) {                                                           \
             bf16x8 a[2];                                                                                    \
             for (int ss = 0; ss < 2; ++ss) {                                                                \
                 bf16 tmp[8];                                                                                \
@@ -248,36 +296,27 @@ __global__ __launch_bounds__(256) void k_nrc_train(const float *__restrict__ q, 
             for (int ct = 0; ct < N / 16; ++ct) {                                                           \
                 f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};                                                    \
                 for (int ss = 0; ss < 2; ++ss) d = mma(a[ss], lds8(Gt + (ct * 16 + r16) * kGStride + ss * 32 + 8 * g), d); \
-                for (int i = 0; i < 4; ++i) red[(wv * 16 + 4 * g + i) * N + ct * 16 + r16] = d[i];           \
+                for (int i = 0; i < 4; ++i) P[nrc_w_off(L) + (kt * 16 + 4 * g + i) * N + ct * 16 + r16] = d[i]; \
             }                                                                                               \
-            __syncthreads();                                                                                \
-            for (int e = threadIdx.x; e < 16 * N; e += 256)                                                 \
-                P[nrc_w_off(L) + kt * 16 * N + e] = ((red[e] + red[16 * N + e]) + red[32 * N + e]) + red[48 * N + e]; \
-            __syncthreads();                                                                                \
         }                                                                                                   \
         if (L > 0) {                                                                                        \
-            /* dX = (G W_L^T) * relu'(X_L) : rows s, columns k; reduction over n */                          \
-            f32x4 dx[4][4];                                                                                 \
-            for (int rt = 0; rt < 4; ++rt)                                                                  \
-                for (int kt = 0; kt < 4; ++kt) dx[rt][kt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};                 \
+            /* dX = (G W_L^T) * relu'(X_L): wave w computes columns [16w, 16w + 16); reduction over n */    \
+            f32x4 dx[4];                                                                                    \
+            for (int rt = 0; rt < 4; ++rt) dx[rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};                         \
             for (int nn = 0; nn < N / 32; ++nn) {                                                           \
-                bf16x8 ga[4];                                                                               \
-                for (int rt = 0; rt < 4; ++rt) ga[rt] = lds8(Gs + (rt * 16 + r16) * kGStride + nn * 32 + 8 * g); \
-                for (int kt = 0; kt < K / 16; ++kt) {                                                       \
-                    const bf16x8 b = glb8(W + nrc_w_off(L) + (kt * 16 + r16) * N + nn * 32 + 8 * g);        \
-                    for (int rt = 0; rt < 4; ++rt) dx[rt][kt] = mma(ga[rt], b, dx[rt][kt]);                 \
-                }                                                                                           \
+                const bf16x8 b = glb8(W + nrc_w_off(L) + (wv * 16 + r16) * N + nn * 32 + 8 * g);            \
+                for (int rt = 0; rt < 4; ++rt)                                                              \
+                    dx[rt] = mma(lds8(Gs + (rt * 16 + r16) * kGStride + nn * 32 + 8 * g), b, dx[rt]);       \
             }                                                                                               \
             __syncthreads();                                                                                \
             for (int rt = 0; rt < 4; ++rt)                                                                  \
-                for (int kt = 0; kt < 4; ++kt)                                                              \
-                    for (int i = 0; i < 4; ++i) {                                                           \
-                        const int row = rt * 16 + 4 * g + i, col = kt * 16 + r16;                           \
-                        const float xv = from_bf16(A[row * kNrcActs + act_off(L) + col]);                    \
-                        const bf16 gb = to_bf16(xv > 0.0f ? dx[rt][kt][i] : 0.0f);                          \
-                        Gs[row * kGStride + col] = gb;                                                      \
-                        Gt[col * kGStride + row] = gb;                                                      \
-                    }                                                                                       \
+                for (int i = 0; i < 4; ++i) {                                                               \
+                    const int row = rt * 16 + 4 * g + i, col = wv * 16 + r16;                               \
+                    const float xv = from_bf16(A[row * kNrcActs + act_off(L) + col]);                        \
+                    const bf16 gb = to_bf16(xv > 0.0f ? dx[rt][i] : 0.0f);                                  \
+                    Gs[row * kGStride + col] = gb;                                                          \
+                    Gt[col * kGStride + row] = gb;                                                          \
+                }                                                                                           \
             __syncthreads();                                                                                \
         }                                                                                                   \
     }
@@ -355,8 +394,7 @@ hipError_t nrc_launch_train(const float *q, const float *target, uint32_t n, con
 {
     if (n == 0) return hipSuccess;
     const uint32_t tiles = (n + 63) / 64;
-    const uint32_t blocks = (tiles + 3) / 4;
-    k_nrc_train<<<blocks, 256, 0, s>>>(q, target, n, 1.0f / (3.0f * (float)n), W, WT, act, partial, partial_loss);
+    k_nrc_train<<<tiles, 256, 0, s>>>(q, target, n, 1.0f / (3.0f * (float)n), W, WT, act, partial, partial_loss);
     return hipGetLastError();
 }
 

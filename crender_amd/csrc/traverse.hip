@@ -10,7 +10,7 @@
 //   * XCD-aware fetch: the ray range is cut into 8 contiguous slices, one per XCD
 //     (blocks are dispatched round-robin over XCDs), so neighbouring rays share an L2;
 //     an XCD that drains its slice steals from the others.
-//   * refill: when 16 or more lanes of a wave are idle, one lane reserves a run of rays
+//   * refill: when 32 or more lanes of a wave are idle, one lane reserves a run of rays
 //     for all idle lanes with a single atomic; lanes take their ray by ballot rank.
 //   * per-lane short stack in LDS (entry-major (ref, distance) pairs; 8, 16 or 32 entries), spilled
 //     a full stack at a time to a per-lane global region when full.  Entries carry the child's entry
@@ -30,8 +30,8 @@ namespace {
 constexpr float kInf = __builtin_huge_valf();
 constexpr uint32_t kEmpty = kInvalidRef;
 constexpr uint32_t kMissKey = 0xFFFFFFFFu;
-constexpr uint32_t kRefillIdle = 16;
-constexpr uint32_t kParkMin    = 8;            // leaf parking ends when fewer lanes search
+constexpr uint32_t kRefillIdle = 32;
+constexpr uint32_t kParkMin    = 4;            // leaf parking ends when fewer lanes search
 constexpr uint32_t kInstExit   = 0xFFFFFFFEu;   // stack marker: leave the current instance
 // Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
 // 1 + 2*gamma(3) keeps FP32 box tests conservative.
