@@ -1824,7 +1824,7 @@ extern "C" void crt_nrc_render_options_default(crt_nrc_render_options *o)
     o->footprint_c  = 0.01f;
     o->train_paths  = 65536;
     o->train_suffix = 4;
-    o->train_steps  = 4;
+    o->train_steps  = 24;
     o->train        = 1;
     o->warmup_passes = 0;
 }

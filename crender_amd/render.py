@@ -389,7 +389,7 @@ class Renderer:
     def render(self, passes: int) -> None:
         check(self._L.crt_renderer_render(self._h, passes))
 
-    def set_nrc(self, nrc, term_bounce=1, footprint_c=0.01, train_paths=65536, train_suffix=4, train_steps=4,
+    def set_nrc(self, nrc, term_bounce=1, footprint_c=0.01, train_paths=65536, train_suffix=4, train_steps=24,
                 train=True, warmup_passes=0) -> None:
         """Render with the neural radiance cache `nrc` (None turns it off)."""
         o = A.NrcRenderOptions()

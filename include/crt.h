@@ -292,7 +292,7 @@ typedef struct crt_nrc_render_options {
     float    footprint_c;   /* default 0.01; 0 = always query at term_bounce */
     uint32_t train_paths;   /* default 65536 */
     uint32_t train_suffix;  /* default 4 */
-    uint32_t train_steps;   /* default 4 */
+    uint32_t train_steps;   /* default 24 */
     uint32_t train;         /* 0 freezes the cache (inference only) */
     uint32_t warmup_passes; /* first passes after set_nrc only train the cache: their image is
                                discarded and they do not count as samples (default 0) */

@@ -87,9 +87,9 @@ def test_equal_time_mse_gain(setup):
     nrc = Nrc(ctx, max_batch=65536, seed=4)
     r = _renderer(ctx, sc, spec, seed=8)
     try:
-        r.set_nrc(nrc, train_paths=65536, train_steps=4, warmup_passes=64)
+        r.set_nrc(nrc, train_paths=65536, warmup_passes=8)     # library defaults otherwise
         t0 = time.perf_counter()
-        r.render(64)
+        r.render(8)
         t_warm = time.perf_counter() - t0
         n_nrc = 0
         while time.perf_counter() - t0 < budget:
@@ -102,4 +102,5 @@ def test_equal_time_mse_gain(setup):
         nrc.close()
     print(f"equal time {budget:.1f}s: plain {n_plain} spp MSE {mse_plain:.3e}; cache warm-up {t_warm:.2f}s + "
           f"{n_nrc} spp MSE {mse_nrc:.3e}; ratio {mse_plain / max(mse_nrc, 1e-30):.2f}")
-    assert mse_nrc < mse_plain
+    # the cache must at least halve the error of the plain tracer at equal time
+    assert mse_nrc < 0.5 * mse_plain
