@@ -898,18 +898,27 @@ __global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Fron
     }
     uint32_t ref[W];
     float    lo[W][3], hi[W][3];
+    bool     inner[W];
+    uint32_t ninner = 0;
     for (int j = 0; j < nc; ++j) {
         const Bvh2Node n = b2[ch[j]];
         for (int k = 0; k < 3; ++k) {
             lo[j][k] = n.lo[k];
             hi[j][k] = n.hi[k];
         }
-        if (b2_leaf(n)) {
-            ref[j] = ref_leaf(n.a, n.b & ~kLeafBit);
-        } else {
-            const uint32_t wi = atomicAdd(wide_count, 1u);
-            ref[j] = wi;
-            out[atomicAdd(nout, 1u)] = Frontier{ch[j], wi};
+        inner[j] = !b2_leaf(n);
+        ninner += inner[j] ? 1u : 0u;
+        if (!inner[j]) ref[j] = ref_leaf(n.a, n.b & ~kLeafBit);
+    }
+    // a node's inner children get consecutive indices (one allocation), so siblings share
+    // DRAM pages and translation entries
+    if (ninner) {
+        uint32_t wi = atomicAdd(wide_count, ninner);
+        uint32_t fo = atomicAdd(nout, ninner);
+        for (int j = 0; j < nc; ++j) {
+            if (!inner[j]) continue;
+            ref[j]    = wi;
+            out[fo++] = Frontier{ch[j], wi++};
         }
     }
     if (W == 4) {
