@@ -204,6 +204,7 @@ def main(argv=None) -> int:
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per traversal lane (0 = default)")
     ap.add_argument("--max-leaf", type=int, default=0, help="BVH leaf size limit (0 = library default)")
     ap.add_argument("--sort-rays", action="store_true", help="reorder secondary rays before tracing")
+    ap.add_argument("--sort-materials", action="store_true", help="shade each bounce's hits grouped by material")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
@@ -234,7 +235,8 @@ def main(argv=None) -> int:
     res, bounces = (1920, 1080), 4
     spec = scenes.soup(args.tris, seed=2, res=res)
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, phase_timing=True,
-                  trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack, sort_rays=args.sort_rays)
+                  trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack, sort_rays=args.sort_rays,
+                  sort_materials=args.sort_materials)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
@@ -280,6 +282,7 @@ def main(argv=None) -> int:
             "waves_per_cu": args.waves_per_cu or 8,
             "trace_stack": args.stack or 32,
             "sort_rays": args.sort_rays,
+            "sort_materials": args.sort_materials,
             "device": dev,
         },
         "roofline": {

@@ -32,6 +32,7 @@ CRT_MODE_COMPAT, CRT_MODE_PBR = 0, 1
 BUFFER = {"progress": 0, "normals": 1, "albedos": 2, "depths": 3, "raw": 4}
 CRT_FLAG_PHASE_TIMING = 1
 CRT_FLAG_SORT_RAYS = 2
+CRT_FLAG_SORT_MATERIALS = 4
 
 
 class CrtError(RuntimeError):

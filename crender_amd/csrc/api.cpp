@@ -406,6 +406,10 @@ int alloc_renderer_buffers(crt_renderer *r)
         RA(reinterpret_cast<uint8_t **>(&r->sort.temp), std::max<size_t>(r->sort.temp_bytes, 1));
         r->sort.capacity = (uint32_t)P;
     }
+    if (r->ctx->opt.flags & CRT_FLAG_SORT_MATERIALS) {
+        RA(&b.shade_perm, P);
+        RA(&b.shade_bcount, (size_t)kShadeClasses * ((P + 255) / 256));
+    }
 #undef RA
     if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
     hipStream_t s = r->ctx->stream;
@@ -495,7 +499,13 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
         TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
         HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
         if (t) HIP_CHECK(mark(3 + 4 * (int)i));
-        HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
+        if (b.shade_perm) {
+            // material-sorted shading: the queue's hits grouped by material class
+            HIP_CHECK(launch_shade_sort(sc, b, cur, s));
+            HIP_CHECK(launch_shade(sc, b, rp, i, cur, s, b.shade_perm));
+        } else {
+            HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
+        }
         if (t) HIP_CHECK(mark(4 + 4 * (int)i));
         if (sc.sun_enabled) {
             HIP_CHECK(launch_shadow_prep(b, s));

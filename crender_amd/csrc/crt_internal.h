@@ -250,6 +250,9 @@ struct PathBuffers {
     float    *tq, *tt;          // compacted training set: queries [n][13], targets [n][3]
     // batched rendering: job, pixel and absolute sample index of each path slot
     uint32_t *pjob, *ppix, *psamp;
+    // material-sorted shading (CRT_FLAG_SORT_MATERIALS): queue order grouped by material
+    // class, and per-block class counts [kShadeClasses][blocks(P)]
+    uint32_t *shade_perm, *shade_bcount;
 };
 constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
@@ -312,7 +315,12 @@ struct BatchDev {
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s);
 hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s);
 hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
-                        uint32_t cur, hipStream_t s);
+                        uint32_t cur, hipStream_t s, const uint32_t *perm = nullptr);
+// Material-sorted shading: classes miss / metal / smooth / glass of the hits of queue cur,
+// a stable counting sort of the queue indices by class into b.shade_perm (device-side
+// count, no host synchronisation); launch_shade then walks the queue in that order.
+constexpr uint32_t kShadeClasses = 4;
+hipError_t launch_shade_sort(const SceneDev &sc, const PathBuffers &b, uint32_t cur, hipStream_t s);
 hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s);
 hipError_t launch_shadow_resolve(const PathBuffers &b, hipStream_t s);
 hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hipStream_t s);

@@ -522,11 +522,13 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
 // each path's job, pixel and sample index come from pjob / ppix / psamp.
 template <bool kMulti>
 __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, RenderParams rp, const JobDev *jobs,
-                                               uint32_t bounce, uint32_t cur)
+                                               uint32_t bounce, uint32_t cur, const uint32_t *perm)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.cnt[cur];
-    const bool     live = k < n;
+    const bool     live = t < n;
+    // perm: queue entries in material-class order (the queue index itself otherwise)
+    const uint32_t k = live && perm ? perm[t] : t;
     ShadeOut o;
     o.push_next = o.push_shadow = o.do_query = false;
     o.no = o.nd = o.so = o.sd = o.scontrib = o.qthr = mk3(0, 0, 0);
@@ -860,9 +862,9 @@ hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)
     return hipGetLastError();
 }
 hipError_t launch_shade(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t bounce,
-                        uint32_t cur, hipStream_t s)
+                        uint32_t cur, hipStream_t s, const uint32_t *perm)
 {
-    k_shade<false><<<blocks(b.P), 256, 0, s>>>(sc, b, rp, nullptr, bounce, cur);
+    k_shade<false><<<blocks(b.P), 256, 0, s>>>(sc, b, rp, nullptr, bounce, cur, perm);
     return hipGetLastError();
 }
 hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s)
@@ -923,7 +925,7 @@ hipError_t launch_batch_generate(const BatchDev &bd, const PathBuffers &b, hipSt
 
 hipError_t launch_shade_multi(const BatchDev &bd, const PathBuffers &b, uint32_t bounce, uint32_t cur, hipStream_t s)
 {
-    k_shade<true><<<blocks(b.P), 256, 0, s>>>(SceneDev{}, b, RenderParams{}, bd.jobs, bounce, cur);
+    k_shade<true><<<blocks(b.P), 256, 0, s>>>(SceneDev{}, b, RenderParams{}, bd.jobs, bounce, cur, nullptr);
     return hipGetLastError();
 }
 

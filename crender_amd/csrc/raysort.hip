@@ -61,6 +61,86 @@ __global__ __launch_bounds__(256) void k_ray_gather(RaySoA src, const uint32_t *
     dst_path[i] = src_path[j];
 }
 
+// ---- material-sorted shading ----------------------------------------------------------
+// Class of queue entry k: 0 = miss (sky), 1 + material type (metal 0, smooth 1, glass 2)
+// otherwise, i.e. the branch of the shading code the entry takes.
+__device__ __forceinline__ uint32_t shade_class(const SceneDev &sc, const HitSoA &h, uint32_t k)
+{
+    const uint32_t tri = h.tri[k];
+    if (tri == kInvalidRef) return 0u;
+    const uint32_t prim = __float_as_uint(sc.tri[3 * (size_t)tri].w);
+    const int32_t  t    = sc.mats[sc.mat_idx[prim]].type;
+    return 1u + (uint32_t)min(max(t, 0), 2);
+}
+
+// bcount[c * gridDim.x + block] = entries of class c among the block's 256 queue entries
+__global__ __launch_bounds__(256) void k_class_count(SceneDev sc, HitSoA h, const uint32_t *count, uint32_t *bcount)
+{
+    __shared__ uint32_t wc[4][kShadeClasses];
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    const bool     in = k < *count;
+    const uint32_t c  = in ? shade_class(sc, h, k) : kShadeClasses;
+    const int      w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (uint32_t j = 0; j < kShadeClasses; ++j) {
+        const uint64_t m = __ballot(c == j);
+        if (lane == 0) wc[w][j] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kShadeClasses) {
+        const uint32_t j = threadIdx.x;
+        bcount[j * gridDim.x + blockIdx.x] = wc[0][j] + wc[1][j] + wc[2][j] + wc[3][j];
+    }
+}
+
+// In-place exclusive prefix sum of bcount[0 .. n) by one workgroup of 1024 lanes.
+__global__ __launch_bounds__(1024) void k_class_scan(uint32_t *bcount, uint32_t n)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += bcount[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {   // Hillis-Steele inclusive scan
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t v = bcount[i];
+        bcount[i] = run;
+        run += v;
+    }
+}
+
+// perm[offset of (class, block) + rank within the block] = queue index: stable by index.
+__global__ __launch_bounds__(256) void k_class_scatter(SceneDev sc, HitSoA h, const uint32_t *count,
+                                                       const uint32_t *boff, uint32_t *perm)
+{
+    __shared__ uint32_t wc[4][kShadeClasses];
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    const bool     in = k < *count;
+    const uint32_t c  = in ? shade_class(sc, h, k) : kShadeClasses;
+    const int      w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t       rank = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kShadeClasses; ++j) {
+        const uint64_t m = __ballot(c == j);
+        if (lane == 0) wc[w][j] = (uint32_t)__popcll(m);
+        if (c == j) rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if (in) {
+        uint32_t off = boff[c * gridDim.x + blockIdx.x] + rank;
+        for (int v = 0; v < w; ++v) off += wc[v][c];
+        perm[off] = k;
+    }
+}
+
 }  // namespace
 
 size_t ray_sort_temp_bytes(uint32_t capacity)
@@ -86,6 +166,15 @@ hipError_t launch_ray_sort(const RaySoA &src, const uint32_t *src_path, const ui
     }
     k_ray_gather<<<n ? (n + 255) / 256 : 1, 256, 0, s>>>(src, src_path, src_count, w.vals_out, dst, dst_path,
                                                          dst_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_sort(const SceneDev &sc, const PathBuffers &b, uint32_t cur, hipStream_t s)
+{
+    const uint32_t nb = (b.P + 255) / 256;
+    k_class_count<<<nb, 256, 0, s>>>(sc, b.hits, b.cnt + cur, b.shade_bcount);
+    k_class_scan<<<1, 1024, 0, s>>>(b.shade_bcount, nb * kShadeClasses);
+    k_class_scatter<<<nb, 256, 0, s>>>(sc, b.hits, b.cnt + cur, b.shade_bcount, b.shade_perm);
     return hipGetLastError();
 }
 

@@ -126,6 +126,7 @@ typedef struct crt_options {
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
 #define CRT_FLAG_SORT_RAYS    2u /* reorder secondary ray queues by origin / direction before tracing */
+#define CRT_FLAG_SORT_MATERIALS 4u /* shade each bounce's hits grouped by material class */
 
 typedef struct crt_render_options {
     uint32_t mode;          /* crt_mode */
