@@ -1406,6 +1406,7 @@ int crt_scene_commit(crt_scene *scene)
     d = SceneDev{};
     d.tri         = scene->d_tri;
     d.nodes       = scene->d_nodes;
+    d.nnodes      = br.node_count;
     d.width       = width;
     d.ntri        = ntri;
     d.root        = 0;

@@ -103,6 +103,7 @@ struct SunDev {
 struct SceneDev {
     const float4      *tri;
     const void        *nodes;
+    uint32_t           nnodes;     // wide nodes in `nodes`
     uint32_t           width;      // 4 or 8
     uint32_t           ntri;
     uint32_t           root;       // root child ref
