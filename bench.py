@@ -160,6 +160,27 @@ def nrc_equal_time(ctx, sc, ref_spp: int) -> dict:
     }
 
 
+def traversal_counts(spec, res, bounces, args) -> dict:
+    """Node visits, leaf visits and triangle tests per traced ray, from one pass of the same
+    workload with the instrumented traversal kernel (outside the timed region)."""
+    from crender_amd import Context, Renderer, scene_from_spec
+
+    ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
+                  trace_stack=args.stack, traversal_stats=True)
+    sc = scene_from_spec(ctx, spec)
+    r = Renderer(ctx, sc, res[0], res[1], bounces)
+    r.set_resolution(*res)
+    r.set_options(mode=0, rr_start=0, seed=0)
+    r.render(1)
+    st = r.current_stats()
+    r.close()
+    sc.close()
+    ctx.close()
+    rays = max(1, st["extension_rays"] + st["shadow_rays"])
+    return {"nodes_per_ray": st["node_visits"] / rays, "leaves_per_ray": st["leaf_visits"] / rays,
+            "tris_per_ray": st["tri_tests"] / rays, "rays": rays}
+
+
 def batch_bench(seed: int = 7) -> dict:
     """64 independent scenes (soup / terrain, 1k-500k tris, 128^2-1024^2, 1-64 spp, depth 6)
     rendered by crt_render_batch after crt_autotune."""
@@ -309,6 +330,7 @@ def main(argv=None) -> int:
         out["nrc"] = nrc_equal_time(ctx, sc, args.nrc_ref_spp)
     sc.close()
     ctx.close()
+    out["traversal"] = traversal_counts(spec, res, bounces, args)
     if not args.no_batch:
         out["batch"] = batch_bench()
     if not args.no_cpu_baseline:

@@ -33,6 +33,7 @@ BUFFER = {"progress": 0, "normals": 1, "albedos": 2, "depths": 3, "raw": 4}
 CRT_FLAG_PHASE_TIMING = 1
 CRT_FLAG_SORT_RAYS = 2
 CRT_FLAG_SORT_MATERIALS = 4
+CRT_FLAG_TRAVERSAL_STATS = 8
 
 
 class CrtError(RuntimeError):
@@ -89,7 +90,8 @@ class Stats(C.Structure):
                 ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("ms_generate", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("ms_render", C.c_double),
-                ("extend_launches", C.c_uint64), ("ms_nrc_train", C.c_double), ("nrc_loss", C.c_float)]
+                ("extend_launches", C.c_uint64), ("ms_nrc_train", C.c_double), ("nrc_loss", C.c_float),
+                ("node_visits", C.c_uint64), ("leaf_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
 
 
 class BvhInfo(C.Structure):

@@ -410,12 +410,14 @@ int alloc_renderer_buffers(crt_renderer *r)
         RA(&b.shade_perm, P);
         RA(&b.shade_bcount, (size_t)kShadeClasses * ((P + 255) / 256));
     }
+    if (r->ctx->opt.flags & CRT_FLAG_TRAVERSAL_STATS) RA(&b.tstats, (size_t)4);
 #undef RA
     if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * NP, s));
     HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+    if (b.tstats) HIP_CHECK(hipMemsetAsync(b.tstats, 0, sizeof(unsigned long long) * 4, s));
     // image.h:30-33: new images are filled with FLT_MAX
     HIP_CHECK(launch_fill(b.aov_alb, NP, 3.402823466e+38f, s));
     HIP_CHECK(launch_fill(b.aov_nrm, NP, 3.402823466e+38f, s));
@@ -497,6 +499,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
             if (t) HIP_CHECK(mark(2 + 4 * (int)i));
         }
         TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
+        q.tstats     = b.tstats;
         HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
         if (t) HIP_CHECK(mark(3 + 4 * (int)i));
         if (b.shade_perm) {
@@ -511,6 +514,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
             HIP_CHECK(launch_shadow_prep(b, s));
             if (!sc.any_alpha) {
                 TraceQueue qs = make_queue(b.sq, HitSoA{}, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
+                qs.tstats     = b.tstats;
                 HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, trace_cfg(ctx), s));
                 HIP_CHECK(launch_shadow_resolve(b, s));
             } else {
@@ -693,6 +697,7 @@ int reset_accum(crt_renderer *r)
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.npix, s));
     HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+    if (r->b.tstats) HIP_CHECK(hipMemsetAsync(r->b.tstats, 0, sizeof(unsigned long long) * 4, s));
     HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipStreamSynchronize(s));
     r->samples.store(0);
@@ -1753,6 +1758,14 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->ms_render     = r->ms_phase[5];
     stats->extend_launches = r->extend_launches;
     stats->ms_nrc_train    = r->ms_nrc_train;
+    if (r->b.tstats) {
+        unsigned long long ts[4];
+        HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
+        HIP_CHECK(hipMemcpy(ts, r->b.tstats, sizeof ts, hipMemcpyDeviceToHost));
+        stats->node_visits = ts[0];
+        stats->leaf_visits = ts[1];
+        stats->tri_tests   = ts[2];
+    }
     if (r->nrc && r->loss_tiles) {
         std::lock_guard<std::mutex> g(r->nrc->mu);
         std::vector<float> pl(r->loss_tiles);

@@ -169,6 +169,7 @@ struct TraceQueue {
     const uint32_t *count;     // device: number of rays
     uint32_t       *counters;  // device: work counters (kXcds entries), zeroed by the caller's kernel
     uint32_t       *error;     // device: set to 1 if a traversal stack would overflow
+    unsigned long long *tstats;  // device [3] or null: node visits, leaf visits, triangle tests
     // batched rendering: the scene of ray i is scenes[path_job[ray_path[i]]]
     const uint32_t   *ray_path;
     const uint32_t   *path_job;
@@ -254,6 +255,8 @@ struct PathBuffers {
     // material-sorted shading (CRT_FLAG_SORT_MATERIALS): queue order grouped by material
     // class, and per-block class counts [kShadeClasses][blocks(P)]
     uint32_t *shade_perm, *shade_bcount;
+    // CRT_FLAG_TRAVERSAL_STATS: node visits, leaf visits, triangle tests since the last reset
+    unsigned long long *tstats;
 };
 constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt

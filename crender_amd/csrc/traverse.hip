@@ -276,10 +276,13 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // restores the world ray; closest-hit distances are compared in world units.
 // The 8-entry stack variant is also the high-occupancy one: its 18 KB of LDS per block
 // leaves room for 8 waves per SIMD, so the register budget is capped to match (64 VGPRs).
-template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst>
+// kStats: the instrumented variant; counts each lane's inner-node tests, leaf visits and
+// triangle tests and adds them to q.tstats when the lane's wave leaves the kernel.
+template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst, bool kStats = false>
 __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ,
                                                                       uint32_t *spill)
 {
+    uint32_t c_nodes = 0, c_leaves = 0, c_tris = 0;
     __shared__ uint2 s_stack[(S + 1) * kTraceBlock];
     const uint32_t n = *q.count;
     if (n == 0) return;
@@ -431,6 +434,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
+                if constexpr (kStats) ++c_nodes;
                 if (nh == 0) {
                     cur = next();
                 } else {
@@ -454,7 +458,9 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 const uint32_t lf = parked ? pend : cur;
                 const uint32_t first = leaf_first(lf), cnt = leaf_count(lf);
                 bool any = false;
+                if constexpr (kStats) ++c_leaves;
                 for (uint32_t k = 0; k < cnt; ++k) {
+                    if constexpr (kStats) ++c_tris;
                     const size_t  ti = (size_t)(first + k);
                     const float4  a  = tris[3 * ti + 0];
                     const float4  b  = tris[3 * ti + 1];
@@ -502,6 +508,19 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             active = false;
         }
     }
+    if constexpr (kStats) {
+        unsigned long long a = c_nodes, b = c_leaves, c = c_tris;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+            c += __shfl_xor(c, o);
+        }
+        if (lane == 0 && q.tstats) {
+            atomicAdd(q.tstats + 0, a);
+            atomicAdd(q.tstats + 1, b);
+            atomicAdd(q.tstats + 2, c);
+        }
+    }
 }
 
 }  // namespace
@@ -531,6 +550,12 @@ hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev 
                           uint32_t *spill, hipStream_t s)
 {
     const dim3 g(tc.grid), b(kTraceBlock);
+    if (!kMulti && q.tstats) {
+        // instrumented traversal: one stack depth (the counts do not depend on it)
+        if (width == 4) k_trace<4, kAny, false, 32, kInst, true><<<g, b, 0, s>>>(sc, q, occ, spill);
+        else k_trace<8, kAny, false, 32, kInst, true><<<g, b, 0, s>>>(sc, q, occ, spill);
+        return hipGetLastError();
+    }
 #define CRT_TRACE(Wd, Sd) k_trace<Wd, kAny, kMulti, Sd, kInst><<<g, b, 0, s>>>(sc, q, occ, spill)
     if (width == 4) {
         if (tc.stack == 8) CRT_TRACE(4, 8);

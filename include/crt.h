@@ -127,6 +127,8 @@ typedef struct crt_options {
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
 #define CRT_FLAG_SORT_RAYS    2u /* reorder secondary ray queues by origin / direction before tracing */
 #define CRT_FLAG_SORT_MATERIALS 4u /* shade each bounce's hits grouped by material class */
+#define CRT_FLAG_TRAVERSAL_STATS 8u /* renderers count node visits, leaf visits and triangle
+                                       tests (an instrumented traversal kernel: slower) */
 
 typedef struct crt_render_options {
     uint32_t mode;          /* crt_mode */
@@ -149,6 +151,9 @@ typedef struct crt_stats {
     uint64_t extend_launches;    /* closest-hit launches covered by ms_extend */
     double   ms_nrc_train;       /* host time of the cache's training passes */
     float    nrc_loss;           /* training loss of the last step (one tile) */
+    uint64_t node_visits;        /* with CRT_FLAG_TRAVERSAL_STATS, over all traced rays: */
+    uint64_t leaf_visits;        /*   inner nodes tested, leaves entered, */
+    uint64_t tri_tests;          /*   ray-triangle tests */
 } crt_stats;
 
 /* BVH summary returned by crt_scene_bvh_info */

@@ -1,5 +1,6 @@
 """The C-ABI library loads and exports every function include/crt.h declares (CPU only)."""
 import ctypes
+from pathlib import Path
 
 import pytest
 
@@ -60,3 +61,40 @@ def test_errors_without_gpu_are_reported_not_fatal():
     rc = L.crt_context_create(None, None)
     assert rc == -1
     assert b"null" in L.crt_last_error()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a crt.h struct has the C struct's size and field offsets
+    (compiled with the host C compiler against the header)."""
+    import ctypes as C
+    import shutil
+    import subprocess
+
+    from crender_amd import _abi as A
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    pairs = {"crt_material": A.Material, "crt_image": A.Image, "crt_model_data": A.ModelData, "crt_sun": A.Sun,
+             "crt_camera": A.Camera, "crt_options": A.Options, "crt_render_options": A.RenderOptions,
+             "crt_stats": A.Stats, "crt_bvh_info": A.BvhInfo, "crt_rays": A.Rays, "crt_hits": A.Hits,
+             "crt_nrc_options": A.NrcOptions, "crt_nrc_render_options": A.NrcRenderOptions,
+             "crt_post_options": A.PostOptions, "crt_denoise_options": A.DenoiseOptions, "crt_job": A.Job,
+             "crt_batch_options": A.BatchOptions, "crt_batch_stats": A.BatchStats, "crt_tune_result": A.TuneResult}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "crt.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} @sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run([cc, "-std=c11", f"-I{inc}", str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in pairs.items():
+        assert got[(cname, "@sizeof")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
