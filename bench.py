@@ -178,7 +178,10 @@ def traversal_counts(spec, res, bounces, args) -> dict:
     ctx.close()
     rays = max(1, st["extension_rays"] + st["shadow_rays"])
     return {"nodes_per_ray": st["node_visits"] / rays, "leaves_per_ray": st["leaf_visits"] / rays,
-            "tris_per_ray": st["tri_tests"] / rays, "rays": rays}
+            "tris_per_ray": st["tri_tests"] / rays, "rays": rays,
+            # lanes busy per wave iteration of the node / leaf loop (SIMD utilisation)
+            "node_loop_lanes": st["node_visits"] / max(1, 64 * st["wave_node_steps"]),
+            "leaf_loop_lanes": st["leaf_visits"] / max(1, 64 * st["wave_leaf_steps"])}
 
 
 def batch_bench(seed: int = 7) -> dict:

@@ -410,14 +410,14 @@ int alloc_renderer_buffers(crt_renderer *r)
         RA(&b.shade_perm, P);
         RA(&b.shade_bcount, (size_t)kShadeClasses * ((P + 255) / 256));
     }
-    if (r->ctx->opt.flags & CRT_FLAG_TRAVERSAL_STATS) RA(&b.tstats, (size_t)4);
+    if (r->ctx->opt.flags & CRT_FLAG_TRAVERSAL_STATS) RA(&b.tstats, (size_t)8);
 #undef RA
     if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * NP, s));
     HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
-    if (b.tstats) HIP_CHECK(hipMemsetAsync(b.tstats, 0, sizeof(unsigned long long) * 4, s));
+    if (b.tstats) HIP_CHECK(hipMemsetAsync(b.tstats, 0, sizeof(unsigned long long) * 8, s));
     // image.h:30-33: new images are filled with FLT_MAX
     HIP_CHECK(launch_fill(b.aov_alb, NP, 3.402823466e+38f, s));
     HIP_CHECK(launch_fill(b.aov_nrm, NP, 3.402823466e+38f, s));
@@ -697,7 +697,7 @@ int reset_accum(crt_renderer *r)
     hipStream_t s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.npix, s));
     HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
-    if (r->b.tstats) HIP_CHECK(hipMemsetAsync(r->b.tstats, 0, sizeof(unsigned long long) * 4, s));
+    if (r->b.tstats) HIP_CHECK(hipMemsetAsync(r->b.tstats, 0, sizeof(unsigned long long) * 8, s));
     HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipStreamSynchronize(s));
     r->samples.store(0);
@@ -1759,12 +1759,14 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->extend_launches = r->extend_launches;
     stats->ms_nrc_train    = r->ms_nrc_train;
     if (r->b.tstats) {
-        unsigned long long ts[4];
+        unsigned long long ts[8];
         HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
         HIP_CHECK(hipMemcpy(ts, r->b.tstats, sizeof ts, hipMemcpyDeviceToHost));
-        stats->node_visits = ts[0];
-        stats->leaf_visits = ts[1];
-        stats->tri_tests   = ts[2];
+        stats->node_visits     = ts[0];
+        stats->leaf_visits     = ts[1];
+        stats->tri_tests       = ts[2];
+        stats->wave_node_steps = ts[3];
+        stats->wave_leaf_steps = ts[4];
     }
     if (r->nrc && r->loss_tiles) {
         std::lock_guard<std::mutex> g(r->nrc->mu);

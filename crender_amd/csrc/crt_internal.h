@@ -169,7 +169,8 @@ struct TraceQueue {
     const uint32_t *count;     // device: number of rays
     uint32_t       *counters;  // device: work counters (kXcds entries), zeroed by the caller's kernel
     uint32_t       *error;     // device: set to 1 if a traversal stack would overflow
-    unsigned long long *tstats;  // device [3] or null: node visits, leaf visits, triangle tests
+    unsigned long long *tstats;  // device [5] or null: node visits, leaf visits, triangle tests,
+                                 // wave iterations of the node loop and of the leaf loop
     // batched rendering: the scene of ray i is scenes[path_job[ray_path[i]]]
     const uint32_t   *ray_path;
     const uint32_t   *path_job;
@@ -255,7 +256,7 @@ struct PathBuffers {
     // material-sorted shading (CRT_FLAG_SORT_MATERIALS): queue order grouped by material
     // class, and per-block class counts [kShadeClasses][blocks(P)]
     uint32_t *shade_perm, *shade_bcount;
-    // CRT_FLAG_TRAVERSAL_STATS: node visits, leaf visits, triangle tests since the last reset
+    // CRT_FLAG_TRAVERSAL_STATS: node visits, leaf visits, triangle tests, wave loop iterations
     unsigned long long *tstats;
 };
 constexpr uint32_t kStatSlots = 64;

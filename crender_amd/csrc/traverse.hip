@@ -277,12 +277,13 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // The 8-entry stack variant is also the high-occupancy one: its 18 KB of LDS per block
 // leaves room for 8 waves per SIMD, so the register budget is capped to match (64 VGPRs).
 // kStats: the instrumented variant; counts each lane's inner-node tests, leaf visits and
-// triangle tests and adds them to q.tstats when the lane's wave leaves the kernel.
+// triangle tests, and each wave's iterations of the node and leaf loops, and adds them to
+// q.tstats when the wave leaves the kernel.
 template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst, bool kStats = false>
 __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ,
                                                                       uint32_t *spill)
 {
-    uint32_t c_nodes = 0, c_leaves = 0, c_tris = 0;
+    uint32_t c_nodes = 0, c_leaves = 0, c_tris = 0, c_wnode = 0, c_wleaf = 0;
     __shared__ uint2 s_stack[(S + 1) * kTraceBlock];
     const uint32_t n = *q.count;
     if (n == 0) return;
@@ -431,6 +432,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             // fewer than kParkMin lanes still searching and some holding a leaf: run the leaf
             // phase now rather than keep most of the wave waiting
             if (kSpec && __popcll(srch) < kParkMin && __ballot(active && pend != kEmpty) != 0) break;
+            if constexpr (kStats) ++c_wnode;
             if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
@@ -449,6 +451,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             const bool parked = kSpec && active && pend != kEmpty;
             const bool leaf   = active && (parked || (cur != kEmpty && ref_is_leaf(cur)));
             if (__ballot(leaf) == 0) break;
+            if constexpr (kStats) ++c_wleaf;
             if (kInst && leaf && level == 0) {
                 // instance leaf: enter the first instance, keep the rest for later
                 const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
@@ -519,6 +522,8 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             atomicAdd(q.tstats + 0, a);
             atomicAdd(q.tstats + 1, b);
             atomicAdd(q.tstats + 2, c);
+            atomicAdd(q.tstats + 3, (unsigned long long)c_wnode);   // wave-uniform counts
+            atomicAdd(q.tstats + 4, (unsigned long long)c_wleaf);
         }
     }
 }

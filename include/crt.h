@@ -153,7 +153,9 @@ typedef struct crt_stats {
     float    nrc_loss;           /* training loss of the last step (one tile) */
     uint64_t node_visits;        /* with CRT_FLAG_TRAVERSAL_STATS, over all traced rays: */
     uint64_t leaf_visits;        /*   inner nodes tested, leaves entered, */
-    uint64_t tri_tests;          /*   ray-triangle tests */
+    uint64_t tri_tests;          /*   ray-triangle tests, */
+    uint64_t wave_node_steps;    /*   wave iterations of the inner-node loop, */
+    uint64_t wave_leaf_steps;    /*   wave iterations of the leaf loop */
 } crt_stats;
 
 /* BVH summary returned by crt_scene_bvh_info */
