@@ -17,9 +17,9 @@
 //     distance so popped nodes beyond the current hit are skipped.
 //   * children are ordered by entry distance with a sorting network; the nearest is
 //     visited next and the others pushed far-to-near, with unconditional LDS writes.
-//   * leaf parking: a lane that reaches a leaf keeps it aside and continues down the tree
-//     until (nearly) every lane of the wave holds a leaf, so the triangle phase runs with
-//     more lanes busy (Aila and Laine 2009, "speculative traversal").
+//   * leaf parking: a lane that reaches a leaf keeps it aside (up to two) and continues
+//     down the tree until (nearly) every lane of the wave holds its leaves, so the triangle
+//     phase runs with more lanes busy (Aila and Laine 2009, "speculative traversal").
 #include <hip/hip_runtime.h>
 #include "crt_internal.h"
 
@@ -348,10 +348,10 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             exit_instance();
         }
     };
-    // leaf parked by the inner-node phase (not for two-level scenes, whose instance leaves
-    // change the lane's ray)
+    // leaves parked by the inner-node phase, pend first (not for two-level scenes, whose
+    // instance leaves change the lane's ray)
     constexpr bool kSpec = !kInst;
-    uint32_t pend = kEmpty;
+    uint32_t pend = kEmpty, pend2 = kEmpty;
     bool     active = false, exhausted = false;
     uint32_t next_slice = 0;   // slices tried by this wave (wave-uniform)
 
@@ -396,6 +396,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 r.bu = r.bv = 0.0f;
                 st.sp = st.gsp = 0;
                 pend      = kEmpty;
+                pend2     = kEmpty;
                 level     = 0;
                 best_inst = kInvalidRef;
                 uint32_t root = sc.root, ntri = sc.ntri;
@@ -419,15 +420,16 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             continue;
         }
         // ---- inner-node phase: lanes expand nodes until every lane holds a leaf or is done.
-        // kSpec: a lane whose next node is a leaf parks it and keeps traversing until every
-        // lane has a leaf, so that the leaf phase runs with more lanes busy ----
+        // kSpec: a lane whose next node is a leaf parks it (two slots) and keeps traversing
+        // until its slots are full, so that the leaf phase runs with more lanes busy ----
         for (;;) {
-            if (kSpec && active && pend == kEmpty && cur != kEmpty && ref_is_leaf(cur)) {
-                pend = cur;
-                cur  = next();
+            if (kSpec && active && pend2 == kEmpty && cur != kEmpty && ref_is_leaf(cur)) {
+                if (pend == kEmpty) pend = cur;
+                else pend2 = cur;
+                cur = next();
             }
             const bool inner = active && cur != kEmpty && !ref_is_leaf(cur);
-            const uint64_t srch = __ballot(inner && (!kSpec || pend == kEmpty));
+            const uint64_t srch = __ballot(inner && (!kSpec || pend2 == kEmpty));
             if (srch == 0) break;
             // fewer than kParkMin lanes still searching and some holding a leaf: run the leaf
             // phase now rather than keep most of the wave waiting
@@ -490,8 +492,12 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                     }
                 }
                 if (parked) {
-                    pend = kEmpty;
-                    if (kAny && any) cur = kEmpty;
+                    pend  = pend2;
+                    pend2 = kEmpty;
+                    if (kAny && any) {
+                        cur  = kEmpty;
+                        pend = kEmpty;
+                    }
                 } else {
                     cur = (kAny && any) ? kEmpty : next();
                 }
