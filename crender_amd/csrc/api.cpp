@@ -564,6 +564,8 @@ int nrc_train_step(crt_renderer *r);
 int render_pass(crt_renderer *r)
 {
     crt_context       *ctx = r->ctx;
+    // the context's stream and spill area are shared with queries, batches and the autotuner
+    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     const PathBuffers &b   = r->b;
     hipStream_t        s   = ctx->stream;
     RenderParams       rp;
@@ -2267,6 +2269,9 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
         }
         work += (uint64_t)J.width * J.height * J.spp;
     }
+    // one stream and one spill area per context: batches, queries, autotuning and renderer
+    // passes on a context run one at a time
+    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     uint64_t P64 = o.max_paths ? o.max_paths : (ctx->batch_paths ? ctx->batch_paths : (4u << 20));
     P64 = std::min<uint64_t>(P64, work);
     P64 = std::min<uint64_t>(P64, 1u << 27);
@@ -2439,6 +2444,7 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
         return CRT_OK;
     };
     auto set_grid = [&](uint32_t blocks) {
+        std::lock_guard<std::mutex> lock(ctx->mu);   // no pass or query may use the old spill area
         dfree(ctx->spill);   // sized by the grid; renderers re-allocate it on demand
         ctx->spill = nullptr;
         ctx->grid  = trace_grid(ctx->device, blocks);
@@ -2486,26 +2492,30 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
             set_grid(keep);
         }
     }
+    auto set_field = [&](uint32_t &field, uint32_t v) {
+        std::lock_guard<std::mutex> lock(ctx->mu);   // read by passes running on the context
+        field = v;
+    };
     for (uint32_t cand : {8u, 16u, 32u}) {
         if (cand == ctx->stack) continue;
         const uint32_t keep = ctx->stack;
-        ctx->stack = cand;
+        set_field(ctx->stack, cand);
         double v = 0.0;
         if ((st = measure(&v))) return st;
         if (v > best) best = v;
-        else ctx->stack = keep;
+        else set_field(ctx->stack, keep);
     }
     for (uint32_t cand : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
         if (cand == ctx->batch_paths) continue;
         if (cand > 2 * work && cand > (1u << 20)) continue;   // larger than the sample: nothing to learn
         const uint32_t keep = ctx->batch_paths;
-        ctx->batch_paths = cand;
+        set_field(ctx->batch_paths, cand);
         double v = 0.0;
         if ((st = measure(&v))) return st;
         if (v > best) best = v;
-        else ctx->batch_paths = keep;
+        else set_field(ctx->batch_paths, keep);
     }
-    ctx->opt.trace_stack = ctx->stack;
+    set_field(ctx->opt.trace_stack, ctx->stack);
     out->bvh_width           = width;
     out->trace_blocks_per_cu = blocks;
     out->trace_stack         = ctx->stack;

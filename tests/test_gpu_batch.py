@@ -110,3 +110,35 @@ def test_autotune_then_batch_still_exact():
             s.close()
     finally:
         ctx.close()
+
+
+def test_autotune_beside_a_background_renderer():
+    """The autotuner re-sizes the context's traversal grid (and its spill area) while a
+    renderer of the same context renders in the background: the passes and the tuner's
+    batches take turns on the context, and the background image is the synchronous one."""
+    from crender_amd import autotune
+    ctx = Context(0)
+    try:
+        soup = scene_from_spec(ctx, scenes.soup(20_000, seed=9))
+        box = scene_from_spec(ctx, scenes.cornell(1))
+        bg = Renderer(ctx, box, 96, 96, 6)
+        bg.set_resolution(96, 96)
+        bg.set_options(mode=0, rr_start=3, seed=12)
+        bg.set_target_spp(40)
+        bg.start()
+        jobs = [dict(scene=soup, width=64, height=64, spp=2, max_bounces=4, mode=0, seed=3)]
+        res = autotune(ctx, jobs)
+        assert res["trials"] >= 4
+        bg.wait()
+        assert bg.current_sample_count() == 40
+        ref = Renderer(ctx, box, 96, 96, 6)
+        ref.set_resolution(96, 96)
+        ref.set_options(mode=0, rr_start=3, seed=12)
+        ref.render(40)
+        assert np.array_equal(bg.raw(), ref.raw())
+        bg.close()
+        ref.close()
+        soup.close()
+        box.close()
+    finally:
+        ctx.close()
