@@ -101,6 +101,7 @@ struct crt_context {
     uint8_t    *st_occ = nullptr;
     uint32_t   *st_cnt = nullptr;   // [0] count, [8..15] work counters, [16] error
     std::mutex  mu;                 // serialises queries on the context stream
+    std::atomic<int> live{0};       // scenes, renderers and caches created on the context
 };
 
 struct crt_scene {
@@ -144,6 +145,7 @@ struct crt_nrc {
     uint32_t        stage_cap = 0;
     uint32_t        tiles_alloc = 0;
     uint64_t        step = 0;
+    int             users = 0;   // renderers the cache is attached to
     std::mutex      mu;
 };
 
@@ -860,6 +862,7 @@ int crt_context_create(const crt_options *opt, crt_context **out)
 int crt_context_destroy(crt_context *ctx)
 {
     if (!ctx) return CRT_OK;
+    if (ctx->live.load() > 0) return fail(CRT_ERR_STATE, "context still has scenes, renderers or caches");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     dfree(ctx->spill);
@@ -971,6 +974,7 @@ int crt_scene_create(crt_context *ctx, crt_scene **out)
     s->ctx  = ctx;
     crt_sun_default(&s->sun);
     crt_camera_default(&s->cam);
+    ctx->live++;
     *out = s;
     return CRT_OK;
 }
@@ -981,6 +985,7 @@ int crt_scene_destroy(crt_scene *scene)
     if (scene->users) return fail(CRT_ERR_STATE, "scene still used by a renderer");
     (void)hipSetDevice(scene->ctx->device);
     free_scene_device(scene);
+    scene->ctx->live--;
     delete scene;
     return CRT_OK;
 }
@@ -1528,6 +1533,7 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
         return st;
     }
     scene->users++;
+    ctx->live++;
     *out = r;
     return CRT_OK;
 }
@@ -1541,6 +1547,8 @@ int crt_renderer_destroy(crt_renderer *r)
     free_renderer_buffers(r);
     for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
     r->scene->users--;
+    if (r->nrc) r->nrc->users--;
+    r->ctx->live--;
     delete r;
     return CRT_OK;
 }
@@ -1871,6 +1879,8 @@ extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc
                 o.train_paths > (1u << 22)))
         return fail(CRT_ERR_INVALID_ARG, "bad cache render options");
     join_worker(r);
+    if (r->nrc) r->nrc->users--;
+    if (nrc) nrc->users++;
     r->nrc           = nrc;
     r->nopt          = o;
     r->nrc_warm_done = 0;
@@ -1961,15 +1971,15 @@ int crt_nrc_param_count(void) { return kNrcParams; }
 int crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out)
 {
     if (!ctx || !out) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    crt_nrc_options o;
+    crt_nrc_options_default(&o);
+    if (opt) o = *opt;
+    if (o.max_batch == 0 || o.max_batch > (1u << 24)) return fail(CRT_ERR_INVALID_ARG, "max_batch must be in [1, 2^24]");
+    HIP_CHECK(hipSetDevice(ctx->device));
     auto *n = new crt_nrc();
     n->ctx = ctx;
-    crt_nrc_options_default(&n->opt);
-    if (opt) n->opt = *opt;
-    if (n->opt.max_batch == 0 || n->opt.max_batch > (1u << 24)) {
-        delete n;
-        return fail(CRT_ERR_INVALID_ARG, "max_batch must be in [1, 2^24]");
-    }
-    HIP_CHECK(hipSetDevice(ctx->device));
+    n->opt = o;
+    ctx->live++;
     n->tiles_alloc = nrc_tiles_alloc(n->opt.max_batch);
     int st = CRT_OK;
     auto A = [&](auto **p, size_t c) {
@@ -2000,9 +2010,11 @@ int crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out)
 int crt_nrc_destroy(crt_nrc *n)
 {
     if (!n) return CRT_OK;
+    if (n->users) return fail(CRT_ERR_STATE, "cache still attached to a renderer");
     void *ps[] = {n->params, n->m, n->v, n->W, n->WT, n->grad, n->act, n->partial, n->partial_loss,
                   n->q_stage, n->t_stage, n->o_stage};
     for (void *p : ps) dfree(p);
+    n->ctx->live--;
     delete n;
     return CRT_OK;
 }

@@ -13,6 +13,7 @@ CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Callable
 
 import numpy as np
@@ -40,6 +41,11 @@ class Context:
         check(L.crt_context_create(C.byref(o), C.byref(h)))
         self._h = h
         self._L = L
+        # objects created on the context, closed before it in dependency order
+        self._children = {k: weakref.WeakSet() for k in ("renderer", "nrc", "scene")}
+
+    def _adopt(self, kind: str, obj) -> None:
+        self._children[kind].add(obj)
 
     @property
     def handle(self):
@@ -51,8 +57,13 @@ class Context:
         return buf.value.decode()
 
     def close(self):
+        """Closes the renderers, caches and scenes still open on the context, then the
+        context (crt_context_destroy refuses a context with live objects)."""
         if self._h:
-            self._L.crt_context_destroy(self._h)
+            for kind in ("renderer", "nrc", "scene"):
+                for obj in list(self._children[kind]):
+                    obj.close()
+            check(self._L.crt_context_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -85,6 +96,7 @@ class Scene:
         h = C.c_void_p()
         check(self._L.crt_scene_create(ctx.handle, C.byref(h)))
         self._h = h
+        ctx._adopt("scene", self)
         self._keep = []
 
     @property
@@ -348,10 +360,12 @@ class Renderer:
     def __init__(self, ctx: Context, scene: Scene, res_x: int, res_y: int, bounces: int):
         self.ctx = ctx
         self.scene = scene
+        self.nrc = None
         self._L = A.lib()
         h = C.c_void_p()
         check(self._L.crt_renderer_create(ctx.handle, scene.handle, res_x, res_y, bounces, C.byref(h)))
         self._h = h
+        ctx._adopt("renderer", self)
 
     def start(self) -> None:
         check(self._L.crt_renderer_start(self._h))
@@ -401,6 +415,7 @@ class Renderer:
         o.train_suffix, o.train_steps, o.train = train_suffix, train_steps, 1 if train else 0
         o.warmup_passes = warmup_passes
         check(self._L.crt_renderer_set_nrc(self._h, nrc.handle if nrc is not None else None, C.byref(o)))
+        self.nrc = nrc   # the cache outlives its attachment
 
     def reset(self) -> None:
         check(self._L.crt_renderer_reset(self._h))
@@ -482,7 +497,9 @@ class Nrc:
         o.lr = lr
         h = C.c_void_p()
         check(self._L.crt_nrc_create(ctx.handle, C.byref(o), C.byref(h)))
+        self.ctx = ctx
         self._h = h
+        ctx._adopt("nrc", self)
         self.n_params = self._L.crt_nrc_param_count()
 
     @property
@@ -522,7 +539,7 @@ class Nrc:
 
     def close(self):
         if self._h:
-            self._L.crt_nrc_destroy(self._h)
+            check(self._L.crt_nrc_destroy(self._h))
             self._h = None
 
     def __del__(self):

@@ -196,12 +196,13 @@ int         crt_abi_version(void);
 /* ---- lifecycle: rtcNewDevice (components.h:67) + the renderer's thread pool ----------- */
 void crt_options_default(crt_options *opt);
 int  crt_context_create(const crt_options *opt, crt_context **out);
+/* CRT_ERR_STATE while scenes, renderers or caches created on the context are alive. */
 int  crt_context_destroy(crt_context *ctx);
 int  crt_context_device_name(crt_context *ctx, char *buf, size_t len);
 
 /* ---- scene-load: cr::scene (scene.h:19-53) ------------------------------------------ */
 int crt_scene_create(crt_context *ctx, crt_scene **out);                      /* scene() */
-int crt_scene_destroy(crt_scene *scene);
+int crt_scene_destroy(crt_scene *scene);   /* CRT_ERR_STATE while a renderer uses it */
 /* scene::add_model (scene.cpp:31-34) -> registry::register_model (registry.cpp:51-97) */
 int crt_scene_add_model(crt_scene *scene, const crt_model_data *model, uint32_t *model_id);
 /* entity::instances (components.h:18-21); transforms are column-major affine 4x4
@@ -278,7 +279,7 @@ typedef struct crt_nrc_options {
 } crt_nrc_options;
 void crt_nrc_options_default(crt_nrc_options *opt);
 int  crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out);
-int  crt_nrc_destroy(crt_nrc *nrc);
+int  crt_nrc_destroy(crt_nrc *nrc);   /* CRT_ERR_STATE while attached to a renderer */
 int  crt_nrc_param_count(void);
 int  crt_nrc_get_params(crt_nrc *nrc, float *params);          /* FP32 master weights, host */
 int  crt_nrc_set_params(crt_nrc *nrc, const float *params);    /* also resets the Adam state */

@@ -170,6 +170,30 @@ def test_error_paths():
     ctx.close()
 
 
+def test_destroy_refuses_live_dependents():
+    """crt_context_destroy refuses a context with live scenes / renderers / caches, and
+    crt_nrc_destroy / crt_scene_destroy refuse an object a renderer still uses;
+    Context.close() closes what is left in dependency order."""
+    from crender_amd import Nrc
+    from crender_amd import _abi as A
+
+    L = A.lib()
+    STATE = -5
+    ctx = Context(0)
+    sc = scene_from_spec(ctx, scenes.cornell(1))
+    r = Renderer(ctx, sc, 16, 16, 4)
+    nrc = Nrc(ctx, max_batch=4096, seed=1)
+    r.set_nrc(nrc, train_paths=4096)
+    assert L.crt_context_destroy(ctx.handle) == STATE
+    assert L.crt_nrc_destroy(nrc.handle) == STATE
+    assert L.crt_scene_destroy(sc.handle) == STATE
+    r.set_nrc(None)
+    nrc.close()                       # detached: destroy succeeds
+    r.render(1)
+    ctx.close()                       # closes the renderer, then the scene
+    assert r._h is None and sc.handle is None
+
+
 def test_golden_rays():
     for name, spec in (("cornell_rays", scenes.cornell(1)), ("soup_rays", scenes.soup(20_000, seed=13))):
         g = np.load(GOLDEN / f"{name}.npz")
