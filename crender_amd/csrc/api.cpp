@@ -174,11 +174,12 @@ struct crt_renderer {
     // phase timing: [0] start, [1] after generate, per bounce i < kTimedBounces:
     // [2+4i] before extend, [3+4i] after extend, [4+4i] after shade, [5+4i] after shadow,
     // [kEvAcc] before accumulate, [kEvEnd] end of pass
+    // Two sets, so that a pass can be enqueued while the previous one still runs.
     static constexpr int kTimedBounces = 15, kEvAcc = 62, kEvEnd = 63;
-    hipEvent_t ev[64] = {};
+    hipEvent_t ev[2][64] = {};
+    uint32_t  *err_host = nullptr;     // pinned: each set's copy of the traversal error flag
     double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
     uint64_t extend_launches = 0;
-    uint32_t ms_phase_pending = 0;
     // neural radiance cache
     crt_nrc               *nrc = nullptr;
     crt_nrc_render_options nopt{};
@@ -450,24 +451,17 @@ TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *co
     return q;
 }
 
-struct PhaseTimer {
-    crt_renderer *r;
-    bool          on;
-    int           next = 0;
-    void mark()
-    {
-        if (on && next < 16) (void)hipEventRecord(r->ev[next++], r->ctx->stream);
-    }
-};
-
 // One set of paths: generate, then per bounce extend / shade / shadow.  With timing on,
-// HIP events are recorded around the kernels and read after the pass's final sync.
-int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, bool timing)
+// the events `ev` are recorded around the kernels and read after the pass's final sync;
+// `timed` receives the number of bounces they cover.
+int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, bool timing, hipEvent_t *ev,
+              uint32_t &timed_out)
 {
     crt_context    *ctx = r->ctx;
     const SceneDev &sc  = r->scene->dev;
     hipStream_t     s   = ctx->stream;
-    auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(r->ev[i], s) : hipSuccess; };
+    auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(ev[i], s) : hipSuccess; };
+    timed_out = 0;
     if (int st = ensure_spill(ctx)) return st;   // the autotuner may have resized the grid
     HIP_CHECK(launch_generate(sc, b, rp, s));
     HIP_CHECK(mark(1));
@@ -537,7 +531,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
     }
     if (timing) {
         r->extend_launches += timed;
-        r->ms_phase_pending = timed;
+        timed_out = timed;
     }
     return CRT_OK;
 }
@@ -562,14 +556,22 @@ NrcParams nrc_params(const crt_renderer *r)
 
 int nrc_train_step(crt_renderer *r);
 
-// One sample pass: renderer.cpp:121-125 (one barrier per pass) as a kernel sequence.
-int render_pass(crt_renderer *r)
+// A sample pass in flight: the event set it records into and what finishing it needs.
+struct PassTicket {
+    int      set     = 0;
+    bool     warmup  = false;
+    bool     timing  = false;
+    uint32_t timed   = 0;   // bounces covered by phase events
+};
+
+// Enqueues one sample pass (renderer.cpp:121-125, the pass barrier being stream order) for
+// sample index `sample`, into event set `set`.  Nothing waits for the GPU here.
+int enqueue_pass(crt_renderer *r, uint64_t sample, int set, PassTicket &t)
 {
     crt_context       *ctx = r->ctx;
-    // the context's stream and spill area are shared with queries, batches and the autotuner
-    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     const PathBuffers &b   = r->b;
     hipStream_t        s   = ctx->stream;
+    hipEvent_t        *ev  = r->ev[set];
     RenderParams       rp;
     rp.cam         = to_camera_dev(r->scene->cam);
     rp.aspect      = r->aspect;
@@ -577,60 +579,104 @@ int render_pass(crt_renderer *r)
     rp.rr_start    = r->opt.rr_start;
     rp.mode        = r->opt.mode;
     rp.seed        = r->opt.seed;
-    rp.sample      = (uint32_t)(r->opt.sample_offset + r->samples.load());
+    rp.sample      = (uint32_t)(r->opt.sample_offset + sample);
     rp.nrc         = nrc_params(r);
     // cache warm-up pass: only the training paths (their own Philox stream), no image
-    const bool warmup = rp.nrc.enabled && r->nrc_warm_done < r->nopt.warmup_passes;
-    if (warmup) {
+    t.set    = set;
+    t.warmup = rp.nrc.enabled && r->nrc_warm_done < r->nopt.warmup_passes;
+    if (t.warmup) {
         rp.sample        = 0x40000000u + r->nrc_warm_done;
         rp.nrc.gen_first = b.npix;
     }
-    const bool timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !warmup;
-    HIP_CHECK(hipEventRecord(r->ev[0], s));
-    r->ms_phase_pending = 0;
-    int st = run_paths(r, b, rp, timing);
+    t.timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !t.warmup;
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    int st = run_paths(r, b, rp, t.timing, ev, t.timed);
     if (st) return st;
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries
         HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
         HIP_CHECK(launch_nrc_apply(b, s));
     }
-    if (timing) HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvAcc], s));
-    if (!warmup) HIP_CHECK(launch_accumulate(b, s));
-    HIP_CHECK(hipEventRecord(r->ev[crt_renderer::kEvEnd], s));
-    HIP_CHECK(hipEventSynchronize(r->ev[crt_renderer::kEvEnd]));
+    if (t.timing) HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvAcc], s));
+    if (!t.warmup) HIP_CHECK(launch_accumulate(b, s));
+    HIP_CHECK(hipMemcpyAsync(r->err_host + set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
+    return CRT_OK;
+}
+
+// Waits for an enqueued pass, books its timing, checks the traversal error flag, trains
+// the cache on the pass's training paths and counts the sample.
+int finish_pass(crt_renderer *r, const PassTicket &t)
+{
+    hipEvent_t *ev = r->ev[t.set];
+    HIP_CHECK(hipEventSynchronize(ev[crt_renderer::kEvEnd]));
     float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[crt_renderer::kEvEnd]));
-    if (!warmup) r->ms_phase[5] += ms;
-    if (timing) {
-        HIP_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[crt_renderer::kEvEnd]));
+    if (!t.warmup) r->ms_phase[5] += ms;
+    if (t.timing) {
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
         r->ms_phase[0] += ms;
-        for (uint32_t i = 0; i < r->ms_phase_pending; ++i) {
+        for (uint32_t i = 0; i < t.timed; ++i) {
             const int k = 2 + 4 * (int)i;
-            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k], r->ev[k + 1]));
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
             r->ms_phase[1] += ms;
-            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k + 1], r->ev[k + 2]));
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[k + 1], ev[k + 2]));
             r->ms_phase[2] += ms;
-            HIP_CHECK(hipEventElapsedTime(&ms, r->ev[k + 2], r->ev[k + 3]));
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[k + 2], ev[k + 3]));
             r->ms_phase[3] += ms;
         }
-        HIP_CHECK(hipEventElapsedTime(&ms, r->ev[crt_renderer::kEvAcc], r->ev[crt_renderer::kEvEnd]));
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[crt_renderer::kEvAcc], ev[crt_renderer::kEvEnd]));
         r->ms_phase[4] += ms;
     }
-    uint32_t err = 0;
-    HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
-    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
-    if (rp.nrc.enabled && (r->nopt.train || warmup)) {
+    if (r->err_host[t.set]) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    const bool train = r->nrc && r->bounces > 0 && (r->nopt.train || t.warmup);
+    if (train) {
         const auto t0 = std::chrono::steady_clock::now();
-        if ((st = nrc_train_step(r))) return st;
+        if (int st = nrc_train_step(r)) return st;
         r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    if (warmup) {
+    if (t.warmup) {
         r->nrc_warm_done++;
         return CRT_OK;
     }
     r->samples.fetch_add(1);
     return CRT_OK;
+}
+
+// Up to `n` sample passes under one hold of the context.  Without the neural cache (whose
+// training reads the pass's results on the host) the next pass is enqueued before the
+// previous one is waited for, so the GPU does not idle between passes; stream order keeps
+// the image identical to one pass at a time.  Stops early when `r->stop` is set; `done`
+// receives the number of passes counted.
+int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
+{
+    crt_context *ctx = r->ctx;
+    // the context's stream and spill area are shared with queries, batches and the autotuner
+    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
+    done = 0;
+    const bool overlap = r->nrc == nullptr;
+    PassTicket inflight;
+    bool       busy = false;
+    int        st   = CRT_OK;
+    for (uint32_t i = 0; i < n && st == CRT_OK && !r->stop.load(); ++i) {
+        PassTicket t;
+        st = enqueue_pass(r, r->samples.load() + (busy ? 1u : 0u), busy ? inflight.set ^ 1 : 0, t);
+        if (st == CRT_OK && busy) {
+            st   = finish_pass(r, inflight);
+            busy = false;
+            if (st == CRT_OK) ++done;
+        }
+        if (st != CRT_OK) break;
+        if (overlap) {
+            inflight = t;
+            busy     = true;
+        } else if ((st = finish_pass(r, t)) == CRT_OK) {
+            ++done;
+        }
+    }
+    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) ++done;
+    if (st != CRT_OK) (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
+    return st;
 }
 
 // Training of the cache after a pass: the training paths (slots npix ..) traced
@@ -674,13 +720,20 @@ int check_ready(crt_renderer *r)
     return CRT_OK;
 }
 
+// Passes per hold of the context: other users of the context (queries, batches, the
+// autotuner) get their turn between such groups.
+constexpr uint32_t kPassGroup = 8;
+
 void worker_loop(crt_renderer *r)
 {
     int st = CRT_OK;
     for (;;) {
         if (r->stop.load()) break;
-        if (r->target != 0 && r->samples.load() >= r->target) break;
-        st = render_pass(r);
+        const uint64_t have = r->samples.load();
+        if (r->target != 0 && have >= r->target) break;
+        const uint32_t n = r->target != 0 ? (uint32_t)std::min<uint64_t>(kPassGroup, r->target - have) : kPassGroup;
+        uint32_t done = 0;
+        st = render_passes(r, n, done);
         if (st != CRT_OK) break;
     }
     std::lock_guard<std::mutex> g(r->mu);
@@ -1503,6 +1556,15 @@ int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *oc
     return query(scene, rays, n, nullptr, occluded, memory);
 }
 
+static void destroy_renderer_events(crt_renderer *r)
+{
+    for (auto &set : r->ev)
+        for (hipEvent_t e : set)
+            if (e) (void)hipEventDestroy(e);
+    if (r->err_host) (void)hipHostFree(r->err_host);
+    r->err_host = nullptr;
+}
+
 int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint32_t res_y, uint32_t bounces,
                         crt_renderer **out)
 {
@@ -1518,17 +1580,16 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
     r->H       = res_y;
     r->bounces = bounces;
     r->opt     = crt_render_options{CRT_MODE_COMPAT, 0u, 0u, 0u};
-    for (hipEvent_t &e : r->ev) {
-        hipError_t ee = hipEventCreate(&e);
-        if (ee != hipSuccess) {
-            delete r;
-            return fail(CRT_ERR_HIP, "hipEventCreate");
-        }
-    }
-    const int st = alloc_renderer_buffers(r);
+    int st = CRT_OK;
+    for (auto &set : r->ev)
+        for (hipEvent_t &e : set)
+            if (st == CRT_OK && hipEventCreate(&e) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
+    if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 2 * sizeof(uint32_t)) != hipSuccess)
+        st = fail(CRT_ERR_OUT_OF_MEMORY, "pinned host allocation");
+    if (st == CRT_OK) st = alloc_renderer_buffers(r);
     if (st != CRT_OK) {
         free_renderer_buffers(r);
-        for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
+        destroy_renderer_events(r);
         delete r;
         return st;
     }
@@ -1545,7 +1606,7 @@ int crt_renderer_destroy(crt_renderer *r)
     join_worker(r);
     (void)hipSetDevice(r->ctx->device);
     free_renderer_buffers(r);
-    for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
+    destroy_renderer_events(r);
     r->scene->users--;
     if (r->nrc) r->nrc->users--;
     r->ctx->live--;
@@ -1671,9 +1732,11 @@ int crt_renderer_render(crt_renderer *r, uint32_t passes)
     join_worker(r);
     HIP_CHECK(hipSetDevice(r->ctx->device));
     r->stop.store(false);
-    for (uint32_t i = 0; i < passes; ++i) {
+    for (uint32_t i = 0; i < passes;) {
         if (r->stop.load()) return fail(CRT_ERR_CANCELLED, "cancelled");
-        if ((st = render_pass(r))) return st;
+        uint32_t done = 0;
+        if ((st = render_passes(r, std::min(kPassGroup, passes - i), done))) return st;
+        i += done;
     }
     return CRT_OK;
 }
