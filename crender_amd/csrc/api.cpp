@@ -170,11 +170,10 @@ struct crt_renderer {
     int                      worker_status = CRT_OK;
     std::string              worker_error;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-    // phase timing
-    // phase timing: [0] start, [1] after generate, per bounce i < kTimedBounces:
-    // [2+4i] before extend, [3+4i] after extend, [4+4i] after shade, [5+4i] after shadow,
-    // [kEvAcc] before accumulate, [kEvEnd] end of pass
-    // Two sets, so that a pass can be enqueued while the previous one still runs.
+    // phase timing: [0] start of pass, then one event between consecutive phases (after
+    // generate; per bounce i < kTimedBounces after extend, after shade and, with the sun, after
+    // shadow), [kEvAcc] before accumulate when the cache's inference precedes it, [kEvEnd]
+    // end of pass.  Two sets, so that a pass can be enqueued while the previous one runs.
     static constexpr int kTimedBounces = 15, kEvAcc = 62, kEvEnd = 63;
     hipEvent_t ev[2][64] = {};
     uint32_t  *err_host = nullptr;     // pinned: each set's copy of the traversal error flag
@@ -451,20 +450,39 @@ TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *co
     return q;
 }
 
-// One set of paths: generate, then per bounce extend / shade / shadow.  With timing on,
-// the events `ev` are recorded around the kernels and read after the pass's final sync;
-// `timed` receives the number of bounces they cover.
-int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, bool timing, hipEvent_t *ev,
-              uint32_t &timed_out)
+// A sample pass in flight: the event set it records into and what finishing it needs.
+struct PassTicket {
+    int  set    = 0;
+    bool warmup = false;
+    bool timing = false;
+    // phase events (timing): each phase ends at its event and starts at the one recorded
+    // before it; -1 where a phase did not run
+    uint32_t timed  = 0;    // bounces with phase events
+    int      ev_gen = -1;   // after generate
+    int      ev_acc = -1;   // before accumulate
+    int      ev_ext[crt_renderer::kTimedBounces]    = {};
+    int      ev_shade[crt_renderer::kTimedBounces]  = {};
+    int      ev_shadow[crt_renderer::kTimedBounces] = {};
+};
+
+// One set of paths: generate, then per bounce extend / shade / shadow.  Each traversal
+// launch's epilogue clears the counters the next steps append to, so a bounce is two or
+// three kernels.  With timing on, one event is recorded between consecutive phases (into
+// `ev`, slots 1 ..) and read after the pass's final sync.
+int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hipEvent_t *ev, PassTicket &t)
 {
     crt_context    *ctx = r->ctx;
     const SceneDev &sc  = r->scene->dev;
     hipStream_t     s   = ctx->stream;
-    auto mark = [&](int i) -> hipError_t { return timing ? hipEventRecord(ev[i], s) : hipSuccess; };
-    timed_out = 0;
+    int             nev = 1;   // next free event slot
+    auto mark = [&](int &slot) -> hipError_t {
+        slot = nev++;
+        return hipEventRecord(ev[slot], s);
+    };
+    t.timed = 0;
     if (int st = ensure_spill(ctx)) return st;   // the autotuner may have resized the grid
     HIP_CHECK(launch_generate(sc, b, rp, s));
-    HIP_CHECK(mark(1));
+    if (t.timing) HIP_CHECK(mark(t.ev_gen));
     uint32_t cur = 0;
     // bounces to run: camera paths stop at max_bounces, or at term + 1 at the latest when
     // they end in the cache (later only through alpha pass-through); training paths at
@@ -475,13 +493,12 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
         if (rp.nrc.gen_first >= b.npix) cam = 0;
         nb = std::max(cam, rp.nrc.train_term + 1);
     }
-    const uint32_t timed = std::min<uint32_t>(nb, crt_renderer::kTimedBounces);
+    const uint32_t timed = t.timing ? std::min<uint32_t>(nb, crt_renderer::kTimedBounces) : 0u;
     const bool sort = (ctx->opt.flags & CRT_FLAG_SORT_RAYS) && r->sort.keys;
     for (uint32_t i = 0; i < nb; ++i) {
-        const bool t = timing && i < timed;
+        const bool tm = i < timed;
         if (sort && i > 0) {
             // reorder the secondary rays (counted in the extend phase)
-            if (t) HIP_CHECK(mark(2 + 4 * (int)i));
             uint32_t n = 0;
             HIP_CHECK(hipMemcpyAsync(&n, b.cnt + cur, sizeof n, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -489,15 +506,17 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
             HIP_CHECK(launch_ray_sort(b.q[cur], b.qpath[cur], b.cnt + cur, n, b.q[cur ^ 1u], b.qpath[cur ^ 1u],
                                       b.cnt + (cur ^ 1u), sc.bmin, sc.bmax, r->sort, s));
             cur ^= 1u;
-            HIP_CHECK(launch_bounce_prep(b, cur, s));
-        } else {
-            HIP_CHECK(launch_bounce_prep(b, cur, s));
-            if (t) HIP_CHECK(mark(2 + 4 * (int)i));
         }
+        // the epilogue clears the queue the shading appends to (the previous bounce's
+        // input), the shadow queue, and the work counters; it counts the extension rays
         TraceQueue q = make_queue(b.q[cur], b.hits, b.cnt + cur, b.cnt + kCntWorkExt, b.cnt + kCntErr);
         q.tstats     = b.tstats;
+        q.done       = b.cnt + kCntDoneExt;
+        q.clear0     = b.cnt + (cur ^ 1u);
+        q.clear1     = b.cnt + kCntShadow;
+        q.stat       = b.stats + 0;
         HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
-        if (t) HIP_CHECK(mark(3 + 4 * (int)i));
+        if (tm) HIP_CHECK(mark(t.ev_ext[i]));
         if (b.shade_perm) {
             // material-sorted shading: the queue's hits grouped by material class
             HIP_CHECK(launch_shade_sort(sc, b, cur, s));
@@ -505,16 +524,19 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
         } else {
             HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
         }
-        if (t) HIP_CHECK(mark(4 + 4 * (int)i));
+        if (tm) HIP_CHECK(mark(t.ev_shade[i]));
+        if (tm) t.ev_shadow[i] = -1;
         if (sc.sun_enabled) {
-            HIP_CHECK(launch_shadow_prep(b, s));
             if (!sc.any_alpha) {
                 TraceQueue qs = make_queue(b.sq, HitSoA{}, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
                 qs.tstats     = b.tstats;
+                qs.done       = b.cnt + kCntDoneShadow;
+                qs.stat       = b.stats + 1;
                 HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, trace_cfg(ctx), s));
                 HIP_CHECK(launch_shadow_resolve(b, s));
             } else {
                 // closest hit + alpha pass-through, renderer.cpp:330-345 (bounded at 64 steps)
+                HIP_CHECK(launch_shadow_prep(b, s));
                 for (int step = 0; step < 64; ++step) {
                     TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
                     HIP_CHECK(launch_trace_closest(sc, qs, ctx->spill, trace_cfg(ctx), s));
@@ -525,14 +547,13 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, boo
                     if (left == 0) break;
                 }
             }
+            if (tm) HIP_CHECK(mark(t.ev_shadow[i]));
         }
-        if (t) HIP_CHECK(mark(5 + 4 * (int)i));
         cur ^= 1u;
     }
-    if (timing) {
-        r->extend_launches += timed;
-        timed_out = timed;
-    }
+    t.timed = timed;
+    r->extend_launches += timed;
+    if (t.timing) t.ev_acc = nev - 1;   // accumulate starts at the last phase event
     return CRT_OK;
 }
 
@@ -556,13 +577,6 @@ NrcParams nrc_params(const crt_renderer *r)
 
 int nrc_train_step(crt_renderer *r);
 
-// A sample pass in flight: the event set it records into and what finishing it needs.
-struct PassTicket {
-    int      set     = 0;
-    bool     warmup  = false;
-    bool     timing  = false;
-    uint32_t timed   = 0;   // bounces covered by phase events
-};
 
 // Enqueues one sample pass (renderer.cpp:121-125, the pass barrier being stream order) for
 // sample index `sample`, into event set `set`.  Nothing waits for the GPU here.
@@ -590,14 +604,17 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, int set, PassTicket &t)
     }
     t.timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !t.warmup;
     HIP_CHECK(hipEventRecord(ev[0], s));
-    int st = run_paths(r, b, rp, t.timing, ev, t.timed);
+    int st = run_paths(r, b, rp, ev, t);
     if (st) return st;
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries
         HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
         HIP_CHECK(launch_nrc_apply(b, s));
+        if (t.timing) {
+            t.ev_acc = crt_renderer::kEvAcc;
+            HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvAcc], s));
+        }
     }
-    if (t.timing) HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvAcc], s));
     if (!t.warmup) HIP_CHECK(launch_accumulate(b, s));
     HIP_CHECK(hipMemcpyAsync(r->err_host + set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
@@ -614,19 +631,23 @@ int finish_pass(crt_renderer *r, const PassTicket &t)
     HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[crt_renderer::kEvEnd]));
     if (!t.warmup) r->ms_phase[5] += ms;
     if (t.timing) {
-        HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        r->ms_phase[0] += ms;
+        auto add = [&](int phase, int from, int to) -> hipError_t {
+            hipError_t e = hipEventElapsedTime(&ms, ev[from], ev[to]);
+            if (e == hipSuccess) r->ms_phase[phase] += ms;
+            return e;
+        };
+        HIP_CHECK(add(0, 0, t.ev_gen));
+        int prev = t.ev_gen;
         for (uint32_t i = 0; i < t.timed; ++i) {
-            const int k = 2 + 4 * (int)i;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-            r->ms_phase[1] += ms;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev[k + 1], ev[k + 2]));
-            r->ms_phase[2] += ms;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev[k + 2], ev[k + 3]));
-            r->ms_phase[3] += ms;
+            HIP_CHECK(add(1, prev, t.ev_ext[i]));
+            HIP_CHECK(add(2, t.ev_ext[i], t.ev_shade[i]));
+            prev = t.ev_shade[i];
+            if (t.ev_shadow[i] >= 0) {
+                HIP_CHECK(add(3, prev, t.ev_shadow[i]));
+                prev = t.ev_shadow[i];
+            }
         }
-        HIP_CHECK(hipEventElapsedTime(&ms, ev[crt_renderer::kEvAcc], ev[crt_renderer::kEvEnd]));
-        r->ms_phase[4] += ms;
+        HIP_CHECK(add(4, t.ev_acc, crt_renderer::kEvEnd));
     }
     if (r->err_host[t.set]) return fail(CRT_ERR_HIP, "traversal stack overflow");
     const bool train = r->nrc && r->bounces > 0 && (r->nopt.train || t.warmup);

@@ -175,6 +175,13 @@ struct TraceQueue {
     const uint32_t   *ray_path;
     const uint32_t   *path_job;
     const TraceScene *scenes;
+    // Epilogue (done != null): the last block to finish zeroes the work counters, the words
+    // clear0 / clear1 (when set; never `count`), adds the ray count to *stat (when set) and
+    // resets *done, which must be 0 at launch.  It prepares the next bounce's counters
+    // without a kernel of its own.
+    uint32_t           *done;
+    uint32_t           *clear0, *clear1;
+    unsigned long long *stat;
 };
 // Launch shape of the persistent traversal: blocks in the grid and LDS stack entries per
 // lane (8, 16 or 32).
@@ -263,7 +270,7 @@ constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
 enum : uint32_t {
     kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4, kCntNrcQ = 5, kCntTrain = 6,
-    kCntWorkExt = 8, kCntWorkShadow = 16, kCntWords = 32
+    kCntWorkExt = 8, kCntWorkShadow = 16, kCntDoneExt = 24, kCntDoneShadow = 25, kCntWords = 32
 };
 
 // Neural radiance cache hooks of the integrator.  Camera paths (slots < npix) query the

@@ -149,17 +149,28 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
         b.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         b.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (rp.max_bounces == 0 && !trainp) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
-        if (p == first) {
-            b.cnt[kCntQ0]   = rp.max_bounces > 0 ? end - first : 0u;
-            b.cnt[kCntNrcQ] = 0;
+    }
+    if (blockIdx.x == 0) {
+        // the pass's counters: queue 0, the cache queries, and the traversal work counters and
+        // finished-block counts (the traversal epilogues clear the rest bounce by bounce)
+        if (threadIdx.x < 8) {
+            b.cnt[kCntWorkExt + threadIdx.x]    = 0;
+            b.cnt[kCntWorkShadow + threadIdx.x] = 0;
+        }
+        if (threadIdx.x == 0) {
+            b.cnt[kCntQ0]          = rp.max_bounces > 0 && end > first ? end - first : 0u;
+            b.cnt[kCntNrcQ]        = 0;
+            b.cnt[kCntDoneExt]     = 0;
+            b.cnt[kCntDoneShadow]  = 0;
         }
     }
     // every path starts at total_bounces = 1 (renderer.cpp:271)
     block_add_stat(b.stats, 2, live && p < b.npix ? 1u : 0u);
 }
 
-// Before bounce i: clear the next queue, the shadow queue and the traversal work counters;
-// count the extension rays about to be traced.
+// Before bounce i of a batch: clear the next queue, the shadow queue and the traversal work
+// counters; count the extension rays about to be traced.  (The single-scene pass leaves this
+// to the epilogue of the previous traversal launch.)
 __global__ void k_bounce_prep(PathBuffers b, uint32_t cur)
 {
     const uint32_t t = threadIdx.x;
@@ -853,7 +864,7 @@ static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); 
 
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s)
 {
-    k_generate<<<blocks(b.P - std::min(rp.nrc.gen_first, b.P)), 256, 0, s>>>(sc, b, rp);
+    k_generate<<<std::max(1u, blocks(b.P - std::min(rp.nrc.gen_first, b.P))), 256, 0, s>>>(sc, b, rp);
     return hipGetLastError();
 }
 hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)

@@ -279,6 +279,25 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // kStats: the instrumented variant; counts each lane's inner-node tests, leaf visits and
 // triangle tests, and each wave's iterations of the node and leaf loops, and adds them to
 // q.tstats when the wave leaves the kernel.
+// The launch's epilogue (TraceQueue::done): every block reports when its waves are done with
+// the work counters; the last one clears the next bounce's counters.  No fence is needed:
+// every work-counter atomic returned (its value was used) before the block's barrier, the
+// last block reads nothing other blocks wrote, and the end of the kernel publishes its
+// stores to the next launch.  (A device-scope fence here would write back and invalidate
+// the XCD's L2 under the blocks still traversing.)
+__device__ __forceinline__ void trace_epilogue(const TraceQueue &q, uint32_t n)
+{
+    if (!q.done) return;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (atomicAdd(q.done, 1u) != gridDim.x - 1u) return;
+    for (uint32_t x = 0; x < kXcds; ++x) q.counters[x] = 0u;
+    if (q.clear0) *q.clear0 = 0u;
+    if (q.clear1) *q.clear1 = 0u;
+    if (q.stat) atomicAdd(q.stat, (unsigned long long)n);
+    *q.done = 0u;
+}
+
 template <int W, bool kAny, bool kMulti, uint32_t S, bool kInst, bool kStats = false>
 __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev sc, TraceQueue q, uint8_t *occ,
                                                                       uint32_t *spill)
@@ -286,7 +305,10 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
     uint32_t c_nodes = 0, c_leaves = 0, c_tris = 0, c_wnode = 0, c_wleaf = 0;
     __shared__ uint2 s_stack[(S + 1) * kTraceBlock];
     const uint32_t n = *q.count;
-    if (n == 0) return;
+    if (n == 0) {
+        trace_epilogue(q, n);
+        return;
+    }
     const uint32_t lane  = lane_id();
     const uint32_t xcd   = blockIdx.x % kXcds;
     const uint32_t slice = (n + kXcds - 1) / kXcds;
@@ -532,6 +554,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             atomicAdd(q.tstats + 4, (unsigned long long)c_wleaf);
         }
     }
+    trace_epilogue(q, n);
 }
 
 }  // namespace
