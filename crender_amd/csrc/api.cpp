@@ -83,6 +83,11 @@ namespace crt {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace crt
 
+// Camera paths a renderer's wavefront aims at (whole sample passes of the image, at most
+// kMaxWavefrontPasses of them): 8M paths put ~4 passes of a 1080p image in one wavefront.
+constexpr uint32_t kWavefrontPaths     = 1u << 23;
+constexpr uint32_t kMaxWavefrontPasses = 8;
+
 struct crt_context {
     int         device = 0;
     crt_options opt{};
@@ -157,6 +162,7 @@ struct crt_renderer {
     uint64_t     target = 0;         // renderer.h:81: 0 = unlimited
     crt_render_options opt{};
     std::atomic<uint64_t> samples{0};
+    uint32_t     kmax = 1;           // sample passes per wavefront (path slots: kmax * W * H)
     PathBuffers  b{};
     RaySortScratch sort{};
     float4      *display = nullptr;
@@ -356,11 +362,17 @@ int alloc_renderer_buffers(crt_renderer *r)
     b.W    = r->W;
     b.H    = r->H;
     b.npix = r->W * r->H;
-    // cache-training paths share the wavefront: slots npix .. npix + train_paths - 1
-    const size_t T = r->nrc ? r->nopt.train_paths : 0u;
+    // a wavefront carries up to kmax sample passes of the image (about wavefront_paths camera
+    // paths: a traversal launch's fixed cost, its ramp-up and the tail of its longest rays,
+    // is then spread over more rays), then train_paths cache-training paths per pass
+    const uint64_t wf = r->ctx->opt.wavefront_paths ? r->ctx->opt.wavefront_paths : kWavefrontPaths;
+    r->kmax = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(kMaxWavefrontPasses, wf / b.npix));
+    b.ncam = b.npix * r->kmax;
+    const size_t T = r->nrc ? (size_t)r->nopt.train_paths * r->kmax : 0u;
     const size_t S = r->nrc ? r->nopt.train_suffix : 0u;
-    b.P = b.npix + (uint32_t)T;
+    b.P = b.ncam + (uint32_t)T;
     const size_t P = b.P, NP = b.npix;
+    if ((uint64_t)b.ncam + T > 0xFFFFFFFFull) return fail(CRT_ERR_INVALID_ARG, "too many path slots");
     int st;
 #define RA(ptr, n)                          \
     if ((st = ralloc(r, ptr, n)) != CRT_OK) return st
@@ -452,9 +464,11 @@ TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *co
 
 // A sample pass in flight: the event set it records into and what finishing it needs.
 struct PassTicket {
-    int  set    = 0;
-    bool warmup = false;
-    bool timing = false;
+    int      set    = 0;
+    bool     warmup = false;
+    bool     timing = false;
+    uint32_t units  = 1;   // passes the wavefront counts for: its samples, or 1 warm-up pass
+    uint32_t nsamp  = 0;   // sample passes of the image (0: cache warm-up)
     // phase events (timing): each phase ends at its event and starts at the one recorded
     // before it; -1 where a phase did not run
     uint32_t timed  = 0;    // bounces with phase events
@@ -490,7 +504,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
     uint32_t nb = rp.max_bounces;
     if (rp.nrc.enabled) {
         uint32_t cam = sc.any_alpha ? rp.max_bounces : std::min(rp.max_bounces, rp.nrc.term + 2);
-        if (rp.nrc.gen_first >= b.npix) cam = 0;
+        if (rp.nsamp == 0) cam = 0;
         nb = std::max(cam, rp.nrc.train_term + 1);
     }
     const uint32_t timed = t.timing ? std::min<uint32_t>(nb, crt_renderer::kTimedBounces) : 0u;
@@ -575,12 +589,13 @@ NrcParams nrc_params(const crt_renderer *r)
     return np;
 }
 
-int nrc_train_step(crt_renderer *r);
+int nrc_train_step(crt_renderer *r, uint32_t mult);
 
 
-// Enqueues one sample pass (renderer.cpp:121-125, the pass barrier being stream order) for
-// sample index `sample`, into event set `set`.  Nothing waits for the GPU here.
-int enqueue_pass(crt_renderer *r, uint64_t sample, int set, PassTicket &t)
+// Enqueues one wavefront of `nsamp` sample passes (renderer.cpp:121-125, the pass barrier
+// being stream order) from sample index `sample` on, into event set `set` -- or one cache
+// warm-up pass while those are due.  Nothing waits for the GPU here.
+int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, PassTicket &t)
 {
     crt_context       *ctx = r->ctx;
     const PathBuffers &b   = r->b;
@@ -594,14 +609,19 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, int set, PassTicket &t)
     rp.mode        = r->opt.mode;
     rp.seed        = r->opt.seed;
     rp.sample      = (uint32_t)(r->opt.sample_offset + sample);
+    rp.nsamp       = nsamp;
     rp.nrc         = nrc_params(r);
     // cache warm-up pass: only the training paths (their own Philox stream), no image
     t.set    = set;
     t.warmup = rp.nrc.enabled && r->nrc_warm_done < r->nopt.warmup_passes;
     if (t.warmup) {
-        rp.sample        = 0x40000000u + r->nrc_warm_done;
-        rp.nrc.gen_first = b.npix;
+        rp.sample = 0x40000000u + r->nrc_warm_done;
+        rp.nsamp  = 0;
     }
+    t.nsamp = rp.nsamp;
+    t.units = t.warmup ? 1u : nsamp;
+    // training paths: train_paths per sample pass (one warm-up pass counts as one)
+    if (rp.nrc.enabled) rp.nrc.train_live = r->nopt.train_paths * std::max(1u, rp.nsamp);
     t.timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !t.warmup;
     HIP_CHECK(hipEventRecord(ev[0], s));
     int st = run_paths(r, b, rp, ev, t);
@@ -615,7 +635,7 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, int set, PassTicket &t)
             HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvAcc], s));
         }
     }
-    if (!t.warmup) HIP_CHECK(launch_accumulate(b, s));
+    if (!t.warmup) HIP_CHECK(launch_accumulate(b, rp.nsamp, s));
     HIP_CHECK(hipMemcpyAsync(r->err_host + set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
     return CRT_OK;
@@ -653,22 +673,23 @@ int finish_pass(crt_renderer *r, const PassTicket &t)
     const bool train = r->nrc && r->bounces > 0 && (r->nopt.train || t.warmup);
     if (train) {
         const auto t0 = std::chrono::steady_clock::now();
-        if (int st = nrc_train_step(r)) return st;
+        if (int st = nrc_train_step(r, std::max(1u, t.nsamp))) return st;
         r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (t.warmup) {
         r->nrc_warm_done++;
         return CRT_OK;
     }
-    r->samples.fetch_add(1);
+    r->samples.fetch_add(t.nsamp);
     return CRT_OK;
 }
 
-// Up to `n` sample passes under one hold of the context.  Without the neural cache (whose
-// training reads the pass's results on the host) the next pass is enqueued before the
-// previous one is waited for, so the GPU does not idle between passes; stream order keeps
-// the image identical to one pass at a time.  Stops early when `r->stop` is set; `done`
-// receives the number of passes counted.
+// Up to `n` passes under one hold of the context, as wavefronts of up to kmax sample passes
+// each (cache warm-up passes one at a time).  Without the neural cache (whose training
+// reads a wavefront's results on the host) the next wavefront is enqueued before the
+// previous one is waited for, so the GPU does not idle between them; stream order and the
+// per-pixel sample order of the accumulation keep the image identical to one pass at a
+// time.  Stops early when `r->stop` is set; `done` receives the number of passes counted.
 int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
 {
     crt_context *ctx = r->ctx;
@@ -677,47 +698,51 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     done = 0;
     const bool overlap = r->nrc == nullptr;
     PassTicket inflight;
-    bool       busy = false;
-    int        st   = CRT_OK;
-    for (uint32_t i = 0; i < n && st == CRT_OK && !r->stop.load(); ++i) {
-        PassTicket t;
-        st = enqueue_pass(r, r->samples.load() + (busy ? 1u : 0u), busy ? inflight.set ^ 1 : 0, t);
+    bool       busy   = false;
+    uint32_t   queued = 0;
+    int        st     = CRT_OK;
+    while (queued < n && st == CRT_OK && !r->stop.load()) {
+        PassTicket     t;
+        const uint32_t k = std::min(r->kmax, n - queued);
+        st = enqueue_pass(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, busy ? inflight.set ^ 1 : 0, t);
         if (st == CRT_OK && busy) {
             st   = finish_pass(r, inflight);
             busy = false;
-            if (st == CRT_OK) ++done;
+            if (st == CRT_OK) done += inflight.units;
         }
         if (st != CRT_OK) break;
+        queued += t.units;
         if (overlap) {
             inflight = t;
             busy     = true;
         } else if ((st = finish_pass(r, t)) == CRT_OK) {
-            ++done;
+            done += t.units;
         }
     }
-    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) ++done;
+    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) done += inflight.units;
     if (st != CRT_OK) (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
     return st;
 }
 
-// Training of the cache after a pass: the training paths (slots npix ..) traced
-// train_suffix bounces past the query vertex, recording every vertex from term_bounce on,
-// and terminated into the cache (self-training) give targets L_v = (L_path - L_before_v) /
-// T_v; train_steps Adam steps follow.
-int nrc_train_step(crt_renderer *r)
+// Training of the cache after a wavefront of `mult` passes: the training paths (slots
+// ncam .., train_paths per pass) traced train_suffix bounces past the query vertex,
+// recording every vertex from term_bounce on, and terminated into the cache (self-training)
+// give targets L_v = (L_path - L_before_v) / T_v; train_steps Adam steps per pass follow.
+int nrc_train_step(crt_renderer *r, uint32_t mult)
 {
     crt_nrc           *n = r->nrc;
     const PathBuffers &t = r->b;
     hipStream_t        s = r->ctx->stream;
     const uint32_t     S = r->nopt.train_suffix;
     HIP_CHECK(hipMemsetAsync(t.cnt + kCntTrain, 0, sizeof(uint32_t), s));
-    const uint32_t max_out = (t.P - t.npix) * S;
-    HIP_CHECK(launch_nrc_targets(t, S, max_out, s));
+    const uint32_t live    = r->nopt.train_paths * mult;
+    const uint32_t max_out = live * S;
+    HIP_CHECK(launch_nrc_targets(t, S, max_out, live, s));
     uint32_t count = 0;
     HIP_CHECK(hipMemcpyAsync(&count, t.cnt + kCntTrain, sizeof count, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     count = std::min(count, max_out);
-    const uint32_t steps = std::max<uint32_t>(1u, r->nopt.train_steps);
+    const uint32_t steps = std::max<uint32_t>(1u, r->nopt.train_steps) * mult;
     const uint32_t batch = std::min(count / steps, n->opt.max_batch);
     if (batch < 64) return CRT_OK;
     std::lock_guard<std::mutex> g(n->mu);
@@ -741,7 +766,7 @@ int check_ready(crt_renderer *r)
     return CRT_OK;
 }
 
-// Passes per hold of the context: other users of the context (queries, batches, the
+// Wavefronts per hold of the context: other users of the context (queries, batches, the
 // autotuner) get their turn between such groups.
 constexpr uint32_t kPassGroup = 8;
 
@@ -752,7 +777,8 @@ void worker_loop(crt_renderer *r)
         if (r->stop.load()) break;
         const uint64_t have = r->samples.load();
         if (r->target != 0 && have >= r->target) break;
-        const uint32_t n = r->target != 0 ? (uint32_t)std::min<uint64_t>(kPassGroup, r->target - have) : kPassGroup;
+        const uint32_t g = kPassGroup * r->kmax;
+        const uint32_t n = r->target != 0 ? (uint32_t)std::min<uint64_t>(g, r->target - have) : g;
         uint32_t done = 0;
         st = render_passes(r, n, done);
         if (st != CRT_OK) break;
@@ -1756,7 +1782,7 @@ int crt_renderer_render(crt_renderer *r, uint32_t passes)
     for (uint32_t i = 0; i < passes;) {
         if (r->stop.load()) return fail(CRT_ERR_CANCELLED, "cancelled");
         uint32_t done = 0;
-        if ((st = render_passes(r, std::min(kPassGroup, passes - i), done))) return st;
+        if ((st = render_passes(r, std::min(kPassGroup * r->kmax, passes - i), done))) return st;
         i += done;
     }
     return CRT_OK;
@@ -2385,6 +2411,7 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
     PathBuffers b{};
     b.P    = P;
     b.npix = P;   // no cache-training paths in batches
+    b.ncam = P;
     if ((st = alloc_path_core(L, b, P))) return st;
     if ((st = lalloc(L, &b.pjob, P))) return st;
     if ((st = lalloc(L, &b.ppix, P))) return st;

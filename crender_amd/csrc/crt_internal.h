@@ -234,8 +234,10 @@ constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multip
 enum : uint32_t { kModeCompat = 0, kModePbr = 1 };
 
 struct PathBuffers {
-    uint32_t  W, H, P;          // image size, path slots (W*H camera paths, then cache-training paths)
-    uint32_t  npix;             // W*H: slots below are pixels, slots from here on train the cache
+    uint32_t  W, H, P;          // image size, path slots (camera paths, then cache-training paths)
+    uint32_t  npix;             // W*H
+    uint32_t  ncam;             // camera path slots: npix per sample pass a wavefront can carry;
+                                // slots from here on train the cache
     RaySoA    q[2];             // extension queues (ping-pong)
     uint32_t *qpath[2];         // path slot of each queued ray
     HitSoA    hits;
@@ -282,7 +284,7 @@ struct NrcParams {
     float    footprint_c;   // > 0: query at `term` only if the path spread exceeds c * a0, else at term + 1
     uint32_t train_term;    // bounce at which training paths query the cache
     uint32_t rec_first, rec_slots;
-    uint32_t gen_first;     // first path slot generated this pass (npix: training paths only)
+    uint32_t train_live;    // training path slots in use this wavefront (from ncam on)
     f3       bmin, binv;    // scene box, for normalising query positions to [0,1]
 };
 
@@ -290,6 +292,9 @@ struct RenderParams {
     camera_dev cam;
     float      aspect;
     uint32_t   max_bounces, rr_start, mode, seed, sample;
+    // sample passes of the image in the wavefront (0: cache training only).  Camera slot p
+    // is pixel p / nsamp, sample `sample + p % nsamp`: the passes of a pixel sit side by side.
+    uint32_t   nsamp = 1;
     NrcParams  nrc;
 };
 
@@ -336,11 +341,12 @@ hipError_t launch_shade_sort(const SceneDev &sc, const PathBuffers &b, uint32_t 
 hipError_t launch_shadow_prep(const PathBuffers &b, hipStream_t s);
 hipError_t launch_shadow_resolve(const PathBuffers &b, hipStream_t s);
 hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hipStream_t s);
-hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s);
+hipError_t launch_accumulate(const PathBuffers &b, uint32_t nsamp, hipStream_t s);
 hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s);
 hipError_t launch_fill(float4 *p, uint64_t n, float v, hipStream_t s);
 hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s);
-hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s);
+hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, uint32_t train_live,
+                              hipStream_t s);
 hipError_t nrc_launch_infer_dev(const float *q, float *out, const uint32_t *n_dev, uint32_t n_max, const __bf16 *WT,
                                 hipStream_t s);
 hipError_t launch_batch_generate(const BatchDev &bd, const PathBuffers &b, hipStream_t s);

@@ -122,33 +122,37 @@ __device__ __forceinline__ void put_ray(const RaySoA &q, uint32_t k, f3 o, f3 d)
     q.tf[k] = kInf;               // model.cpp:23
 }
 
-// Camera rays of the pass: one per pixel (slot p = pixel), then the cache-training paths,
-// which start at random pixels (slots >= npix); slots below rp.nrc.gen_first are skipped.
+// Rays of the wavefront: rp.nsamp camera rays per pixel (slot p = pixel * nsamp + j, sample
+// rp.sample + j), then rp.nrc.train_live cache-training paths (slots ncam ..), which start at
+// random pixels.  Queue 0 holds them in that order, without the unused slots between.
 __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, RenderParams rp)
 {
-    const uint32_t first = rp.nrc.gen_first;
-    const uint32_t end   = rp.nrc.enabled ? b.P : b.npix;   // training slots idle without the cache
-    const uint32_t p     = first + blockIdx.x * blockDim.x + threadIdx.x;
-    const bool     live  = p < end;
+    const uint32_t ns     = rp.nsamp ? rp.nsamp : 1u;
+    const uint32_t ncl    = b.npix * rp.nsamp;                           // camera slots in use
+    const uint32_t ntrain = rp.nrc.enabled ? rp.nrc.train_live : 0u;
+    const uint32_t t      = blockIdx.x * blockDim.x + threadIdx.x;      // queue slot
+    const bool     live   = t < ncl + ntrain;
+    const bool     trainp = t >= ncl;
     if (live) {
-        const bool     trainp = p >= b.npix;
-        const uint32_t sample = trainp ? (rp.sample | 0x80000000u) : rp.sample;
-        const uint32_t pix    = trainp ? philox4x32_10(u4{p, sample, 0xFFFFFFFEu, 0u}, rp.seed, kKeyHi).x % b.npix : p;
+        const uint32_t p = trainp ? b.ncam + (t - ncl) : t;
+        // RNG key: the pixel of a camera path; npix + its index for a training path
+        const uint32_t key    = trainp ? b.npix + (t - ncl) : p / ns;
+        const uint32_t sample = trainp ? (rp.sample | 0x80000000u) : rp.sample + p % ns;
+        const uint32_t pix = trainp ? philox4x32_10(u4{key, sample, 0xFFFFFFFEu, 0u}, rp.seed, kKeyHi).x % b.npix : key;
         const uint32_t x = pix % b.W, y = pix / b.W;
-        const rnd4     j = rng4(rp.seed, p, sample, kJitterBounce, 0);
+        const rnd4     j = rng4(rp.seed, key, sample, kJitterBounce, 0);
         if (trainp)
             for (uint32_t r = 0; r < rp.nrc.rec_slots; ++r)
-                b.r_thr[(size_t)(p - b.npix) * rp.nrc.rec_slots + r] = make_float4(0, 0, 0, 0);
+                b.r_thr[(size_t)(p - b.ncam) * rp.nrc.rec_slots + r] = make_float4(0, 0, 0, 0);
         if (rp.nrc.enabled) b.nstate[p] = make_float4(0, 0, 0, 0);
         f3 o, d;
         // renderer.cpp:260-263
         camera_ray(rp.cam, ((float)x + j.a) / (float)b.W, ((float)y + j.b) / (float)b.H, rp.aspect, o, d);
-        const uint32_t slot = p - first;
-        put_ray(b.q[0], slot, o, d);
-        b.qpath[0][slot] = p;
+        put_ray(b.q[0], t, o, d);
+        b.qpath[0][t] = p;
         b.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         b.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (rp.max_bounces == 0 && !trainp) aov_write(b, p, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+        if (rp.max_bounces == 0 && !trainp && p % ns == ns - 1u) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
     }
     if (blockIdx.x == 0) {
         // the pass's counters: queue 0, the cache queries, and the traversal work counters and
@@ -158,14 +162,14 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
             b.cnt[kCntWorkShadow + threadIdx.x] = 0;
         }
         if (threadIdx.x == 0) {
-            b.cnt[kCntQ0]          = rp.max_bounces > 0 && end > first ? end - first : 0u;
+            b.cnt[kCntQ0]          = rp.max_bounces > 0 ? ncl + ntrain : 0u;
             b.cnt[kCntNrcQ]        = 0;
             b.cnt[kCntDoneExt]     = 0;
             b.cnt[kCntDoneShadow]  = 0;
         }
     }
     // every path starts at total_bounces = 1 (renderer.cpp:271)
-    block_add_stat(b.stats, 2, live && p < b.npix ? 1u : 0u);
+    block_add_stat(b.stats, 2, live && !trainp ? 1u : 0u);
 }
 
 // Before bounce i of a batch: clear the next queue, the shadow queue and the traversal work
@@ -245,7 +249,7 @@ __device__ __forceinline__ HitRec hit_record(const SceneDev &sc, f3 o, f3 d_in, 
 __device__ __forceinline__ bool nrc_terminates(const NrcParams &np, const PathBuffers &b, uint32_t p, uint32_t bounce,
                                                const struct HitRec &h, f3 d)
 {
-    if (p >= b.npix) return bounce >= np.train_term;   // training path
+    if (p >= b.ncam) return bounce >= np.train_term;   // training path
     if (bounce < np.term) return false;
     if (np.footprint_c <= 0.0f || bounce > np.term) return true;
     const float4 st  = b.nstate[p];
@@ -287,6 +291,13 @@ struct ShadeOut {
     uint32_t seg;
 };
 
+// True for the camera slot whose AOVs the pass writes: the wavefront's last sample of its
+// pixel (the reference keeps the last pass's AOVs).
+__device__ __forceinline__ bool aov_slot(const PathBuffers &b, const RenderParams &rp, uint32_t p)
+{
+    return p < b.ncam && (rp.nsamp <= 1u || p % rp.nsamp == rp.nsamp - 1u);
+}
+
 // Shades queue entry k (path slot p, pixel pix, absolute sample index `sample`) at `bounce`:
 // renderer.cpp:273-356 (compat) or the physically based variant (pbr).
 __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, uint32_t k,
@@ -305,11 +316,11 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
     f3             thr  = mk3(thr4.x, thr4.y, thr4.z);
     float4         rad4 = b.rad[p];
     f3             rad  = mk3(rad4.x, rad4.y, rad4.z);
-    const bool     can_continue = bounce + 1 < (p >= b.npix ? rp.nrc.train_term + 1 : rp.max_bounces);
+    const bool     can_continue = bounce + 1 < (p >= b.ncam ? rp.nrc.train_term + 1 : rp.max_bounces);
     qthr = thr;
     if (tri == kInvalidRef) {
         const f3 s = sky(sc, d);
-        if (bounce == 0 && p < b.npix) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
+        if (bounce == 0 && aov_slot(b, rp, p)) aov_write(b, pix, s, mk3(0, 0, 0), 0.0f);
         rad = rad + thr * s;
     } else {
         const uint32_t     inst = sc.instanced && b.hits.inst ? b.hits.inst[k] : kInvalidRef;
@@ -319,22 +330,22 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
         seg = 1;
         if (col.w == 0.0f) {
             // alpha pass-through (renderer.cpp:294-301)
-            if (bounce == 0 && p < b.npix) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
+            if (bounce == 0 && aov_slot(b, rp, p)) aov_write(b, pix, mk3(0, 0, 0), mk3(0, 0, 0), 0.0f);
             no = h.point + d * 0.1f;
             nd = d;
             push_next = can_continue;
         } else if (rp.nrc.enabled && nrc_terminates(rp.nrc, b, p, bounce, h, d)) {
             // the cache stands in for this vertex's emission, direct light and everything after it
             const f3 cc = mk3(col.x, col.y, col.z);
-            if (bounce == 0 && p < b.npix) aov_write(b, pix, cc, h.normal, h.dist);
+            if (bounce == 0 && aov_slot(b, rp, p)) aov_write(b, pix, cc, h.normal, h.dist);
             nrc_query(rp.nrc, h.point, d, h.normal, cc, m, nq);
             do_query = true;
         } else {
             const rnd4 r  = rng4(rp.seed, pix, sample, bounce, 0);
             const f3   n  = h.normal;
             const f3   cc = mk3(col.x, col.y, col.z);
-            if (p >= b.npix && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
-                const size_t slot = (size_t)(p - b.npix) * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
+            if (p >= b.ncam && bounce >= rp.nrc.rec_first && bounce < rp.nrc.rec_first + rp.nrc.rec_slots) {
+                const size_t slot = (size_t)(p - b.ncam) * rp.nrc.rec_slots + (bounce - rp.nrc.rec_first);
                 nrc_query(rp.nrc, h.point, d, n, cc, m, b.rq + slot * 13);
                 b.r_thr[slot] = make_float4(thr.x, thr.y, thr.z, 1.0f);
                 b.r_rad[slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
@@ -451,7 +462,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
                     nee_n = nf;
                 }
             }
-            if (bounce == 0 && p < b.npix) aov_write(b, pix, rp.mode == kModeCompat ? alb : cc, n, h.dist);
+            if (bounce == 0 && aov_slot(b, rp, p)) aov_write(b, pix, rp.mode == kModeCompat ? alb : cc, n, h.dist);
             const f3 thr_prev = thr;
             if (rp.mode == kModeCompat) {
                 thr = thr * alb;
@@ -551,7 +562,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
             const JobDev &J = jobs[b.pjob[p]];
             shade_path(J.sc, b, J.rp, k, p, b.ppix[p], b.psamp[p], bounce, cur, o);
         } else {
-            shade_path(sc, b, rp, k, p, p, p >= b.npix ? (rp.sample | 0x80000000u) : rp.sample, bounce, cur, o);
+            const uint32_t ns     = rp.nsamp ? rp.nsamp : 1u;
+            const bool     trainp = p >= b.ncam;
+            const uint32_t key    = trainp ? b.npix + (p - b.ncam) : p / ns;
+            shade_path(sc, b, rp, k, p, key, trainp ? (rp.sample | 0x80000000u) : rp.sample + p % ns, bounce, cur, o);
         }
     }
     const uint32_t nqs = (!kMulti && rp.nrc.enabled) ? block_append(&b.cnt[kCntNrcQ], o.do_query) : 0u;
@@ -571,7 +585,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev sc, PathBuffers b, Rende
         put_ray(b.q[cur ^ 1u], nn, o.no, o.nd);
         b.qpath[cur ^ 1u][nn] = p;
     }
-    block_add_stat(b.stats, 2, p < b.npix ? o.seg : 0u);
+    block_add_stat(b.stats, 2, p < b.ncam ? o.seg : 0u);
 }
 
 __global__ __launch_bounds__(256) void k_shadow_resolve(PathBuffers b)
@@ -672,16 +686,24 @@ __global__ void k_shadow_swap(PathBuffers b)
     if (threadIdx.x < 8) b.cnt[kCntWorkShadow + threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_accumulate(PathBuffers b)
+// raw[pixel] += the radiance of each of the wavefront's samples of the pixel, in sample
+// order: the same sums as one accumulation per pass.
+__global__ __launch_bounds__(256) void k_accumulate(PathBuffers b, uint32_t nsamp)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= b.npix) return;
     const uint32_t x = p % b.W, y = p / b.W;
     const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
-    const float4   r = b.rad[p];
-    b.raw[3 * (size_t)q + 0] += r.x;
-    b.raw[3 * (size_t)q + 1] += r.y;
-    b.raw[3 * (size_t)q + 2] += r.z;
+    float sx = b.raw[3 * (size_t)q + 0], sy = b.raw[3 * (size_t)q + 1], sz = b.raw[3 * (size_t)q + 2];
+    for (uint32_t j = 0; j < nsamp; ++j) {
+        const float4 r = b.rad[(size_t)p * nsamp + j];
+        sx += r.x;
+        sy += r.y;
+        sz += r.z;
+    }
+    b.raw[3 * (size_t)q + 0] = sx;
+    b.raw[3 * (size_t)q + 1] = sy;
+    b.raw[3 * (size_t)q + 2] = sz;
 }
 
 // renderer.cpp:371-383: pow(clamp(sum / n, 0, 1), 1/2.2); FLT_MAX before the first pass
@@ -750,13 +772,14 @@ __global__ __launch_bounds__(256) void k_nrc_apply(PathBuffers b)
 
 // Training targets from the recorded vertices: L_v = (L_path - L_before_v) / T_v per
 // channel; valid, finite samples are compacted into (tq, tt).
-__global__ __launch_bounds__(256) void k_nrc_targets(PathBuffers b, uint32_t rec_slots, uint32_t max_out)
+__global__ __launch_bounds__(256) void k_nrc_targets(PathBuffers b, uint32_t rec_slots, uint32_t max_out,
+                                                     uint32_t train_live)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t p = b.npix + i / rec_slots;   // training path slot
+    const uint32_t p = b.ncam + i / rec_slots;   // training path slot
     bool  ok = false;
     float tg[3] = {0, 0, 0};
-    if (p < b.P) {
+    if (p < b.ncam + train_live) {
         const float4 t = b.r_thr[i];
         if (t.w > 0.0f) {
             const float4 rb = b.rad[p], r0 = b.r_rad[i];
@@ -864,7 +887,8 @@ static inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); 
 
 hipError_t launch_generate(const SceneDev &sc, const PathBuffers &b, const RenderParams &rp, hipStream_t s)
 {
-    k_generate<<<std::max(1u, blocks(b.P - std::min(rp.nrc.gen_first, b.P))), 256, 0, s>>>(sc, b, rp);
+    const uint64_t n = (uint64_t)b.npix * rp.nsamp + (rp.nrc.enabled ? rp.nrc.train_live : 0u);
+    k_generate<<<std::max(1u, blocks(n)), 256, 0, s>>>(sc, b, rp);
     return hipGetLastError();
 }
 hipError_t launch_bounce_prep(const PathBuffers &b, uint32_t cur, hipStream_t s)
@@ -895,9 +919,9 @@ hipError_t launch_shadow_alpha_step(const SceneDev &sc, const PathBuffers &b, hi
     k_shadow_swap<<<1, 64, 0, s>>>(b);
     return hipGetLastError();
 }
-hipError_t launch_accumulate(const PathBuffers &b, hipStream_t s)
+hipError_t launch_accumulate(const PathBuffers &b, uint32_t nsamp, hipStream_t s)
 {
-    k_accumulate<<<blocks(b.npix), 256, 0, s>>>(b);
+    k_accumulate<<<blocks(b.npix), 256, 0, s>>>(b, nsamp);
     return hipGetLastError();
 }
 hipError_t launch_display(const float *raw, uint64_t n, uint32_t npix, float4 *out, hipStream_t s)
@@ -916,9 +940,11 @@ hipError_t launch_nrc_apply(const PathBuffers &b, hipStream_t s)
     k_nrc_apply<<<blocks(b.P), 256, 0, s>>>(b);
     return hipGetLastError();
 }
-hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, hipStream_t s)
+hipError_t launch_nrc_targets(const PathBuffers &b, uint32_t rec_slots, uint32_t max_out, uint32_t train_live,
+                              hipStream_t s)
 {
-    k_nrc_targets<<<blocks((uint64_t)(b.P - b.npix) * rec_slots), 256, 0, s>>>(b, rec_slots, max_out);
+    k_nrc_targets<<<std::max(1u, blocks((uint64_t)train_live * rec_slots)), 256, 0, s>>>(b, rec_slots, max_out,
+                                                                                       train_live);
     return hipGetLastError();
 }
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,

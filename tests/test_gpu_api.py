@@ -194,6 +194,38 @@ def test_destroy_refuses_live_dependents():
     assert r._h is None and sc.handle is None
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_multi_pass_wavefronts_equal_single_passes(mode):
+    """A wavefront carrying several sample passes (the default for small images: up to 8)
+    gives the same image, AOVs and counts as one pass per wavefront, for a pass count that
+    is not a multiple of the wavefront's, synchronously and in the background."""
+    spec = scenes.sunlit(3, mode=mode)
+    res = (40, 30)
+    out = {}
+    for name, wf in (("one", 1), ("many", 0)):
+        ctx = Context(0, wavefront_paths=wf)
+        sc = scene_from_spec(ctx, spec)
+        r = Renderer(ctx, sc, res[0], res[1], 5)
+        r.set_resolution(*res)
+        r.set_options(mode=mode, rr_start=2, seed=9)
+        r.render(5)
+        r.render(6)
+        bg = Renderer(ctx, sc, res[0], res[1], 5)
+        bg.set_resolution(*res)
+        bg.set_options(mode=mode, rr_start=2, seed=9)
+        bg.set_target_spp(11)
+        bg.start()
+        bg.wait()
+        st = r.current_stats()
+        out[name] = (r.raw(), r.current_normals(), r.current_albedos(), r.current_depths(), bg.raw(),
+                     r.current_sample_count(), bg.current_sample_count(), st["extension_rays"], st["shadow_rays"])
+        ctx.close()
+    a, b = out["one"], out["many"]
+    for x, y in zip(a[:5], b[:5]):
+        assert np.array_equal(x, y)
+    assert a[5:] == b[5:] and a[5] == 11 and a[6] == 11
+
+
 def test_golden_rays():
     for name, spec in (("cornell_rays", scenes.cornell(1)), ("soup_rays", scenes.soup(20_000, seed=13))):
         g = np.load(GOLDEN / f"{name}.npz")
