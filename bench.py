@@ -133,12 +133,12 @@ def nrc_equal_time(ctx, sc, ref_spp: int) -> dict:
     plain.close()
     nrc = Nrc(ctx, seed=5)
     r = mk(2)
-    warm = 8
+    warm = 4
     r.set_nrc(nrc, warmup_passes=warm)
     tw0 = time.perf_counter()
     r.render(warm)
     t_warm = time.perf_counter() - tw0
-    nc = _curve(r, [1, 2, 4, 8, 16, 32, 64], refimg, t_start=t_warm)
+    nc = _curve(r, [4, 8, 16, 32, 64], refimg, t_start=t_warm)   # the plain tracer's checkpoints
     st = r.current_stats()
     r.close()
     nrc.close()
@@ -147,7 +147,7 @@ def nrc_equal_time(ctx, sc, ref_spp: int) -> dict:
     t_match = _time_to_reach(nc, target)
     return {
         "workload": "soup 1M tris, 1920x1080, pbr, depth 16, RR from bounce 3; cache: term bounce 1 (footprint c=0.01), "
-                    "65536 training paths/pass, 4 bounces past the query, 24 Adam steps/pass, 8 warm-up passes",
+                    "65536 training paths/pass, 4 bounces past the query, 24 Adam steps/pass, 4 warm-up passes",
         "reference_spp": ref_spp,
         "plain": [{"spp": a, "seconds": b, "mse": c} for a, b, c in pc],
         "cache": [{"spp": a, "seconds": b, "mse": c} for a, b, c in nc],

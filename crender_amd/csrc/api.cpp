@@ -151,6 +151,11 @@ struct crt_nrc {
     uint32_t        tiles_alloc = 0;
     uint64_t        step = 0;
     int             users = 0;   // renderers the cache is attached to
+    // Training after a renderer's wavefront runs on its own stream, beside the next
+    // wavefront's traversal; the next inference waits for ev_trained.
+    hipStream_t     ts = nullptr;
+    hipEvent_t      ev_targets = nullptr, ev_trained = nullptr;
+    std::atomic<bool> pending{false};  // training enqueued on ts and not yet waited for
     std::mutex      mu;
 };
 
@@ -182,7 +187,8 @@ struct crt_renderer {
     // end of pass.  Two sets, so that a pass can be enqueued while the previous one runs.
     static constexpr int kTimedBounces = 15, kEvAcc = 62, kEvEnd = 63;
     hipEvent_t ev[2][64] = {};
-    uint32_t  *err_host = nullptr;     // pinned: each set's copy of the traversal error flag
+    uint32_t  *err_host = nullptr;     // pinned: per event set, the traversal error flag [set] and
+                                       // the cache-training target count [2 + set]
     double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
     uint64_t extend_launches = 0;
     // neural radiance cache
@@ -469,6 +475,7 @@ struct PassTicket {
     bool     timing = false;
     uint32_t units  = 1;   // passes the wavefront counts for: its samples, or 1 warm-up pass
     uint32_t nsamp  = 0;   // sample passes of the image (0: cache warm-up)
+    bool     train  = false;   // cache training targets were computed for this wavefront
     // phase events (timing): each phase ends at its event and starts at the one recorded
     // before it; -1 where a phase did not run
     uint32_t timed  = 0;    // bounces with phase events
@@ -589,7 +596,8 @@ NrcParams nrc_params(const crt_renderer *r)
     return np;
 }
 
-int nrc_train_step(crt_renderer *r, uint32_t mult);
+int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count);
+int nrc_wait_training(crt_nrc *n);
 
 
 // Enqueues one wavefront of `nsamp` sample passes (renderer.cpp:121-125, the pass barrier
@@ -627,7 +635,9 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     int st = run_paths(r, b, rp, ev, t);
     if (st) return st;
     if (rp.nrc.enabled) {
-        // one inference for the camera and the training paths' queries
+        // one inference for the camera and the training paths' queries, with the weights of
+        // every training step enqueued so far
+        if (r->nrc->pending) HIP_CHECK(hipStreamWaitEvent(s, r->nrc->ev_trained, 0));
         HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
         HIP_CHECK(launch_nrc_apply(b, s));
         if (t.timing) {
@@ -636,6 +646,16 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
         }
     }
     if (!t.warmup) HIP_CHECK(launch_accumulate(b, rp.nsamp, s));
+    t.train = rp.nrc.enabled && (r->nopt.train || t.warmup);
+    if (t.train) {
+        // training targets from the wavefront's training paths; their count goes to pinned
+        // memory for the host, which sizes the training steps
+        const uint32_t S = r->nopt.train_suffix, live = rp.nrc.train_live;
+        HIP_CHECK(hipMemsetAsync(b.cnt + kCntTrain, 0, sizeof(uint32_t), s));
+        HIP_CHECK(launch_nrc_targets(b, S, live * S, live, s));
+        HIP_CHECK(hipMemcpyAsync(r->err_host + 2 + set, b.cnt + kCntTrain, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(r->nrc->ev_targets, s));
+    }
     HIP_CHECK(hipMemcpyAsync(r->err_host + set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
     return CRT_OK;
@@ -670,10 +690,9 @@ int finish_pass(crt_renderer *r, const PassTicket &t)
         HIP_CHECK(add(4, t.ev_acc, crt_renderer::kEvEnd));
     }
     if (r->err_host[t.set]) return fail(CRT_ERR_HIP, "traversal stack overflow");
-    const bool train = r->nrc && r->bounces > 0 && (r->nopt.train || t.warmup);
-    if (train) {
+    if (t.train) {
         const auto t0 = std::chrono::steady_clock::now();
-        if (int st = nrc_train_step(r, std::max(1u, t.nsamp))) return st;
+        if (int st = nrc_train_step(r, std::max(1u, t.nsamp), r->err_host[2 + t.set])) return st;
         r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (t.warmup) {
@@ -721,6 +740,10 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     }
     if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) done += inflight.units;
     if (st != CRT_OK) (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
+    if (r->nrc) {
+        const int sw = nrc_wait_training(r->nrc);
+        if (st == CRT_OK) st = sw;
+    }
     return st;
 }
 
@@ -728,24 +751,20 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
 // ncam .., train_paths per pass) traced train_suffix bounces past the query vertex,
 // recording every vertex from term_bounce on, and terminated into the cache (self-training)
 // give targets L_v = (L_path - L_before_v) / T_v; train_steps Adam steps per pass follow.
-int nrc_train_step(crt_renderer *r, uint32_t mult)
+int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count)
 {
-    crt_nrc           *n = r->nrc;
-    const PathBuffers &t = r->b;
-    hipStream_t        s = r->ctx->stream;
-    const uint32_t     S = r->nopt.train_suffix;
-    HIP_CHECK(hipMemsetAsync(t.cnt + kCntTrain, 0, sizeof(uint32_t), s));
-    const uint32_t live    = r->nopt.train_paths * mult;
-    const uint32_t max_out = live * S;
-    HIP_CHECK(launch_nrc_targets(t, S, max_out, live, s));
-    uint32_t count = 0;
-    HIP_CHECK(hipMemcpyAsync(&count, t.cnt + kCntTrain, sizeof count, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    count = std::min(count, max_out);
+    crt_nrc           *n  = r->nrc;
+    const PathBuffers &t  = r->b;
+    const uint32_t     S  = r->nopt.train_suffix;
+    count = std::min(count, r->nopt.train_paths * mult * S);
     const uint32_t steps = std::max<uint32_t>(1u, r->nopt.train_steps) * mult;
     const uint32_t batch = std::min(count / steps, n->opt.max_batch);
     if (batch < 64) return CRT_OK;
     std::lock_guard<std::mutex> g(n->mu);
+    // the training stream starts after the targets; the next wavefront's inference waits
+    // for ev_trained, and its targets (which overwrite tq / tt) come after that inference
+    hipStream_t s = n->ts;
+    HIP_CHECK(hipStreamWaitEvent(s, n->ev_targets, 0));
     for (uint32_t k = 0; k < steps; ++k) {
         const float *q  = t.tq + (size_t)k * batch * 13;
         const float *tt = t.tt + (size_t)k * batch * 3;
@@ -754,8 +773,21 @@ int nrc_train_step(crt_renderer *r, uint32_t mult)
         HIP_CHECK(nrc_launch_adam(n->partial, nrc_partial_rows(batch), n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                                   n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
     }
+    HIP_CHECK(hipEventRecord(n->ev_trained, s));
+    n->pending = true;
     // the loss of the last step (sum of its per-tile partials) is read when stats ask for it
     r->loss_tiles = (batch + 63) / 64;
+    return CRT_OK;
+}
+
+// Waits for the cache training enqueued by renderers (the cache API and the end of a group
+// of wavefronts see finished weights).
+int nrc_wait_training(crt_nrc *n)
+{
+    if (n && n->pending) {
+        HIP_CHECK(hipStreamSynchronize(n->ts));
+        n->pending = false;
+    }
     return CRT_OK;
 }
 
@@ -1631,7 +1663,7 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
     for (auto &set : r->ev)
         for (hipEvent_t &e : set)
             if (st == CRT_OK && hipEventCreate(&e) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
-    if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 2 * sizeof(uint32_t)) != hipSuccess)
+    if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 4 * sizeof(uint32_t)) != hipSuccess)
         st = fail(CRT_ERR_OUT_OF_MEMORY, "pinned host allocation");
     if (st == CRT_OK) st = alloc_renderer_buffers(r);
     if (st != CRT_OK) {
@@ -1889,6 +1921,7 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
         stats->wave_leaf_steps = ts[4];
     }
     if (r->nrc && r->loss_tiles) {
+        if (int wst = nrc_wait_training(r->nrc)) return wst;
         std::lock_guard<std::mutex> g(r->nrc->mu);
         std::vector<float> pl(r->loss_tiles);
         HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
@@ -2090,8 +2123,12 @@ int crt_nrc_create(crt_context *ctx, const crt_nrc_options *opt, crt_nrc **out)
     n->ctx = ctx;
     n->opt = o;
     ctx->live++;
-    n->tiles_alloc = nrc_tiles_alloc(n->opt.max_batch);
     int st = CRT_OK;
+    if (hipStreamCreateWithFlags(&n->ts, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&n->ev_targets, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&n->ev_trained, hipEventDisableTiming) != hipSuccess)
+        st = fail(CRT_ERR_HIP, "cache stream / events");
+    n->tiles_alloc = nrc_tiles_alloc(n->opt.max_batch);
     auto A = [&](auto **p, size_t c) {
         if (st == CRT_OK && dalloc(p, c) != hipSuccess) st = fail(CRT_ERR_OUT_OF_MEMORY, "nrc allocation");
     };
@@ -2121,6 +2158,11 @@ int crt_nrc_destroy(crt_nrc *n)
 {
     if (!n) return CRT_OK;
     if (n->users) return fail(CRT_ERR_STATE, "cache still attached to a renderer");
+    (void)hipSetDevice(n->ctx->device);
+    if (n->ts) (void)hipStreamSynchronize(n->ts);
+    if (n->ev_targets) (void)hipEventDestroy(n->ev_targets);
+    if (n->ev_trained) (void)hipEventDestroy(n->ev_trained);
+    if (n->ts) (void)hipStreamDestroy(n->ts);
     void *ps[] = {n->params, n->m, n->v, n->W, n->WT, n->grad, n->act, n->partial, n->partial_loss,
                   n->q_stage, n->t_stage, n->o_stage};
     for (void *p : ps) dfree(p);
@@ -2132,6 +2174,7 @@ int crt_nrc_destroy(crt_nrc *n)
 int crt_nrc_get_params(crt_nrc *n, float *params)
 {
     if (!n || !params) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
     HIP_CHECK(hipMemcpy(params, n->params, sizeof(float) * kNrcParams, hipMemcpyDeviceToHost));
     return CRT_OK;
 }
@@ -2139,6 +2182,7 @@ int crt_nrc_get_params(crt_nrc *n, float *params)
 int crt_nrc_set_params(crt_nrc *n, const float *params)
 {
     if (!n || !params) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
     std::lock_guard<std::mutex> g(n->mu);
     return nrc_upload(n, params);
 }
@@ -2146,6 +2190,7 @@ int crt_nrc_set_params(crt_nrc *n, const float *params)
 int crt_nrc_infer(crt_nrc *n, const float *queries, uint32_t count, float *out, int memory)
 {
     if (!n || ((!queries || !out) && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
     if (count == 0) return CRT_OK;
     std::lock_guard<std::mutex> g(n->mu);
     int st = memory == CRT_MEM_HOST ? nrc_stage(n, count) : CRT_OK;
@@ -2164,6 +2209,7 @@ static int nrc_fwd_bwd(crt_nrc *n, const float *queries, const float *targets, u
                        float *loss)
 {
     if (count > n->opt.max_batch) return fail(CRT_ERR_INVALID_ARG, "batch larger than max_batch");
+    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
     int st = memory == CRT_MEM_HOST ? nrc_stage(n, count) : CRT_OK;
     if (st) return st;
     const float *q = nrc_input(n, queries, (size_t)count * kNrcQuery, n->q_stage, memory, &st);
@@ -2201,6 +2247,7 @@ int crt_nrc_gradient(crt_nrc *n, const float *queries, const float *targets, uin
 int crt_nrc_train_step(crt_nrc *n, const float *queries, const float *targets, uint32_t count, float *loss, int memory)
 {
     if (!n || ((!queries || !targets) && count)) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
     if (count == 0) return CRT_OK;
     std::lock_guard<std::mutex> g(n->mu);
     int st = nrc_fwd_bwd(n, queries, targets, count, memory, loss);
