@@ -305,6 +305,8 @@ def main(argv=None) -> int:
             "bvh_width": info["width"],
             "waves_per_cu": args.waves_per_cu or 8,
             "trace_stack": args.stack or 32,
+            # sample passes per wavefront: the library's default rule, floor(2^23 / (W*H)) in [1, 8]
+            "passes_per_wavefront": max(1, min(8, (1 << 23) // (res[0] * res[1]))),
             "sort_rays": args.sort_rays,
             "sort_materials": args.sort_materials,
             "device": dev,

@@ -476,6 +476,7 @@ struct PassTicket {
     uint32_t units  = 1;   // passes the wavefront counts for: its samples, or 1 warm-up pass
     uint32_t nsamp  = 0;   // sample passes of the image (0: cache warm-up)
     bool     train  = false;   // cache training targets were computed for this wavefront
+    uint32_t count  = 0;       // their number (read when the wavefront is finished)
     // phase events (timing): each phase ends at its event and starts at the one recorded
     // before it; -1 where a phase did not run
     uint32_t timed  = 0;    // bounces with phase events
@@ -599,11 +600,25 @@ NrcParams nrc_params(const crt_renderer *r)
 int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count);
 int nrc_wait_training(crt_nrc *n);
 
+// Enqueues the cache training of a finished wavefront (on the cache's stream).
+int train_after(crt_renderer *r, const PassTicket &t)
+{
+    if (!t.train) return CRT_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int  st = nrc_train_step(r, std::max(1u, t.nsamp), t.count);
+    r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
 
 // Enqueues one wavefront of `nsamp` sample passes (renderer.cpp:121-125, the pass barrier
 // being stream order) from sample index `sample` on, into event set `set` -- or one cache
-// warm-up pass while those are due.  Nothing waits for the GPU here.
-int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, PassTicket &t)
+// warm-up pass while those are due.  Nothing waits for the GPU here.  `train_prev`: a
+// finished wavefront whose cache training is enqueued here, once this wavefront's
+// traversal is (host launches of the training steps then overlap its GPU work), and
+// before its inference, which needs the trained weights.
+int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, PassTicket &t,
+                 const PassTicket *train_prev = nullptr)
 {
     crt_context       *ctx = r->ctx;
     const PathBuffers &b   = r->b;
@@ -634,6 +649,7 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     HIP_CHECK(hipEventRecord(ev[0], s));
     int st = run_paths(r, b, rp, ev, t);
     if (st) return st;
+    if (train_prev && (st = train_after(r, *train_prev))) return st;
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries, with the weights of
         // every training step enqueued so far
@@ -663,7 +679,7 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
 
 // Waits for an enqueued pass, books its timing, checks the traversal error flag, trains
 // the cache on the pass's training paths and counts the sample.
-int finish_pass(crt_renderer *r, const PassTicket &t)
+int finish_pass(crt_renderer *r, PassTicket &t)
 {
     hipEvent_t *ev = r->ev[t.set];
     HIP_CHECK(hipEventSynchronize(ev[crt_renderer::kEvEnd]));
@@ -690,11 +706,7 @@ int finish_pass(crt_renderer *r, const PassTicket &t)
         HIP_CHECK(add(4, t.ev_acc, crt_renderer::kEvEnd));
     }
     if (r->err_host[t.set]) return fail(CRT_ERR_HIP, "traversal stack overflow");
-    if (t.train) {
-        const auto t0 = std::chrono::steady_clock::now();
-        if (int st = nrc_train_step(r, std::max(1u, t.nsamp), r->err_host[2 + t.set])) return st;
-        r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
+    t.count = r->err_host[2 + t.set];
     if (t.warmup) {
         r->nrc_warm_done++;
         return CRT_OK;
@@ -716,14 +728,17 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     done = 0;
     const bool overlap = r->nrc == nullptr;
-    PassTicket inflight;
+    PassTicket inflight, trained;
     bool       busy   = false;
+    bool       due    = false;   // `trained` still needs its cache training enqueued
     uint32_t   queued = 0;
     int        st     = CRT_OK;
     while (queued < n && st == CRT_OK && !r->stop.load()) {
         PassTicket     t;
         const uint32_t k = std::min(r->kmax, n - queued);
-        st = enqueue_pass(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, busy ? inflight.set ^ 1 : 0, t);
+        st  = enqueue_pass(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, busy ? inflight.set ^ 1 : 0, t,
+                           due ? &trained : nullptr);
+        due = false;
         if (st == CRT_OK && busy) {
             st   = finish_pass(r, inflight);
             busy = false;
@@ -736,8 +751,11 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
             busy     = true;
         } else if ((st = finish_pass(r, t)) == CRT_OK) {
             done += t.units;
+            trained = t;
+            due     = t.train;
         }
     }
+    if (due && st == CRT_OK) st = train_after(r, trained);
     if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) done += inflight.units;
     if (st != CRT_OK) (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
     if (r->nrc) {
