@@ -677,8 +677,9 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     return CRT_OK;
 }
 
-// Waits for an enqueued pass, books its timing, checks the traversal error flag, trains
-// the cache on the pass's training paths and counts the sample.
+// Waits for an enqueued wavefront, books its timing, checks the traversal error flag,
+// records its cache-training target count (the training itself is enqueued by the caller,
+// after the next wavefront's traversal) and counts its samples.
 int finish_pass(crt_renderer *r, PassTicket &t)
 {
     hipEvent_t *ev = r->ev[t.set];
