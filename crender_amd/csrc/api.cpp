@@ -517,6 +517,11 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
     }
     const uint32_t timed = t.timing ? std::min<uint32_t>(nb, crt_renderer::kTimedBounces) : 0u;
     const bool sort = (ctx->opt.flags & CRT_FLAG_SORT_RAYS) && r->sort.keys;
+    // While the cache trains beside the traversal, the traversal keeps a 16-entry LDS stack:
+    // two 32-entry traversal blocks (2 x 67.6 KB) leave no room in a CU's LDS for a training
+    // block (31.7 KB), two 16-entry ones (2 x 34.3 KB) do.
+    TraceConfig tc = trace_cfg(ctx);
+    if (rp.nrc.enabled && r->nopt.train) tc.stack = std::min<uint32_t>(tc.stack, 16u);
     for (uint32_t i = 0; i < nb; ++i) {
         const bool tm = i < timed;
         if (sort && i > 0) {
@@ -537,7 +542,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
         q.clear0     = b.cnt + (cur ^ 1u);
         q.clear1     = b.cnt + kCntShadow;
         q.stat       = b.stats + 0;
-        HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, trace_cfg(ctx), s));
+        HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, tc, s));
         if (tm) HIP_CHECK(mark(t.ev_ext[i]));
         if (b.shade_perm) {
             // material-sorted shading: the queue's hits grouped by material class
@@ -554,14 +559,14 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
                 qs.tstats     = b.tstats;
                 qs.done       = b.cnt + kCntDoneShadow;
                 qs.stat       = b.stats + 1;
-                HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, trace_cfg(ctx), s));
+                HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, tc, s));
                 HIP_CHECK(launch_shadow_resolve(b, s));
             } else {
                 // closest hit + alpha pass-through, renderer.cpp:330-345 (bounded at 64 steps)
                 HIP_CHECK(launch_shadow_prep(b, s));
                 for (int step = 0; step < 64; ++step) {
                     TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
-                    HIP_CHECK(launch_trace_closest(sc, qs, ctx->spill, trace_cfg(ctx), s));
+                    HIP_CHECK(launch_trace_closest(sc, qs, ctx->spill, tc, s));
                     HIP_CHECK(launch_shadow_alpha_step(sc, b, s));
                     uint32_t left = 0;
                     HIP_CHECK(hipMemcpyAsync(&left, b.cnt + kCntShadow, sizeof left, hipMemcpyDeviceToHost, s));
