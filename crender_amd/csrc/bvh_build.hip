@@ -227,7 +227,7 @@ __device__ void sah_wave(const int *bins, SahOut *out)
     }
 }
 
-// Leaf / split decision (C_trav = 1, C_isect = 1).  Returns 1 for a leaf; otherwise
+// Leaf / split decision (C_trav = 0.5, C_isect = 1).  Returns 1 for a leaf; otherwise
 // sets nleft and whether the split is by position (axis < 0).
 __device__ __forceinline__ int decide(const SahOut &s, uint32_t max_leaf, uint32_t &nleft, int &axis)
 {
@@ -235,7 +235,7 @@ __device__ __forceinline__ int decide(const SahOut &s, uint32_t max_leaf, uint32
     if (n <= 1) return 1;
     const float pa = area3(s.lo[0], s.lo[1], s.lo[2], s.hi[0], s.hi[1], s.hi[2]);
     if (s.axis >= 0 && pa > 0.0f) {
-        const float split_cost = 1.0f + s.cost / pa;
+        const float split_cost = 0.5f + s.cost / pa;
         if (n <= max_leaf && (float)n <= split_cost) return 1;
         nleft = s.nleft;
         axis  = s.axis;
