@@ -23,9 +23,12 @@ aggregate Mrays/s and per-scene latency.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -39,40 +42,109 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_RAY = 48
 
 
-def cpu_baseline(spec, res, bounces, seconds_target: float, threads: int) -> dict:
-    """Oracle ("port") render of a band of rows of the same frame on host cores."""
+def _host_info() -> dict:
+    """nproc, the cores this process may run on, and the CPU model (lscpu)."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Thread(s) per core", "Core(s) per socket"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _baseline_library() -> tuple[str, str]:
+    """The oracle's sources built for this host as the reference's Release build is
+    (-O3 -march=native, FP contraction allowed); the prebuilt checker build otherwise."""
+    out = Path(tempfile.mkdtemp(prefix="crt_baseline_")) / "liboracle_native.so"
+    try:
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "baseline", f"BASELINE_OUT={out}"], check=True,
+                       capture_output=True, timeout=300)
+        return str(out), "-O3 -march=native"
+    except (OSError, subprocess.SubprocessError):
+        return str(ROOT / "oracle" / "liboracle.so"), "-O2 -march=x86-64-v2 -ffp-contract=off (native build failed)"
+
+
+def cpu_baseline(spec, res, bounces, seconds_target: float) -> dict:
+    """The oracle's restatement of the reference loop (one task per row, a barrier per sample
+    pass, renderer.cpp:116-144,240-256) on this host's cores: median of 3 timed runs after one
+    warm-up, each on the same band of rows sized for about seconds_target / 3."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O  # CPU baseline leg only
 
+    libpath, flags = _baseline_library()
+    O.use_library(libpath)
+    host = _host_info()
+    # every core the process may use; on a GPU box that is its CPU share (OMP_NUM_THREADS)
+    threads = host["affinity"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     t0 = time.perf_counter()
     bvh = O.Bvh(spec.corners())
     build_s = time.perf_counter() - t0
     W, H = res
-    # calibrate on one full-frame pass, then size the sample for the target CPU time: whole
-    # frames over several sample passes, or a band of rows when one frame takes longer
-    r = O.render(spec, res=res, spp=1, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh, nthreads=threads,
-                 aovs=False)
-    per_frame = max(r["stats"]["seconds"], 1e-6)
-    if per_frame < seconds_target:
+    kw = dict(res=res, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh, nthreads=threads, aovs=False)
+    # warm-up and calibration: one band of 64 rows
+    r = O.render(spec, spp=1, rows=(H // 2 - 32, H // 2 + 32), **kw)
+    per_row = max(r["stats"]["seconds"], 1e-6) / 64
+    run_s = seconds_target / 3
+    if per_row * H < run_s:
         nrows, r0 = H, 0
-        spp = int(min(256, max(1, round(seconds_target / per_frame))))
+        spp = int(min(256, max(1, round(run_s / (per_row * H)))))
     else:
-        nrows = int(min(H, max(8, H * seconds_target / per_frame)))
+        nrows = int(min(H, max(8, run_s / per_row)))
         r0, spp = max(0, H // 2 - nrows // 2), 1
-    r = O.render(spec, res=res, spp=spp, max_bounces=bounces, seed=0, trav=O.BVH32, bvh=bvh,
-                 rows=(r0, r0 + nrows), nthreads=threads, aovs=False)
-    st = r["stats"]
-    rays = st["ext_rays"] + st["shadow_rays"]
+    runs = []
+    for _ in range(3):
+        st = O.render(spec, spp=spp, rows=(r0, r0 + nrows), **kw)["stats"]
+        runs.append(st)
+    rates = sorted((s["ext_rays"] + s["shadow_rays"]) / s["seconds"] / 1e6 for s in runs)
+    med = runs[[((s["ext_rays"] + s["shadow_rays"]) / s["seconds"] / 1e6) for s in runs].index(rates[1])]
+    rays = med["ext_rays"] + med["shadow_rays"]
     return {
-        "value": rays / st["seconds"] / 1e6,
+        "value": rates[1],
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"rows {r0}-{r0 + nrows} of the {W}x{H} frame x {spp} spp ({nrows * W * spp} paths, {rays} rays, "
-                  f"{st['seconds']:.1f} s); CPU BVH (FP32 binned SAH, built in {build_s:.1f} s, untimed); "
-                  f"Embree is not available on this image",
-        "ref_counting_mrays_s": st["ref_segments"] / st["seconds"] / 1e6,
+        "sample": f"median of 3 runs after a warm-up, each rows {r0}-{r0 + nrows} of the {W}x{H} frame x {spp} spp "
+                  f"({nrows * W * spp} paths, {rays} rays, {med['seconds']:.1f} s); oracle restatement of "
+                  f"renderer.cpp:240-384 built {flags}, one task per row, barrier per pass, CPU BVH (FP32 "
+                  f"binned SAH, built in {build_s:.1f} s, untimed); Embree is not available on this image",
+        "runs_mrays_s": rates,
+        "host": host,
+        "ref_counting_mrays_s": med["ref_segments"] / med["seconds"] / 1e6,
     }
+
+
+# Sources that decide what the closest-hit traversal reads and writes: HBM traffic measured
+# with profiler counters holds only for the build they were measured on.
+TRAVERSAL_SOURCES = ("crender_amd/csrc/traverse.hip", "crender_amd/csrc/crt_math.h", "crender_amd/csrc/crt_internal.h",
+                     "crender_amd/csrc/bvh_build.hip", "crender_amd/csrc/integrator.hip")
+PMC_FILE = ROOT / "profiles" / "trace_closest_soup1m_1080p_pmc.json"
+
+
+def traversal_source_hash() -> str:
+    h = hashlib.sha256()
+    for rel in TRAVERSAL_SOURCES:
+        h.update(rel.encode())
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def _pmc_traffic():
+    """HBM bytes per closest-hit launch from the committed counter summary, if it was
+    measured on the current traversal sources; else (None, why)."""
+    if not PMC_FILE.exists():
+        return None, "no counter summary"
+    pmc = json.loads(PMC_FILE.read_text())
+    now = traversal_source_hash()
+    if pmc.get("source_hash") != now:
+        return None, (f"stale: {PMC_FILE.name} was measured on traversal sources {pmc.get('source_hash')}, "
+                      f"the current ones hash to {now}")
+    return pmc.get("hbm_bytes_per_launch"), f"{PMC_FILE.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, source {now})"
 
 
 def _mse(img, ref) -> float:
@@ -229,7 +301,7 @@ def main(argv=None) -> int:
     ap.add_argument("--max-leaf", type=int, default=0, help="BVH leaf size limit (0 = library default)")
     ap.add_argument("--sort-rays", action="store_true", help="reorder secondary rays before tracing")
     ap.add_argument("--sort-materials", action="store_true", help="shade each bounce's hits grouped by material")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work of the 3 timed baseline runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
     ap.add_argument("--no-nrc", action="store_true", help="skip the neural radiance cache equal-time MSE")
@@ -282,10 +354,7 @@ def main(argv=None) -> int:
     launches = s1["extend_launches"] - s0["extend_launches"]
     gpu_ms = s1["ms_render"] - s0["ms_render"]
     achieved = BYTES_PER_RAY * ext / (ms_ext * 1e-3) / 1e9 if ms_ext > 0 else 0.0
-    traffic = None
-    pmc = ROOT / "profiles" / "trace_closest_soup1m_1080p_pmc.json"
-    if pmc.exists():
-        traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+    traffic, traffic_note = _pmc_traffic()
     out = {
         "metric": METRIC,
         "value": rays / wall / 1e6,
@@ -318,6 +387,7 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": "k_trace (closest hit)",
             "bytes_per_launch": BYTES_PER_RAY * ext / max(launches, 1),
             "mean_launch_ms": ms_ext / max(launches, 1),
@@ -339,8 +409,7 @@ def main(argv=None) -> int:
     if not args.no_batch:
         out["batch"] = batch_bench()
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(spec, res, bounces, args.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_baseline(spec, res, bounces, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     else:
         out["cpu_baseline"] = None

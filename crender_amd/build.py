@@ -40,9 +40,17 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
+# Per-file flags.  The integrator's shading, camera and sampling code is compiled without
+# FP contraction so that every product and sum rounds as written, like the CPU restatement
+# it is checked against (the traversal keeps its explicit fmaf node test).
+PER_FILE = {
+    "integrator.hip": ["-ffp-contract=off"],
+}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
-    _run([HIPCC, *COMMON, "-c", str(src), "-o", str(obj)])
+    _run([HIPCC, *COMMON, *PER_FILE.get(src.name, []), "-c", str(src), "-o", str(obj)])
     return obj
 
 
@@ -65,14 +73,23 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     return LIB
 
 
+REFERENCE = Path("/root/reference")
+
+
 def build_oracle(verbose: bool = True) -> Path:
-    """Build the CPU oracle (test infrastructure) with its Makefile."""
+    """Build the CPU oracle (test infrastructure) with its Makefile, and, where the reference
+    tree is present, oracle/_ref/ from the reference's own sources (its vendored
+    tinyobjloader)."""
     odir = ROOT / "oracle"
     make = shutil.which("make") or "make"
     _run([make, "-C", str(odir), "-s"])
     lib = odir / "liboracle.so"
     if verbose:
         print(f"built {lib}")
+    if (REFERENCE / "external" / "tinyobj" / "tinobj.h").exists():
+        _run([make, "-C", str(odir), "-s", "ref", f"REFERENCE={REFERENCE}"])
+        if verbose:
+            print(f"built {odir / '_ref' / 'tinyobj_dump'} from the reference's tinobj.h")
     return lib
 
 

@@ -197,6 +197,9 @@ struct crt_renderer {
     double                 ms_nrc_train = 0.0;
     uint32_t               loss_tiles = 0;   // tiles of the last training step (loss partials)
     uint32_t               nrc_warm_done = 0;   // warm-up passes run since set_nrc
+    // A failed pass may have added samples to the accumulation without counting them (or the
+    // reverse): passes are refused until reset_accum (start / reset / resize) clears it.
+    bool                   needs_reset = false;
 };
 
 namespace {
@@ -281,6 +284,7 @@ void free_scene_device(crt_scene *s)
     s->d_texels = nullptr;
     s->d_texs = nullptr;
     s->d_model_first = nullptr;
+    s->dev       = SceneDev{};   // no stale device pointers survive the free
     s->committed = false;
 }
 
@@ -457,9 +461,13 @@ camera_dev to_camera_dev(const crt_camera &c)
     return d;
 }
 
-TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *count, uint32_t *work, uint32_t *err)
+// unit_dir: renderer rays are traced along their normalised direction (cast_ray), queries
+// of the ray API along the direction as given (rtcIntersect1).
+TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *count, uint32_t *work, uint32_t *err,
+                      bool unit_dir = true)
 {
     TraceQueue q{};
+    q.unit_dir = unit_dir ? 1u : 0u;
     q.rays     = rays;
     q.hits     = hits;
     q.count    = count;
@@ -604,6 +612,7 @@ NrcParams nrc_params(const crt_renderer *r)
 
 int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count);
 int nrc_wait_training(crt_nrc *n);
+int nrc_wait_locked(crt_nrc *n);   // nrc_wait_training with n->mu held by the caller
 
 // Enqueues the cache training of a finished wavefront (on the cache's stream).
 int train_after(crt_renderer *r, const PassTicket &t)
@@ -658,7 +667,9 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries, with the weights of
         // every training step enqueued so far
-        if (r->nrc->pending) HIP_CHECK(hipStreamWaitEvent(s, r->nrc->ev_trained, 0));
+        // unconditional: a wait on an event that was never recorded is a no-op, and the
+        // pending flag may have been cleared by a concurrent nrc_wait_training
+        HIP_CHECK(hipStreamWaitEvent(s, r->nrc->ev_trained, 0));
         HIP_CHECK(nrc_launch_infer_dev(b.nq, b.nq_out, b.cnt + kCntNrcQ, b.P, r->nrc->WT, s));
         HIP_CHECK(launch_nrc_apply(b, s));
         if (t.timing) {
@@ -733,6 +744,8 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     // the context's stream and spill area are shared with queries, batches and the autotuner
     std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     done = 0;
+    if (r->needs_reset)
+        return fail(CRT_ERR_STATE, "a previous pass failed; reset (or start) the renderer before rendering again");
     const bool overlap = r->nrc == nullptr;
     PassTicket inflight, trained;
     bool       busy   = false;
@@ -763,7 +776,10 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     }
     if (due && st == CRT_OK) st = train_after(r, trained);
     if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) done += inflight.units;
-    if (st != CRT_OK) (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
+    if (st != CRT_OK) {
+        (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
+        r->needs_reset = true;                     // accumulation and sample count may disagree
+    }
     if (r->nrc) {
         const int sw = nrc_wait_training(r->nrc);
         if (st == CRT_OK) st = sw;
@@ -806,13 +822,22 @@ int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count)
 
 // Waits for the cache training enqueued by renderers (the cache API and the end of a group
 // of wavefronts see finished weights).
-int nrc_wait_training(crt_nrc *n)
+int nrc_wait_locked(crt_nrc *n)
 {
-    if (n && n->pending) {
+    if (n->pending) {
         HIP_CHECK(hipStreamSynchronize(n->ts));
         n->pending = false;
     }
     return CRT_OK;
+}
+
+int nrc_wait_training(crt_nrc *n)
+{
+    if (!n) return CRT_OK;
+    // under the cache's lock, which nrc_train_step holds while it enqueues and sets the flag:
+    // a step enqueued after the check is either waited for here or still flagged
+    std::lock_guard<std::mutex> g(n->mu);
+    return nrc_wait_locked(n);
 }
 
 int check_ready(crt_renderer *r)
@@ -861,6 +886,7 @@ int reset_accum(crt_renderer *r)
     HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
     HIP_CHECK(hipStreamSynchronize(s));
     r->samples.store(0);
+    r->needs_reset = false;
     for (double &m : r->ms_phase) m = 0.0;
     r->extend_launches = 0;
     r->ms_nrc_train    = 0.0;
@@ -936,7 +962,7 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
         h.v    = dev && hits->v ? hits->v : ctx->st_v;
         h.tri  = ctx->st_tri;
         h.inst = ctx->st_inst;
-        TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
+        TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16, false);
         HIP_CHECK(launch_trace_closest(scene->dev, q, ctx->spill, trace_cfg(ctx), s));
         uint32_t *prim = dev ? hits->prim_id : ctx->st_prim;
         uint32_t *geom = dev ? hits->geom_id : (hits->geom_id ? ctx->st_geom : nullptr);
@@ -958,7 +984,7 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
                 if (ng[k]) HIP_CHECK(hipMemcpyAsync(ng[k], ctx->st_ng[k], sizeof(float) * n, hipMemcpyDeviceToHost, s));
         }
     } else {
-        TraceQueue q = make_queue(rs, HitSoA{}, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16);
+        TraceQueue q = make_queue(rs, HitSoA{}, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16, false);
         uint8_t   *occ = memory == CRT_MEM_DEVICE ? occluded : ctx->st_occ;
         HIP_CHECK(launch_trace_any(scene->dev, q, occ, ctx->spill, trace_cfg(ctx), s));
         if (memory == CRT_MEM_HOST) HIP_CHECK(hipMemcpyAsync(occluded, ctx->st_occ, n, hipMemcpyDeviceToHost, s));
@@ -1451,6 +1477,10 @@ int crt_scene_commit(crt_scene *scene)
     if (!scene) return fail(CRT_ERR_INVALID_ARG, "null scene");
     crt_context *ctx = scene->ctx;
     HIP_CHECK(hipSetDevice(ctx->device));
+    // Renderer passes (also those of a background renderer of this scene), batches and
+    // queries read the scene's device buffers under the context lock: the old buffers are
+    // freed and the new ones built while none of them runs.
+    std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     free_scene_device(scene);
     // concatenate models (scene.cpp:83-95 loops over them; one BVH over all here)
     std::vector<float>       corners, uv;
@@ -2233,7 +2263,7 @@ static int nrc_fwd_bwd(crt_nrc *n, const float *queries, const float *targets, u
                        float *loss)
 {
     if (count > n->opt.max_batch) return fail(CRT_ERR_INVALID_ARG, "batch larger than max_batch");
-    if (int wst = nrc_wait_training(n)) return wst;   // training enqueued by renderers
+    if (int wst = nrc_wait_locked(n)) return wst;   // training enqueued by renderers (n->mu is held)
     int st = memory == CRT_MEM_HOST ? nrc_stage(n, count) : CRT_OK;
     if (st) return st;
     const float *q = nrc_input(n, queries, (size_t)count * kNrcQuery, n->q_stage, memory, &st);

@@ -166,6 +166,11 @@ struct TraceScene;
 struct TraceQueue {
     RaySoA          rays;
     HitSoA          hits;
+    // 1: trace normalize(direction), as cr::scene::cast_ray does for every ray of the render
+    // loop (model.cpp:109-110: glm::normalize(inv * dir) with the identity instance), so hit
+    // distances are in units of the unit direction; 0: the direction as given (rtcIntersect1
+    // semantics of crt_intersect / crt_occluded).
+    uint32_t        unit_dir;
     const uint32_t *count;     // device: number of rays
     uint32_t       *counters;  // device: work counters (kXcds entries), zeroed by the caller's kernel
     uint32_t       *error;     // device: set to 1 if a traversal stack would overflow

@@ -10,6 +10,11 @@
 #include <stdint.h>
 
 #define CRT_HD __host__ __device__ __forceinline__
+// Inside a function body: that body's products and sums round as written, whatever the
+// translation unit's contraction setting (a fused multiply-add needs both operations to
+// allow it, so no expression built from these helpers is fused either).  The estimator's
+// arithmetic must round like the CPU restatement it is checked against.
+#define CRT_EXACT _Pragma("clang fp contract(off)")
 
 namespace crt {
 
@@ -26,17 +31,35 @@ struct f3 {
 };
 
 CRT_HD f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
-CRT_HD f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
-CRT_HD f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
-CRT_HD f3 operator*(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
-CRT_HD f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+CRT_HD f3 operator+(f3 a, f3 b) { CRT_EXACT return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+CRT_HD f3 operator-(f3 a, f3 b) { CRT_EXACT return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+CRT_HD f3 operator*(f3 a, f3 b) { CRT_EXACT return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+CRT_HD f3 operator*(f3 a, float s) { CRT_EXACT return mk3(a.x * s, a.y * s, a.z * s); }
 CRT_HD f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
-CRT_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-CRT_HD f3 cross(f3 a, f3 b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+CRT_HD float dot(f3 a, f3 b) { CRT_EXACT return a.x * b.x + a.y * b.y + a.z * b.z; }
+CRT_HD f3 cross(f3 a, f3 b)
+{
+    CRT_EXACT
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// FP32 transcendental functions of the estimator, evaluated in FP64 and rounded once to
+// FP32.  FP64 sin, cos, tan, atan2, asin, acos and pow are accurate to about one FP64 ulp in
+// any good library, so the rounded result is the correctly rounded FP32 value except when the
+// exact value lies within ~1e-16 relative of an FP32 rounding boundary: the device and a CPU
+// restatement built on a different library consume the same FP32 numbers.  (The reference's
+// glm calls go to the host library's FP32 functions, which are within one ulp of these.)
+CRT_HD float sin_cr(float x) { return (float)sin((double)x); }
+CRT_HD float cos_cr(float x) { return (float)cos((double)x); }
+CRT_HD float tan_cr(float x) { return (float)tan((double)x); }
+CRT_HD float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+CRT_HD float asin_cr(float x) { return (float)asin((double)x); }
+CRT_HD float acos_cr(float x) { return (float)acos((double)x); }
+CRT_HD float pow_cr(float x, float y) { return (float)pow((double)x, (double)y); }
+
 // glm::normalize = v * inversesqrt(dot(v, v)); 1/sqrt kept as two correctly rounded ops.
-CRT_HD f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+CRT_HD f3 normalize(f3 a) { CRT_EXACT return a * (1.0f / sqrtf(dot(a, a))); }
 // glm::reflect(I, N) = I - 2 dot(N, I) N
-CRT_HD f3 reflect(f3 i, f3 n) { return i - n * (2.0f * dot(n, i)); }
+CRT_HD f3 reflect(f3 i, f3 n) { CRT_EXACT return i - n * (2.0f * dot(n, i)); }
 CRT_HD float maxc(f3 a) { return fmaxf(fmaxf(a.x, a.y), a.z); }
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) -------------------------------------------
@@ -81,15 +104,17 @@ CRT_HD rnd4 rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce
 // sampling.h:156-166
 CRT_HD f3 sphere(float u, float v)
 {
+    CRT_EXACT
     const float ct  = 2.0f * u - 1.0f;
     const float st  = sqrtf(1.0f - ct * ct);
     const float phi = kTau * v;
-    return mk3(st * cosf(phi), ct, st * sinf(phi));
+    return mk3(st * cos_cr(phi), ct, st * sin_cr(phi));
 }
 
 // sampling.h:21-33 (Duff et al. 2017 orthonormal basis)
 CRT_HD void onb(f3 n, f3 &t, f3 &b)
 {
+    CRT_EXACT
     const float s  = (n.z < 0.0f) ? -1.0f : 1.0f;
     const float a  = -1.0f / (s + n.z);
     const float bb = n.x * n.y * a;
@@ -100,12 +125,14 @@ CRT_HD void onb(f3 n, f3 &t, f3 &b)
 // ---- GGX (sampling.h:93-140) ------------------------------------------------------------
 CRT_HD float ggx_d(float noh, float a)
 {
+    CRT_EXACT
     const float a2 = a * a;
     const float d  = (noh * a2 - noh) * noh + 1.0f;
     return a2 / (d * d * kPi);
 }
 CRT_HD float ggx_vis(float nov, float nol, float a)
 {
+    CRT_EXACT
     const float a2   = a * a;
     const float ggxv = nol * sqrtf(nov * nov * (1.0f - a2) + a2);
     const float ggxl = nov * sqrtf(nol * nol * (1.0f - a2) + a2);
@@ -113,7 +140,8 @@ CRT_HD float ggx_vis(float nov, float nol, float a)
 }
 CRT_HD f3 schlick(float u, f3 f0)
 {
-    const float f = powf(1.0f - u, 5.0f);
+    CRT_EXACT
+    const float f = pow_cr(1.0f - u, 5.0f);
     return mk3(f0.x + (1.0f - f0.x) * f, f0.y + (1.0f - f0.y) * f, f0.z + (1.0f - f0.z) * f);
 }
 
@@ -128,11 +156,12 @@ struct camera_dev {
 
 CRT_HD void camera_ray(const camera_dev &c, float x, float y, float aspect, f3 &o, f3 &d)
 {
+    CRT_EXACT
     f3 dir;
     if (c.mode == 0) {
         const float u = (2.0f * x - 1.0f) * aspect;
         const float v = 2.0f * y - 1.0f;
-        const float w = 1.0f / tanf(0.5f * (c.fov * 0.01745329251994329576923690768489f));
+        const float w = 1.0f / tan_cr(0.5f * (c.fov * 0.01745329251994329576923690768489f));
         dir = mk3(c.m[0] * u + c.m[4] * v + c.m[8] * w, c.m[1] * u + c.m[5] * v + c.m[9] * w,
                   c.m[2] * u + c.m[6] * v + c.m[10] * w);
         o = mk3(c.pos[0], c.pos[1], c.pos[2]);

@@ -62,16 +62,20 @@ __device__ __forceinline__ void block_add_stat(unsigned long long *stats, uint32
     __syncthreads();
 }
 
-// image.h:104-109 nearest texel with wrap-around
+// image.h:104-109: nearest texel, static_cast<uint64_t>(u * width) % width.  The cast of a
+// negative product is undefined in C++; this follows what x86-64 code without AVX-512 does
+// (the signed truncation reinterpreted as uint64), so u * width in (-1, 0] maps to column 0
+// and -1.5 to (2^64 - 1) % width.  For power-of-two sizes that is the positive modulo.
+__device__ __forceinline__ uint64_t texel_index(float u, uint32_t size)
+{
+    return (uint64_t)(int64_t)(u * (float)size) % (uint64_t)size;
+}
+
 __device__ __forceinline__ float4 tex_uv(const SceneDev &sc, int32_t t, float u, float v)
 {
     const TextureDev td = sc.texs[t];
-    const int64_t    w = td.w, h = td.h;
-    int64_t          x = (int64_t)(u * (float)td.w);
-    int64_t          y = (int64_t)(v * (float)td.h);
-    x = ((x % w) + w) % w;
-    y = ((y % h) + h) % h;
-    return sc.texels[td.offset + (uint64_t)(x + y * w)];
+    const uint64_t   x = texel_index(u, td.w), y = texel_index(v, td.h);
+    return sc.texels[td.offset + x + y * (uint64_t)td.w];
 }
 
 // renderer.cpp:277-289, scene.cpp:67-77
@@ -79,8 +83,8 @@ __device__ __forceinline__ f3 sky(const SceneDev &sc, f3 d)
 {
     if (sc.sky_tex < 0) return mk3(0.0f, 0.0f, 0.0f);
     const float dy = fminf(fmaxf(d.y, -1.0f), 1.0f);
-    const float u  = 0.5f + atan2f(d.z, d.x) * kInvTau;
-    const float v  = 0.5f - asinf(dy) * kInvPi;
+    const float u  = 0.5f + atan2_cr(d.z, d.x) * kInvTau;
+    const float v  = 0.5f - asin_cr(dy) * kInvPi;
     const float4 c = tex_uv(sc, sc.sky_tex, u + sc.sky_rot[0], v + sc.sky_rot[1]);
     return mk3(c.x, c.y, c.z);
 }
@@ -95,7 +99,7 @@ __device__ __forceinline__ float4 mat_colour(const SceneDev &sc, const MaterialD
 __device__ __forceinline__ f3 sun_radiance(const SunDev &s, f3 dir)
 {
     const float c   = fminf(fmaxf(dot(dir, -s.dir), -1.0f), 1.0f);
-    const float ang = acosf(c);
+    const float ang = acos_cr(c);
     return ang < s.size ? s.colour * s.intensity : mk3(0.0f, 0.0f, 0.0f);
 }
 
@@ -401,7 +405,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
                         const float cf  = ni <= 1.0f ? -dt : sqrtf(disc);
                         const float r0  = (1.0f - m.ior) / (1.0f + m.ior);
                         const float f0s = r0 * r0;
-                        const float fr  = f0s + (1.0f - f0s) * powf(1.0f - cf, 5.0f);
+                        const float fr  = f0s + (1.0f - f0s) * pow_cr(1.0f - cf, 5.0f);
                         refl = r2.b < fr;
                     }
                     if (refl) {
@@ -434,8 +438,8 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
                         const f3    t2v   = cross(vh, t1v);
                         const float rr    = sqrtf(r.a);
                         const float phi   = kTau * r.b;
-                        const float t1    = rr * cosf(phi);
-                        float       t2    = rr * sinf(phi);
+                        const float t1    = rr * cos_cr(phi);
+                        float       t2    = rr * sin_cr(phi);
                         const float s     = 0.5f * (1.0f + vh.z);
                         t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
                         const float nz2 = fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2);
@@ -478,12 +482,12 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const PathBuffers
                 const f3 nd_sun = -sun.dir;
                 f3 tb, bb;
                 onb(nd_sun, tb, bb);
-                const float ct  = 1.0f - r.d * (1.0f - cosf(sun.size));
+                const float ct  = 1.0f - r.d * (1.0f - cos_cr(sun.size));
                 const float st  = sqrtf(1.0f - ct * ct);
                 const float phi = kTau * r.c;
-                const f3    loc = mk3(cosf(phi) * st, ct, sinf(phi) * st);
+                const f3    loc = mk3(cos_cr(phi) * st, ct, sin_cr(phi) * st);
                 sd = tb * loc.x + nd_sun * loc.y + bb * loc.z;
-                const float pdf  = 1.0f / (kTau * (1.0f - cosf(sun.size)));
+                const float pdf  = 1.0f / (kTau * (1.0f - cos_cr(sun.size)));
                 const float cosv = fminf(fmaxf(dot(nee_n, sd), 0.0f), 1.0f);
                 if (cosv > 0.0f) {
                     const f3 L = sun_radiance(sun, sd);
@@ -718,7 +722,7 @@ __global__ __launch_bounds__(256) void k_display(const float *raw, uint64_t n, u
     }
     const float inv = (float)n;
     float c[3];
-    for (int k = 0; k < 3; ++k) c[k] = powf(fminf(fmaxf(raw[3 * (size_t)i + k] / inv, 0.0f), 1.0f), 1.0f / 2.2f);
+    for (int k = 0; k < 3; ++k) c[k] = pow_cr(fminf(fmaxf(raw[3 * (size_t)i + k] / inv, 0.0f), 1.0f), 1.0f / 2.2f);
     out[i] = make_float4(c[0], c[1], c[2], 1.0f);
 }
 

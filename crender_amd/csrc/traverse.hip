@@ -409,6 +409,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                 rid = base + rank;
                 r.o   = mk3(q.rays.ox[rid], q.rays.oy[rid], q.rays.oz[rid]);
                 r.d   = mk3(q.rays.dx[rid], q.rays.dy[rid], q.rays.dz[rid]);
+                if (q.unit_dir && (r.d.x != 0.0f || r.d.y != 0.0f || r.d.z != 0.0f)) r.d = normalize(r.d);
                 r.tn  = q.rays.tn[rid];
                 r.tmax = q.rays.tf[rid];
                 r.inv = mk3(safe_rcp(r.d.x), safe_rcp(r.d.y), safe_rcp(r.d.z));

@@ -346,6 +346,9 @@ def scene_from_spec(ctx: Context, spec, commit: bool = True) -> Scene:
     sc = Scene(ctx)
     sc.add_model(spec)
     sc.set_skybox(spec.sky)
+    rot = getattr(spec, "sky_rotation", (0.0, 0.0))
+    if rot[0] or rot[1]:
+        sc.set_skybox_rotation(float(rot[0]), float(rot[1]))
     sc.set_sun_enabled(spec.sun_enabled)
     if spec.sun:
         sc.set_sun(**spec.sun)

@@ -48,6 +48,7 @@ class SceneSpec:
     texture_indices: np.ndarray | None = None  # (3T,) uint32
     textures: list = field(default_factory=list)  # list of (h, w, 4) float32
     sky: np.ndarray | None = None        # (h, w, 4) float32 or None (black)
+    sky_rotation: tuple = (0.0, 0.0)     # added to the equirect (u, v) (scene.cpp:62-77)
     sun_enabled: bool = False
     sun: dict | None = None
     camera: dict = field(default_factory=dict)
@@ -266,6 +267,63 @@ def sunlit(seed: int = 4, mode: int = 0) -> SceneSpec:
                      res=(96, 64), spp=4, max_bounces=6, rr_start=3, mode=mode)
 
 
+def cutout(seed: int = 5, mode: int = 0) -> SceneSpec:
+    """Textures, alpha cut-outs and a rotated equirect sky under the sun: a ground quad with a
+    5x3 checker texture tiled over uv in [-1.3, 2.7] (negative coordinates wrap, image.h:
+    104-109), a fence quad whose 4x4 texture has transparent columns (colour.w == 0, so camera
+    rays and sun shadow rays pass through, renderer.cpp:294-301,330-345), a fully transparent
+    quad in front of part of the view, diffuse / glass / metal boxes, and a 16x8 sky image
+    with a rotation that takes the lookup below 0 (scene.cpp:62-77)."""
+    st = Stream(seed, stream=7)
+    ang = st.uniform(3, dims=1)[:, 0] * 30.0
+    tris, mats, uvs = [], [], []
+
+    def add(ts, m, uv=None):
+        tris.extend(ts)
+        mats.extend([m] * len(ts))
+        uvs.extend(uv if uv is not None else [((0, 0), (0, 0), (0, 0))] * len(ts))
+
+    g = [(-4, 0, -4), (-4, 0, 4), (4, 0, 4), (4, 0, -4)]
+    gu = [(-1.3, -0.7), (-1.3, 2.7), (2.7, 2.7), (2.7, -0.7)]
+    add(_quad(*g), 0, [(gu[0], gu[1], gu[2]), (gu[0], gu[2], gu[3])])
+    f = [(-1.5, 0, 1.6), (-1.5, 1.4, 1.6), (1.5, 1.4, 1.6), (1.5, 0, 1.6)]       # fence, facing -z
+    fu = [(0, 0), (0, 1), (2, 1), (2, 0)]
+    add(_quad(*f), 1, [(fu[0], fu[1], fu[2]), (fu[0], fu[2], fu[3])])
+    add(_quad((-0.9, 0.2, -1.5), (-0.9, 1.0, -1.5), (-0.1, 1.0, -1.5), (-0.1, 0.2, -1.5)), 2)   # clear pane
+    add(_box((0.6, 0.6, 0.6), float(ang[0]), (-1.0, 0.0, 0.4)), 3)
+    add(_box((0.6, 0.9, 0.6), float(ang[1]), (0.2, 0.0, 0.7)), 4)
+    add(_box((0.5, 0.5, 0.5), float(ang[2]), (1.1, 0.0, 0.1)), 5)
+    verts, idx, mi = _from_tris(tris, mats)
+    uv = np.asarray([c for t in uvs for c in t], dtype=np.float32)
+    checker = np.zeros((3, 5, 4), np.float32)
+    for y in range(3):
+        for x in range(5):
+            v = 0.25 + 0.5 * ((x + y) % 2)
+            checker[y, x] = (v, 0.6 * v + 0.1 * x, 0.3 + 0.1 * y, 1.0)
+    fence = np.zeros((4, 4, 4), np.float32)
+    for y in range(4):
+        for x in range(4):
+            fence[y, x] = (0.7, 0.5, 0.3, 0.0 if x % 2 else 1.0)
+    sky = np.zeros((8, 16, 4), np.float32)
+    for y in range(8):
+        for x in range(16):
+            sky[y, x] = (0.2 + 0.05 * x, 0.3 + 0.08 * y, 0.6 - 0.05 * y + 0.2 * ((x + y) % 2), 1.0)
+    materials = [
+        MaterialSpec(colour=(1.0, 1.0, 1.0, 1.0), tex=0),
+        MaterialSpec(colour=(1.0, 1.0, 1.0, 1.0), tex=1),
+        MaterialSpec(colour=(0.2, 0.9, 0.2, 0.0)),
+        MaterialSpec(colour=(0.8, 0.3, 0.2, 1.0)),
+        MaterialSpec(type=GLASS, ior=1.5, colour=(0.95, 0.95, 1.0, 1.0)),
+        MaterialSpec(type=METAL, roughness=0.3, reflectiveness=0.9, colour=(0.9, 0.8, 0.5, 1.0)),
+    ]
+    cam = camera_dict((0.0, 1.4, -3.5), rotation=(0.0, -14.0, 0.0), fov=55.0)
+    return SceneSpec(name=f"cutout-{seed}", vertices=verts, vertex_indices=idx, material_indices=mi,
+                     materials=materials, texture_coords=uv, texture_indices=idx.copy(),
+                     textures=[checker, fence], sky=sky, sky_rotation=(0.13, -0.21), sun_enabled=True,
+                     sun={"direction": (0.3, -0.6, -0.75), "colour": (1.0, 0.95, 0.85), "intensity": 60.0},
+                     camera=cam, res=(96, 64), spp=4, max_bounces=6, rr_start=3, mode=mode)
+
+
 def _value_noise(x, z, freq, st: Stream, octave: int):
     """Value noise on a (freq+1)^2 lattice of Philox uniforms in [-1, 1], smoothstep-blended."""
     n = freq + 1
@@ -395,17 +453,28 @@ def batch_suite(n: int = 64, seed: int = 7):
     soup and terrain (with icospheres), triangle counts log-uniform in [1k, 500k],
     resolutions uniform in {128^2, 256^2, 512^2, 1024^2}, spp uniform in {1, 4, 16, 64}.
     Returns a list of (SceneSpec, width, height, spp)."""
+    return [(batch_job_scene(p), p["res"], p["res"], p["spp"]) for p in batch_suite_params(n, seed)]
+
+
+def batch_suite_params(n: int = 64, seed: int = 7):
+    """The draws of :func:`batch_suite` without building the scenes: a list of dicts with
+    kind ('soup' / 'terrain'), tris, res, spp and the scene seed."""
     u = Stream(seed, stream=20).uniform(n, dims=4).astype(np.float64)
     out = []
     for k in range(n):
         tris = int(np.exp(np.log(1e3) + u[k, 0] * (np.log(5e5) - np.log(1e3))))
         res = (128, 256, 512, 1024)[min(3, int(u[k, 1] * 4))]
         spp = (1, 4, 16, 64)[min(3, int(u[k, 2] * 4))]
-        if k % 2 == 0:
-            spec = soup(tris, seed=seed * 1000 + k, res=(res, res))
-        else:
-            spheres = max(1, tris // 3200)                  # ~10% of the triangles in spheres
-            grid = max(2, int(np.sqrt(max(tris - 320 * spheres, 2) / 2)) + 1)
-            spec = terrain(grid=grid, spheres=spheres, seed=seed * 1000 + k, res=(res, res))
-        out.append((spec, res, res, spp))
+        out.append({"index": k, "kind": "soup" if k % 2 == 0 else "terrain", "tris": tris, "res": res, "spp": spp,
+                    "seed": seed * 1000 + k})
     return out
+
+
+def batch_job_scene(p) -> SceneSpec:
+    """Scene of one batch job drawn by :func:`batch_suite_params`."""
+    res = (p["res"], p["res"])
+    if p["kind"] == "soup":
+        return soup(p["tris"], seed=p["seed"], res=res)
+    spheres = max(1, p["tris"] // 3200)                  # ~10% of the triangles in spheres
+    grid = max(2, int(np.sqrt(max(p["tris"] - 320 * spheres, 2) / 2)) + 1)
+    return terrain(grid=grid, spheres=spheres, seed=p["seed"], res=res)

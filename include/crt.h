@@ -248,7 +248,10 @@ int crt_renderer_set_max_bounces(crt_renderer *r, uint32_t bounces);         /* 
 int crt_renderer_set_target_spp(crt_renderer *r, uint64_t target);           /* renderer.cpp:215-218 */
 int crt_renderer_set_options(crt_renderer *r, const crt_render_options *opt);
 int crt_renderer_get_options(crt_renderer *r, crt_render_options *opt);
-/* Synchronous passes on the caller's thread (the renderer must be paused). */
+/* Synchronous passes on the caller's thread (the renderer must be paused).  After a pass
+ * fails (e.g. CRT_ERR_HIP "traversal stack overflow"), the accumulation may hold samples that
+ * were not counted; further passes return CRT_ERR_STATE until crt_renderer_reset,
+ * crt_renderer_start or a resolution change clears the accumulation. */
 int crt_renderer_render(crt_renderer *r, uint32_t passes);
 /* Clears accumulation, sample count and stats without starting (start() minus resume). */
 int crt_renderer_reset(crt_renderer *r);

@@ -547,6 +547,74 @@ static int32_t bvh_closest32(const orc_bvh *bvh, const float o[3], const float d
     return best;
 }
 
+/* The same walk with FP64 closest-hit decisions: FP64 watertight test per triangle (the
+ * brute-force ground truth's test) over the FP32 BVH, whose boxes are culled against the FP32
+ * rounding of the current FP64 distance widened by one part in 10^5.  For classifying
+ * precision-sensitive hits on scenes too large for FP64 brute force. */
+static int32_t bvh_closest64(const orc_bvh *bvh, const float o[3], const float d[3], float tnear,
+                             float tfar, double *t, double *u, double *v)
+{
+    if (bvh->ntri == 0) return -1;
+    const double od[3] = {o[0], o[1], o[2]};
+    const double dd[3] = {d[0], d[1], d[2]};
+    wt_ray64 r;
+    wt_setup64(dd, &r);
+    const float inv[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    uint32_t    stack[128];
+    int         sp = 0;
+    int32_t     best = -1;
+    double      bt = tfar, bu = 0.0, bv = 0.0;
+    float       te;
+    if (!box_hit(&bvh->nodes[0], o, inv, tfar, &te)) return -1;
+    uint32_t ni = 0;
+    for (;;) {
+        const bnode *nd = &bvh->nodes[ni];
+        const float  cull = best < 0 ? tfar : (float)(bt * (1.0 + 1e-5)) + 1e-30f;
+        if (nd->count) {
+            for (uint32_t i = nd->first; i < nd->first + nd->count; ++i) {
+                const float *p = bvh->tri + 9 * (size_t)i;
+                double       tt, uu, vv;
+                if (wt_test64(&r, p, p + 3, p + 6, od, tnear, bt, &tt, &uu, &vv)) {
+                    const int32_t pid = (int32_t)bvh->prims[i];
+                    if (best < 0 || tt < bt || pid < best) {
+                        best = pid;
+                        bt   = tt;
+                        bu   = uu;
+                        bv   = vv;
+                    }
+                }
+            }
+        } else {
+            float     t0, t1;
+            const int h0 = box_hit(&bvh->nodes[nd->first], o, inv, cull, &t0);
+            const int h1 = box_hit(&bvh->nodes[nd->first + 1], o, inv, cull, &t1);
+            if (h0 && h1) {
+                stack[sp++] = t0 <= t1 ? nd->first + 1 : nd->first;
+                ni          = t0 <= t1 ? nd->first : nd->first + 1;
+                continue;
+            }
+            if (h0 || h1) {
+                ni = h0 ? nd->first : nd->first + 1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        ni = stack[--sp];
+    }
+    if (best >= 0) {
+        *t = bt;
+        *u = bu;
+        *v = bv;
+    }
+    return best;
+}
+
+int32_t orc_bvh_closest64(const orc_bvh *bvh, const float o[3], const float d[3], float tnear, float tfar,
+                          double *t, double *u, double *v)
+{
+    return bvh_closest64(bvh, o, d, tnear, tfar, t, u, v);
+}
+
 /* Public wrapper; wt_test32 accepts t <= bt, so an equal-t triangle reaches the primID
  * comparison above and only a smaller primID replaces the current hit. */
 int32_t orc_bvh_closest32(const orc_bvh *bvh, const float o[3], const float d[3], float tnear,
@@ -581,9 +649,12 @@ static void *batch_worker(void *arg)
             const float o[3] = {job->ox[i], job->oy[i], job->oz[i]};
             const float d[3] = {job->dx[i], job->dy[i], job->dz[i]};
             int32_t     p;
-            if (job->trav == ORC_TRAV_BRUTE64) {
+            if (job->trav == ORC_TRAV_BRUTE64 || job->trav == ORC_TRAV_BVH64) {
                 double tt = 0, uu = 0, vv = 0;
-                p = orc_closest_brute64(job->tri, job->ntri, o, d, job->tn[i], job->tf[i], &tt, &uu, &vv);
+                if (job->trav == ORC_TRAV_BVH64)
+                    p = bvh_closest64(job->bvh, o, d, job->tn[i], job->tf[i], &tt, &uu, &vv);
+                else
+                    p = orc_closest_brute64(job->tri, job->ntri, o, d, job->tn[i], job->tf[i], &tt, &uu, &vv);
                 job->t[i] = tt;
                 job->u[i] = uu;
                 job->v[i] = vv;

@@ -23,31 +23,46 @@ void orc_rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, u
 }
 
 /* image.h:104-109: static_cast<uint64_t>(u * width) % width.  The cast truncates toward
- * zero; negative products (undefined in the original) wrap modulo the size here. */
+ * zero.  For a negative product it is undefined in C++; x86-64 code without AVX-512
+ * (cvttss2si on the signed range) yields the signed truncation reinterpreted as uint64,
+ * which is what is restated here: (-1, 0] -> 0, -1.5 -> (2^64 - 1) % width. */
+uint64_t orc_texel_index(float u, uint32_t size)
+{
+    return (uint64_t)(int64_t)(u * (float)size) % (uint64_t)size;
+}
+
 void orc_get_uv(const orc_image *img, float u, float v, float out[4])
 {
-    const int64_t w = img->w, h = img->h;
-    int64_t       x = (int64_t)(u * (float)img->w);
-    int64_t       y = (int64_t)(v * (float)img->h);
-    x = ((x % w) + w) % w;
-    y = ((y % h) + h) % h;
-    const float *p = img->rgba + (size_t)(x + y * w) * 4;
+    const uint64_t x = orc_texel_index(u, img->w), y = orc_texel_index(v, img->h);
+    const float   *p = img->rgba + (size_t)(x + y * (uint64_t)img->w) * 4;
     out[0] = p[0];
     out[1] = p[1];
     out[2] = p[2];
     out[3] = p[3];
 }
 
+/* FP32 instantiation.  Transcendental functions are the FP64 ones rounded once to FP32 (the
+ * correctly rounded FP32 value but for rare double-rounding cases), so the result does not
+ * depend on which library's FP32 functions are linked; glibc's sinf and friends agree with
+ * these to within one ulp. */
+static inline float sin_cr(float x) { return (float)sin((double)x); }
+static inline float cos_cr(float x) { return (float)cos((double)x); }
+static inline float tan_cr(float x) { return (float)tan((double)x); }
+static inline float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+static inline float asin_cr(float x) { return (float)asin((double)x); }
+static inline float acos_cr(float x) { return (float)acos((double)x); }
+static inline float pow_cr(float x, float y) { return (float)pow((double)x, (double)y); }
+
 #define R float
 #define SFX f32
 #define SQRT_ sqrtf
-#define SIN_ sinf
-#define COS_ cosf
-#define TAN_ tanf
-#define ATAN2_ atan2f
-#define ASIN_ asinf
-#define ACOS_ acosf
-#define POW_ powf
+#define SIN_ sin_cr
+#define COS_ cos_cr
+#define TAN_ tan_cr
+#define ATAN2_ atan2_cr
+#define ASIN_ asin_cr
+#define ACOS_ acos_cr
+#define POW_ pow_cr
 #define FABS_ fabsf
 #include "integrator_impl.h"
 #undef R
@@ -120,7 +135,7 @@ static int render_common(const orc_scene *sc, const orc_settings *st, void *raw,
     if (sc->ntri && (!sc->tri || !sc->mat_idx || !sc->mats)) return -2;
     for (uint32_t i = 0; i < sc->ntri; ++i)
         if (sc->mat_idx[i] >= sc->nmat) return -3;
-    if (sc->trav == ORC_TRAV_BVH32 && !sc->bvh) return -4;
+    if ((sc->trav == ORC_TRAV_BVH32 || sc->trav == ORC_TRAV_BVH64) && !sc->bvh) return -4;
     int nthreads = st->nthreads > 0 ? st->nthreads : orc_default_threads();
     if (nthreads > 256) nthreads = 256;
     const uint32_t  row_begin = st->row_begin;
@@ -185,4 +200,45 @@ int orc_ggx_sample(const double wo_local[3], double alpha, double u1, double u2,
     wi_local[2] = ll.z;
     *weight     = g;
     return 1;
+}
+
+/* Exported single steps of the bounce loop for the closed-form checks of the branches. */
+#define SCATTER_API(SUFFIX, RT)                                                                       \
+    void orc_scatter_##SUFFIX(const orc_scene *sc, uint32_t mode, int32_t prim, const float point[3],  \
+                              const float normal[3], float uvx, float uvy, const float in_d[3],        \
+                              const float u[4], const float u2[4], orc_scatter_out *out)               \
+    {                                                                                                  \
+        orc_settings st;                                                                               \
+        memset(&st, 0, sizeof st);                                                                     \
+        st.mode = mode;                                                                                \
+        hit_##SUFFIX h;                                                                                \
+        memset(&h, 0, sizeof h);                                                                       \
+        h.prim   = prim;                                                                               \
+        h.point  = mk_##SUFFIX(point[0], point[1], point[2]);                                          \
+        h.normal = mk_##SUFFIX(normal[0], normal[1], normal[2]);                                       \
+        h.uvx    = uvx;                                                                                \
+        h.uvy    = uvy;                                                                                \
+        scatter_##SUFFIX ph;                                                                           \
+        process_hit_##SUFFIX(sc, &st, &h, mk_##SUFFIX(in_d[0], in_d[1], in_d[2]), u, u2, &ph);         \
+        const RT v[15] = {ph.o.x, ph.o.y, ph.o.z, ph.d.x, ph.d.y, ph.d.z, ph.albedo.x, ph.albedo.y,    \
+                          ph.albedo.z, ph.nee_n.x, ph.nee_n.y, ph.nee_n.z, ph.emission, 0, 0};         \
+        for (int k = 0; k < 3; ++k) {                                                                  \
+            out->o[k]      = (float)v[k];                                                              \
+            out->d[k]      = (float)v[3 + k];                                                          \
+            out->albedo[k] = (float)v[6 + k];                                                          \
+            out->nee_n[k]  = (float)v[9 + k];                                                          \
+        }                                                                                              \
+        for (int k = 0; k < 4; ++k) out->colour[k] = ph.colour[k];                                     \
+        out->emission = (float)v[12];                                                                  \
+        out->flags    = (ph.alpha ? 1 : 0) | (ph.delta ? 2 : 0) | (ph.absorbed ? 4 : 0);               \
+    }
+SCATTER_API(f32, float)
+SCATTER_API(f64, double)
+
+void orc_sky_f32(const orc_scene *sc, const float d[3], float out[3])
+{
+    const vec_f32 s = sky_f32(sc, mk_f32(d[0], d[1], d[2]));
+    out[0] = s.x;
+    out[1] = s.y;
+    out[2] = s.z;
 }

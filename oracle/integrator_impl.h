@@ -108,9 +108,13 @@ static int F_(cast)(const orc_scene *sc, V o, V d_in, F_(hit) * h)
     const V d = F_(norm)(d_in);
     R       t = 0, bu = 0, bv = 0;
     int32_t p;
-    if (sc->trav == ORC_TRAV_BRUTE64) {
+    if (sc->trav == ORC_TRAV_BRUTE64 || sc->trav == ORC_TRAV_BVH64) {
         double tt = 0, uu = 0, vv = 0;
-        if (sizeof(R) == sizeof(double)) {
+        if (sc->trav == ORC_TRAV_BVH64) {
+            const float of[3] = {(float)o.x, (float)o.y, (float)o.z};
+            const float df[3] = {(float)d.x, (float)d.y, (float)d.z};
+            p = orc_bvh_closest64(sc->bvh, of, df, 1e-5f, INFINITY, &tt, &uu, &vv);
+        } else if (sizeof(R) == sizeof(double)) {
             const double od[3] = {(double)o.x, (double)o.y, (double)o.z};
             const double dd[3] = {(double)d.x, (double)d.y, (double)d.z};
             p = orc_brute64d(sc->tri, sc->ntri, od, dd, 1e-5, INFINITY, &tt, &uu, &vv);

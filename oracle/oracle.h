@@ -60,7 +60,8 @@ float orc_solid_angle_pdf(float theta_max);
 /* ---- scene description ------------------------------------------------------------ */
 enum { ORC_METAL = 0, ORC_SMOOTH = 1, ORC_GLASS = 2 };   /* material.h:14-19 */
 enum { ORC_COMPAT = 0, ORC_PBR = 1 };
-enum { ORC_TRAV_BRUTE64 = 0, ORC_TRAV_BRUTE32 = 1, ORC_TRAV_BVH32 = 2 };
+/* BVH64: the FP32 BVH walked with FP64 closest-hit decisions (orc_bvh_closest64) */
+enum { ORC_TRAV_BRUTE64 = 0, ORC_TRAV_BRUTE32 = 1, ORC_TRAV_BVH32 = 2, ORC_TRAV_BVH64 = 3 };
 
 typedef struct orc_material {
     int32_t type;          /* ORC_METAL / ORC_SMOOTH / ORC_GLASS */
@@ -104,7 +105,7 @@ typedef struct orc_scene {
     orc_sun             sun;
     int32_t             sun_enabled;
     orc_camera          cam;
-    const orc_bvh      *bvh;      /* required for ORC_TRAV_BVH32 */
+    const orc_bvh      *bvh;      /* required for ORC_TRAV_BVH32 / ORC_TRAV_BVH64 */
     int32_t             trav;     /* ORC_TRAV_* */
 } orc_scene;
 
@@ -159,6 +160,8 @@ uint32_t orc_bvh_node_count(const orc_bvh *bvh);
 double   orc_bvh_sah_cost(const orc_bvh *bvh);
 int32_t  orc_bvh_closest32(const orc_bvh *bvh, const float o[3], const float d[3], float tnear,
                            float tfar, float *t, float *u, float *v);
+int32_t  orc_bvh_closest64(const orc_bvh *bvh, const float o[3], const float d[3], float tnear,
+                           float tfar, double *t, double *u, double *v);
 int      orc_default_threads(void);
 
 /* ---- integrator ---------------------------------------------------------------------- */
@@ -172,8 +175,31 @@ int orc_render_f64(const orc_scene *scene, const orc_settings *s, double *raw,
 /* One camera ray (camera.cpp:14-39) for jittered pixel coords x,y in [0,1]. */
 void orc_camera_ray(const orc_camera *cam, float x, float y, float aspect, float o[3], float d[3]);
 
-/* Nearest texel with wrap-around, image.h:104-109 (negative coordinates wrap too). */
-void orc_get_uv(const orc_image *img, float u, float v, float out[4]);
+/* Nearest texel with wrap-around, image.h:104-109: column static_cast<uint64_t>(u * w) % w,
+ * negative products taken as x86-64 code without AVX-512 converts them (orc_texel_index). */
+uint64_t orc_texel_index(float u, uint32_t size);
+void     orc_get_uv(const orc_image *img, float u, float v, float out[4]);
+
+/* One material scatter (process_hit, renderer.cpp:21-102, in the given mode) of a hit
+ * record on primitive `prim` of the scene, for the incoming direction in_d and the
+ * uniforms u (block 0) and u2 (block 1) of the bounce.  flags: 1 alpha pass-through,
+ * 2 delta lobe (pbr: no NEE), 4 absorbed (pbr GGX sample under the surface). */
+typedef struct orc_scatter_out {
+    float   o[3], d[3];      /* next ray */
+    float   albedo[3];       /* throughput multiplier */
+    float   colour[4];       /* material / texture colour */
+    float   nee_n[3];        /* normal of the NEE offset and cosine */
+    float   emission;
+    int32_t flags;
+} orc_scatter_out;
+void orc_scatter_f32(const orc_scene *sc, uint32_t mode, int32_t prim, const float point[3],
+                     const float normal[3], float uvx, float uvy, const float in_d[3],
+                     const float u[4], const float u2[4], orc_scatter_out *out);
+void orc_scatter_f64(const orc_scene *sc, uint32_t mode, int32_t prim, const float point[3],
+                     const float normal[3], float uvx, float uvy, const float in_d[3],
+                     const float u[4], const float u2[4], orc_scatter_out *out);
+/* Radiance of a missed ray (renderer.cpp:277-289 -> scene.cpp:67-77), FP32 evaluation. */
+void orc_sky_f32(const orc_scene *sc, const float d[3], float out[3]);
 /* Four uniforms in [0,1) for counter {pixel, sample, bounce, block}, key {seed, ORC_KEY_HI}. */
 void orc_rng4(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t block, float u[4]);
 

@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 LIBPATH = HERE / "liboracle.so"
 
 COMPAT, PBR = 0, 1
-BRUTE64, BRUTE32, BVH32 = 0, 1, 2
+BRUTE64, BRUTE32, BVH32, BVH64 = 0, 1, 2, 3
 
 
 class Material(C.Structure):
@@ -55,15 +55,32 @@ class Stats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class ScatterOut(C.Structure):
+    _fields_ = [("o", C.c_float * 3), ("d", C.c_float * 3), ("albedo", C.c_float * 3), ("colour", C.c_float * 4),
+                ("nee_n", C.c_float * 3), ("emission", C.c_float), ("flags", C.c_int32)]
+
+
+SCATTER_ALPHA, SCATTER_DELTA, SCATTER_ABSORBED = 1, 2, 4
+
 _L = None
+_LIBPATH = LIBPATH
+
+
+def use_library(path) -> None:
+    """Binds this module to another build of the oracle's sources (the CPU baseline's
+    optimised build); must be called before the first use."""
+    global _LIBPATH
+    if _L is not None and Path(path) != _LIBPATH:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIBPATH = Path(path)
 
 
 def lib() -> C.CDLL:
     global _L
     if _L is None:
-        if not LIBPATH.exists():
-            raise ImportError(f"{LIBPATH} missing: build with `make -C oracle`")
-        L = C.CDLL(str(LIBPATH))
+        if not _LIBPATH.exists():
+            raise ImportError(f"{_LIBPATH} missing: build with `make -C oracle`")
+        L = C.CDLL(str(_LIBPATH))
         fp, up, vp = C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_void_p
         L.orc_philox4x32_10.argtypes = [up, up, up]
         L.orc_philox_batch.argtypes = [up, C.c_uint32, up, up]
@@ -97,6 +114,13 @@ def lib() -> C.CDLL:
         L.orc_display.restype = C.c_float
         L.orc_display.argtypes = [C.c_float, C.c_uint64]
         L.orc_default_threads.restype = C.c_int
+        L.orc_texel_index.restype = C.c_uint64
+        L.orc_texel_index.argtypes = [C.c_float, C.c_uint32]
+        L.orc_get_uv.argtypes = [C.POINTER(Image), C.c_float, C.c_float, fp]
+        for name in ("orc_scatter_f32", "orc_scatter_f64"):
+            getattr(L, name).argtypes = [C.POINTER(Scene), C.c_uint32, C.c_int32, fp, fp, C.c_float, C.c_float, fp,
+                                         fp, fp, C.POINTER(ScatterOut)]
+        L.orc_sky_f32.argtypes = [C.POINTER(Scene), fp, fp]
         _L = L
     return _L
 
@@ -162,6 +186,19 @@ def intersect(tri, org, dirs, mode=BRUTE64, bvh: Bvh | None = None, tnear=1e-5, 
     return prim, t, u, v
 
 
+def _default_sun():
+    """entity::sun defaults (components.h:23-29) evaluated in FP32 as the reference does:
+    size = pi_f / 48 and direction = glm::normalize(vec3(0.8, -1, 0)) = v * (1 / sqrt(dot(v, v)))."""
+    f = np.float32
+    v = np.array([0.8, -1.0, 0.0], np.float32)
+    dd = f(f(v[0] * v[0]) + f(v[1] * v[1])) + f(v[2] * v[2])
+    inv = f(1.0) / np.sqrt(dd)
+    return f(3.14159265359) / f(48.0), (v * inv).astype(np.float32)
+
+
+DEFAULT_SUN_SIZE, DEFAULT_SUN_DIR = _default_sun()
+
+
 class SceneBinding:
     """Keeps the numpy arrays of an oracle scene alive."""
 
@@ -197,11 +234,16 @@ class SceneBinding:
         s.ntex = len(self.textures)
         if self.sky is not None:
             s.sky = Image(_f(self.sky), self.sky.shape[1], self.sky.shape[0])
+            rot = getattr(spec, "sky_rotation", (0.0, 0.0))
+            s.sky_rot[0], s.sky_rot[1] = float(rot[0]), float(rot[1])
         sun = spec.sun or {}
-        s.sun.size = sun.get("size", 3.14159265359 / 48.0)
+        s.sun.size = sun.get("size", DEFAULT_SUN_SIZE)
         s.sun.intensity = sun.get("intensity", 100.0)
-        d = np.asarray(sun.get("direction", (0.8, -1.0, 0.0)), dtype=np.float64)
-        d = (d / np.linalg.norm(d)).astype(np.float32)
+        if "direction" in sun:
+            d = np.asarray(sun["direction"], dtype=np.float64)
+            d = (d / np.linalg.norm(d)).astype(np.float32)
+        else:
+            d = DEFAULT_SUN_DIR
         for k in range(3):
             s.sun.dir[k] = d[k]
             s.sun.colour[k] = sun.get("colour", (1.0, 0.9, 0.7))[k]
@@ -299,3 +341,48 @@ def intersect_instanced(models, instances, org, dirs, mode=BRUTE64, nthreads=0):
             bu = np.where(win, u, bu)
             bv = np.where(win, v, bv)
     return bm, bi, bp, best, bu, bv
+
+
+def texel_index(u: float, size: int) -> int:
+    """Texel column of coordinate u in an image `size` texels wide (image.h:104-109)."""
+    return int(lib().orc_texel_index(C.c_float(u), size))
+
+
+def get_uv(img: np.ndarray, u: float, v: float) -> np.ndarray:
+    """image::get_uv of an (h, w, 4) float32 image: the nearest texel with wrap-around."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros(4, np.float32)
+    im = Image(_f(img), img.shape[1], img.shape[0])
+    lib().orc_get_uv(C.byref(im), u, v, _f(out))
+    return out
+
+
+def scatter(spec, prim, point, normal, in_d, u=(0, 0, 0, 0), u2=(0, 0, 0, 0), uv=(0.0, 0.0), mode=None,
+            precision=32) -> dict:
+    """One process_hit step (renderer.cpp:21-102) of the oracle for a hit on triangle `prim`
+    of `spec` at `point` with unit normal `normal`, incoming direction in_d and the bounce's
+    uniforms u (block 0) and u2 (block 1)."""
+    b = SceneBinding(spec)
+    f3 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(3)  # noqa: E731
+    p, n, d = f3(point), f3(normal), f3(in_d)
+    ua = np.ascontiguousarray(u, dtype=np.float32).reshape(4)
+    ub = np.ascontiguousarray(u2, dtype=np.float32).reshape(4)
+    out = ScatterOut()
+    fn = lib().orc_scatter_f64 if precision == 64 else lib().orc_scatter_f32
+    fn(C.byref(b.c), spec.mode if mode is None else mode, prim, _f(p), _f(n), uv[0], uv[1], _f(d), _f(ua), _f(ub),
+       C.byref(out))
+    return {"o": np.array(out.o, np.float32), "d": np.array(out.d, np.float32),
+            "albedo": np.array(out.albedo, np.float32), "colour": np.array(out.colour, np.float32),
+            "nee_n": np.array(out.nee_n, np.float32), "emission": float(out.emission),
+            "alpha": bool(out.flags & SCATTER_ALPHA), "delta": bool(out.flags & SCATTER_DELTA),
+            "absorbed": bool(out.flags & SCATTER_ABSORBED)}
+
+
+def sky(spec, dirs) -> np.ndarray:
+    """Radiance of missed rays with directions `dirs` (n, 3): renderer.cpp:277-289 -> scene.cpp:67-77."""
+    b = SceneBinding(spec)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros_like(dirs)
+    for i in range(dirs.shape[0]):
+        lib().orc_sky_f32(C.byref(b.c), _f(dirs[i]), _f(out[i]))
+    return out

@@ -101,28 +101,39 @@ def test_soup_100k_vs_oracle(oracle, width):
     ctx.close()
 
 
-def test_soup_1m_headline_scale_vs_oracle(oracle):
-    """Hit-ID agreement at the headline workload's size: the 1M-triangle soup, primary rays
-    of its 1920x1080 camera (every 16th pixel) plus diffuse rays from inside the cloud."""
+@pytest.fixture(scope="module")
+def soup_1m(oracle):
+    """The headline workload's 1M-triangle soup, primary rays of its 1920x1080 camera (every
+    16th pixel) plus diffuse rays from inside the cloud, and the oracle's answers."""
     spec = scenes.soup(1_000_000, seed=2)
     tri = spec.corners()
-    ctx = Context(0)
-    sc = scene_from_spec(ctx, spec)
     o1, d1 = scenes.rays_from_camera(spec, (1920, 1080))
     o1, d1 = o1[::16], d1[::16]
     o2, d2 = scenes.diffuse_rays(40_000, seed=12)
     o = np.concatenate([o1, o2])
     d = np.concatenate([d1, d2])
-    got = sc.cast_rays(o, d)
-    gp = _prims(got)
     bvh = oracle.Bvh(tri)
     pb, tb, _, _ = oracle.intersect(tri, o, d, oracle.BVH32, bvh=bvh)
+    sel = np.linspace(0, len(o) - 1, 400).astype(np.int64)
+    p64, t64, _, _ = oracle.intersect(tri, o[sel], d[sel], oracle.BRUTE64)
+    return spec, tri, o, d, pb, tb, sel, p64, t64
+
+
+@pytest.mark.parametrize("stack", [8, 16, 32])
+def test_soup_1m_headline_scale_vs_oracle(soup_1m, stack):
+    """Hit-ID agreement at the headline size for every LDS stack depth the traversal is built
+    with (8 and 16 entries spill to the global stack far more often on this deep BVH; the
+    renderers that train the radiance cache trace with 16)."""
+    spec, tri, o, d, pb, tb, sel, p64, t64 = soup_1m
+    print(f"soup 1M, trace_stack {stack}", flush=True)
+    ctx = Context(0, trace_stack=stack)
+    sc = scene_from_spec(ctx, spec)
+    got = sc.cast_rays(o, d)
+    gp = _prims(got)
     mism = int(np.sum(gp != pb))
     assert mism <= max(1, int(1e-5 * len(gp))), f"{mism} of {len(gp)} differ from the FP32 oracle"
     hit = (pb >= 0) & (gp == pb)
     assert np.max(np.abs(got["t"][hit] - tb[hit]) / np.maximum(tb[hit], 1e-6)) <= 1e-5
-    sel = np.linspace(0, len(o) - 1, 400).astype(np.int64)
-    p64, t64, _, _ = oracle.intersect(tri, o[sel], d[sel], oracle.BRUTE64)
     assert _classify_fp64(tri, o[sel], d[sel], gp[sel], p64, t64) == 0
     sc.close()
     ctx.close()

@@ -7,7 +7,9 @@ defaults appended where the reference would index out of bounds, and a round tri
 seeded Cornell box through OBJ.
 """
 import struct
+import subprocess
 import zlib
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -131,3 +133,83 @@ def test_cornell_round_trip(tmp_path):
     assert m.material_indices.tolist() == list(np.asarray(spec.material_indices))
     for got, want in zip(m.materials, spec.materials):
         assert got["colour"][:3] == pytest.approx(tuple(want.colour[:3]))
+
+
+# ---- against the reference's own tinyobjloader (oracle/_ref, built where the reference is) --
+REF_TINYOBJ = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "tinyobj_dump"
+
+
+def _tinyobj(path):
+    """Triangles (vertex indices, texcoord indices, material) of tinyobjloader's reading."""
+    out = subprocess.run([str(REF_TINYOBJ), str(path)], capture_output=True, text=True, check=True).stdout
+    tris, tcs, mats = [], [], []
+    for line in out.splitlines():
+        w = line.split()
+        assert w[0] != "n", f"tinyobj left a face of {w[1]} vertices untriangulated"
+        if w[0] == "f":
+            tris.append([int(x) for x in w[1:4]])
+            tcs.append([int(x) for x in w[4:7]])
+            mats.append(int(w[7]))
+    return np.array(tris, np.int64).reshape(-1, 3), np.array(tcs, np.int64).reshape(-1, 3), np.array(mats)
+
+
+def _random_polygon(rng, n):
+    """A simple (star-shaped) polygon with n vertices: sorted angles, random radii."""
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rad = rng.uniform(0.3, 1.0, n)
+    return np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+
+
+needs_ref = pytest.mark.skipif(not REF_TINYOBJ.exists(), reason="oracle/_ref/tinyobj_dump not built "
+                               "(the reference tree is absent)")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(6))
+def test_triangulation_matches_reference_tinyobj(tmp_path, seed):
+    """Random star-shaped concave polygons of 4-12 vertices in random planes, quads, relative
+    indices, texcoords and two materials: the loader's triangles and primID order equal
+    tinyobjloader's (tinobj.h:1393-1714) face for face."""
+    rng = np.random.default_rng(seed)
+    lines, nv, nt = ["mtllib m.mtl"], 0, 0
+    for f in range(40):
+        n = int(rng.integers(3, 13))
+        poly = _random_polygon(rng, n)
+        # embed in a random plane (rotation), so the dominant-axis projection varies
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        pts = np.concatenate([poly, rng.normal(scale=1e-3, size=(n, 1)) * 0], 1) @ q.T + rng.normal(size=3)
+        for p in pts:
+            lines.append(f"v {p[0]:.7f} {p[1]:.7f} {p[2]:.7f}")
+        for p in poly:
+            lines.append(f"vt {0.5 + 0.5 * p[0]:.6f} {0.5 + 0.5 * p[1]:.6f}")
+        if f % 7 == 0:
+            lines.append(f"usemtl m{f % 2}")
+        if f % 3 == 0:   # relative indices
+            idx = " ".join(f"{k - n}/{k - n}" for k in range(n))
+        else:
+            idx = " ".join(f"{nv + k + 1}/{nt + k + 1}" for k in range(n))
+        lines.append("f " + idx)
+        nv += n
+        nt += n
+    mtl = "newmtl m0\nKd 0.8 0.2 0.2\n\nnewmtl m1\nKd 0.2 0.8 0.2\n"
+    p = _obj(tmp_path, "\n".join(lines) + "\n", mtl=mtl)
+    ref_v, ref_t, ref_m = _tinyobj(p)
+    m = ObjModel(p)
+    assert m.vertex_indices.shape == ref_v.shape
+    np.testing.assert_array_equal(m.vertex_indices, ref_v)
+    np.testing.assert_array_equal(m.texture_indices, ref_t)
+    # faces before the first usemtl have material -1 in tinyobj; the loader appends a default
+    got_m = np.asarray(m.material_indices, np.int64)
+    named = ref_m >= 0
+    np.testing.assert_array_equal(got_m[named], ref_m[named])
+
+
+@needs_ref
+def test_cornell_obj_matches_reference_tinyobj(tmp_path):
+    spec = scenes.cornell(3)
+    path = tmp_path / "cornell.obj"
+    scenes.write_obj(spec, path)
+    ref_v, _, ref_m = _tinyobj(path)
+    m = ObjModel(path)
+    np.testing.assert_array_equal(m.vertex_indices, ref_v)
+    np.testing.assert_array_equal(np.asarray(m.material_indices, np.int64), ref_m)
