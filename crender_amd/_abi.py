@@ -177,6 +177,7 @@ SIGNATURES = {
     "crt_scene_bvh_info": (C.c_int, [P, C.POINTER(BvhInfo)]),
     "crt_scene_bvh_export": (C.c_int, [P, P, C.POINTER(C.c_uint64), P, C.POINTER(C.c_uint32)]),
     "crt_intersect": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, C.POINTER(Hits), C.c_int]),
+    "crt_intersect_noalloc": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, C.POINTER(Hits)]),
     "crt_occluded": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, P, C.c_int]),
     "crt_renderer_create": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(P)]),
     "crt_renderer_destroy": (C.c_int, [P]),

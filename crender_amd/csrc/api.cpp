@@ -105,6 +105,7 @@ struct crt_context {
     float      *st_ng[3] = {};
     uint8_t    *st_occ = nullptr;
     uint32_t   *st_cnt = nullptr;   // [0] count, [8..15] work counters, [16] error
+    uint8_t    *noalloc = nullptr;  // this device's crt_intersect_noalloc scratch (noalloc.hip)
     std::mutex  mu;                 // serialises queries on the context stream
     std::atomic<int> live{0};       // scenes, renderers and caches created on the context
 };
@@ -1037,7 +1038,17 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     }
     ctx->grid  = trace_grid(o.device, o.trace_waves_per_cu ? (o.trace_waves_per_cu + 3) / 4 : 0);
     ctx->stack = o.trace_stack ? o.trace_stack : kStackLds;
-    *out       = ctx;
+    // the allocation-free query's scratch: a device array whose address is looked up once per
+    // device, here, so that crt_intersect_noalloc itself makes no setup call
+    void *na = nullptr;
+    e = noalloc_scratch(o.device, &na);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(CRT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    ctx->noalloc = static_cast<uint8_t *>(na);
+    *out         = ctx;
     return CRT_OK;
 }
 
@@ -1681,6 +1692,53 @@ int crt_intersect(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *
 {
     if (!hits) return fail(CRT_ERR_INVALID_ARG, "null hits");
     return query(scene, rays, n, hits, nullptr, memory);
+}
+
+int crt_intersect_noalloc(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits)
+{
+    if (!scene || !rays || !hits) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (!scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    if (n > kNoAllocRays)
+        return fail(CRT_ERR_INVALID_ARG, "crt_intersect_noalloc takes at most " + std::to_string(kNoAllocRays) +
+                                             " rays per call (use crt_intersect, or split the batch)");
+    const float *in[8] = {rays->org_x, rays->org_y, rays->org_z, rays->dir_x, rays->dir_y, rays->dir_z, rays->tnear, rays->tfar};
+    for (const float *p : in)
+        if (!p && n) return fail(CRT_ERR_INVALID_ARG, "null ray component");
+    if ((!hits->t || !hits->prim_id) && n) return fail(CRT_ERR_INVALID_ARG, "hits.t and hits.prim_id are required");
+    if (n == 0) return CRT_OK;
+    crt_context *ctx = scene->ctx;
+    if (!ctx->noalloc) return fail(CRT_ERR_STATE, "no scratch for this device");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_CHECK(hipSetDevice(ctx->device));
+    hipStream_t s   = ctx->stream;
+    uint8_t    *sp  = ctx->noalloc;   // the device address of the scratch array, never its name
+    uint32_t   *cnt = reinterpret_cast<uint32_t *>(sp + kNaCnt);
+    RaySoA rs;
+    rs.ox = const_cast<float *>(rays->org_x);
+    rs.oy = const_cast<float *>(rays->org_y);
+    rs.oz = const_cast<float *>(rays->org_z);
+    rs.dx = const_cast<float *>(rays->dir_x);
+    rs.dy = const_cast<float *>(rays->dir_y);
+    rs.dz = const_cast<float *>(rays->dir_z);
+    rs.tn = const_cast<float *>(rays->tnear);
+    rs.tf = const_cast<float *>(rays->tfar);
+    HitSoA h{};
+    h.t    = hits->t;
+    h.u    = hits->u ? hits->u : reinterpret_cast<float *>(sp + kNaU);
+    h.v    = hits->v ? hits->v : reinterpret_cast<float *>(sp + kNaV);
+    h.tri  = reinterpret_cast<uint32_t *>(sp + kNaTri);
+    h.inst = reinterpret_cast<uint32_t *>(sp + kNaInst);
+    HIP_CHECK(launch_noalloc_init(cnt, n, s));
+    TraceQueue q = make_queue(rs, h, cnt, cnt + 8, cnt + 16, false);
+    HIP_CHECK(launch_trace_closest(scene->dev, q, reinterpret_cast<uint32_t *>(sp + kNaSpill),
+                                   TraceConfig{kNoAllocGrid, ctx->stack}, s));
+    HIP_CHECK(launch_hits_to_api(scene->dev, h, n, hits->prim_id, hits->geom_id, hits->ng_x, hits->ng_y, hits->ng_z,
+                                 hits->inst_id, s));
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpyAsync(&err, cnt + 16, sizeof err, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    return CRT_OK;
 }
 
 int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory)

@@ -235,6 +235,24 @@ constexpr uint32_t kXcds       = 8;
 constexpr uint32_t kStackLds   = 32;    // default LDS stack entries per lane
 constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multiple of kStackLds)
 
+// ---- allocation-free queries (noalloc.hip) ---------------------------------------------
+// Layout of the per-device scratch array (byte offsets): counters (count, work counters at
+// +8, error flag at +16), leaf-order triangle and instance indices, u and v for callers that
+// pass none, and the traversal spill stack of a kNoAllocGrid-block launch.
+constexpr uint32_t kNoAllocRays     = 1u << 20;   // rays per crt_intersect_noalloc call
+constexpr uint32_t kNoAllocGrid     = 64;         // traversal blocks (a multiple of kXcds = 8)
+constexpr uint32_t kNoAllocCounters = 64;
+constexpr size_t   kNaCnt   = 0;
+constexpr size_t   kNaTri   = 256;
+constexpr size_t   kNaInst  = kNaTri + 4ull * kNoAllocRays;
+constexpr size_t   kNaU     = kNaInst + 4ull * kNoAllocRays;
+constexpr size_t   kNaV     = kNaU + 4ull * kNoAllocRays;
+constexpr size_t   kNaSpill = kNaV + 4ull * kNoAllocRays;
+constexpr size_t   kNoAllocBytes = kNaSpill + (size_t)kNoAllocGrid * kTraceBlock * kSpillDepth * 2u * sizeof(uint32_t);
+// Device address of this device's scratch array (looked up once per device).
+hipError_t noalloc_scratch(int device, void **ptr);
+hipError_t launch_noalloc_init(uint32_t *cnt, uint32_t n, hipStream_t s);
+
 // ---- integrator ---------------------------------------------------------------------
 enum : uint32_t { kModeCompat = 0, kModePbr = 1 };
 

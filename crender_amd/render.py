@@ -231,6 +231,19 @@ class Scene:
         check(self._L.crt_intersect(self._h, C.byref(rays), n, C.byref(hits), A.CRT_MEM_HOST))
         return out
 
+    def cast_rays_device(self, rays: dict, hits: dict, n: int, noalloc: bool = False) -> None:
+        """crt_intersect on device memory (or crt_intersect_noalloc): `rays` maps org_x ..
+        dir_z, tnear, tfar and `hits` maps t, u, v, prim_id, geom_id, ng_x, ng_y, ng_z, inst_id
+        to device addresses (ints; missing hit fields are not written)."""
+        rk = ("org_x", "org_y", "org_z", "dir_x", "dir_y", "dir_z", "tnear", "tfar")
+        hk = ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z", "inst_id")
+        r = A.Rays(*[C.c_void_p(rays[k]) for k in rk])
+        h = A.Hits(*[C.c_void_p(hits[k]) if hits.get(k) else None for k in hk])
+        if noalloc:
+            check(self._L.crt_intersect_noalloc(self._h, C.byref(r), n, C.byref(h)))
+        else:
+            check(self._L.crt_intersect(self._h, C.byref(r), n, C.byref(h), A.CRT_MEM_DEVICE))
+
     def occluded(self, org, dirs, tnear=1e-5, tfar=np.inf) -> np.ndarray:
         org = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
         dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)

@@ -230,6 +230,17 @@ int crt_scene_bvh_export(crt_scene *scene, void *nodes, uint64_t *node_bytes, ui
 
 /* ---- ray query: scene::cast_ray (scene.cpp:79-98) -> rtcIntersect1 (model.cpp:27) ------ */
 int crt_intersect(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, int memory);
+/* No counterpart in the reference: crt_intersect for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Rays and hits are device pointers; the call allocates nothing.  Its scratch
+ * (queue counters, intermediate hit indices, the traversal's spill stack) is a fixed
+ * 48 MiB device array per GPU, held until the process ends, so a call takes at most
+ * 1,048,576 rays (CRT_ERR_INVALID_ARG beyond) and traces with a 64-block grid (slower than
+ * crt_intersect on large batches).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (from different contexts or threads) race.  Results equal
+ * crt_intersect's with CRT_MEM_DEVICE. */
+#define CRT_NOALLOC_MAX_RAYS 1048576u
+int crt_intersect_noalloc(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits);
 /* any hit in (tnear, tfar]; occluded[i] = 1/0 */
 int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory);
 
