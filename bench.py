@@ -299,6 +299,7 @@ def main(argv=None) -> int:
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent traversal occupancy (0 = default)")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per traversal lane (0 = default)")
     ap.add_argument("--max-leaf", type=int, default=0, help="BVH leaf size limit (0 = library default)")
+    ap.add_argument("--wavefront-paths", type=int, default=0, help="camera paths per wavefront (0 = default)")
     ap.add_argument("--sort-rays", action="store_true", help="reorder secondary rays before tracing")
     ap.add_argument("--sort-materials", action="store_true", help="shade each bounce's hits grouped by material")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work of the 3 timed baseline runs")
@@ -332,7 +333,7 @@ def main(argv=None) -> int:
     spec = scenes.soup(args.tris, seed=2, res=res)
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, phase_timing=True,
                   trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack, sort_rays=args.sort_rays,
-                  sort_materials=args.sort_materials)
+                  sort_materials=args.sort_materials, wavefront_paths=args.wavefront_paths)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
@@ -374,8 +375,9 @@ def main(argv=None) -> int:
             "bvh_width": info["width"],
             "waves_per_cu": args.waves_per_cu or 8,
             "trace_stack": args.stack or 32,
-            # sample passes per wavefront: the library's default rule, floor(2^23 / (W*H)) in [1, 8]
-            "passes_per_wavefront": max(1, min(8, (1 << 23) // (res[0] * res[1]))),
+            # sample passes per wavefront: floor(wavefront paths / (W*H)) in [1, 8], 2^24 paths by default
+            "passes_per_wavefront": max(1, min(8, (args.wavefront_paths or (1 << 24)) // (res[0] * res[1]))),
+            "wavefronts_in_flight": 2,
             "sort_rays": args.sort_rays,
             "sort_materials": args.sort_materials,
             "device": dev,

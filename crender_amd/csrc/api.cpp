@@ -84,8 +84,11 @@ int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace crt
 
 // Camera paths a renderer's wavefront aims at (whole sample passes of the image, at most
-// kMaxWavefrontPasses of them): 8M paths put ~4 passes of a 1080p image in one wavefront.
-constexpr uint32_t kWavefrontPaths     = 1u << 23;
+// kMaxWavefrontPasses of them): 16M paths put 8 passes of a 1080p image in one wavefront.
+// A renderer that trains the radiance cache takes half: each wavefront's training steps
+// scale with its passes, and the next wavefront's cache inference waits for them.
+constexpr uint32_t kWavefrontPaths      = 1u << 24;
+constexpr uint32_t kWavefrontPathsCache = 1u << 23;
 constexpr uint32_t kMaxWavefrontPasses = 8;
 
 struct crt_context {
@@ -94,6 +97,10 @@ struct crt_context {
     hipStream_t stream = nullptr;
     uint32_t    grid   = 0;
     uint32_t   *spill  = nullptr;
+    // a renderer keeps two wavefronts in flight, one per stream, so that one's traversal
+    // fills the GPU while the other's launches drain; the second has its own spill area
+    hipStream_t stream2 = nullptr;
+    uint32_t   *spill2  = nullptr;
     uint32_t    stack  = kStackLds;   // LDS stack entries per traversal lane (8, 16, 32)
     uint32_t    batch_paths = 0;   // tuned wavefront size of crt_render_batch (0 = default)
     // staging for crt_intersect / crt_occluded
@@ -170,6 +177,14 @@ struct crt_renderer {
     std::atomic<uint64_t> samples{0};
     uint32_t     kmax = 1;           // sample passes per wavefront (path slots: kmax * W * H)
     PathBuffers  b{};
+    // second set of path buffers (queues, hits, path state, counters, statistics, cache
+    // queries and targets) for the wavefront on the second stream; the image (raw sum and
+    // AOVs) is shared.  Not used with ray or material sorting (one sort scratch).
+    PathBuffers  b2{};
+    bool         two_sets = false;
+    hipEvent_t   ev_accdone[2] = {};   // after a set's accumulation: the next one's waits for it
+    hipEvent_t   ev_aov[2]     = {};   // after a set's bounce-0 shading (AOV writes)
+    uint32_t     warm_enq = 0;         // cache warm-up passes enqueued
     RaySortScratch sort{};
     float4      *display = nullptr;
     std::vector<void *> allocs;
@@ -207,10 +222,11 @@ namespace {
 
 TraceConfig trace_cfg(const crt_context *ctx) { return TraceConfig{ctx->grid, ctx->stack}; }
 
-int ensure_spill(crt_context *ctx)
+int ensure_spill(crt_context *ctx, int set = 0)
 {
-    if (ctx->spill) return CRT_OK;
-    HIP_CHECK(hipMalloc((void **)&ctx->spill, trace_spill_bytes(ctx->grid)));
+    uint32_t *&sp = set ? ctx->spill2 : ctx->spill;
+    if (sp) return CRT_OK;
+    HIP_CHECK(hipMalloc((void **)&sp, trace_spill_bytes(ctx->grid)));
     return CRT_OK;
 }
 
@@ -294,6 +310,7 @@ void free_renderer_buffers(crt_renderer *r)
     for (void *p : r->allocs) dfree(p);
     r->allocs.clear();
     r->b       = PathBuffers{};
+    r->b2      = PathBuffers{};
     r->display = nullptr;
 }
 
@@ -366,18 +383,13 @@ int alloc_nrc_queries(std::vector<void *> &L, PathBuffers &b, size_t P)
     return CRT_OK;
 }
 
-int alloc_renderer_buffers(crt_renderer *r)
+// One set of path buffers for kmax sample passes plus the cache's training paths; the image
+// (raw sum, AOVs) only when `image` (the second set shares the first one's).
+int alloc_path_set(crt_renderer *r, PathBuffers &b, bool image)
 {
-    free_renderer_buffers(r);
-    PathBuffers &b = r->b;
     b.W    = r->W;
     b.H    = r->H;
     b.npix = r->W * r->H;
-    // a wavefront carries up to kmax sample passes of the image (about wavefront_paths camera
-    // paths: a traversal launch's fixed cost, its ramp-up and the tail of its longest rays,
-    // is then spread over more rays), then train_paths cache-training paths per pass
-    const uint64_t wf = r->ctx->opt.wavefront_paths ? r->ctx->opt.wavefront_paths : kWavefrontPaths;
-    r->kmax = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(kMaxWavefrontPasses, wf / b.npix));
     b.ncam = b.npix * r->kmax;
     const size_t T = r->nrc ? (size_t)r->nopt.train_paths * r->kmax : 0u;
     const size_t S = r->nrc ? r->nopt.train_suffix : 0u;
@@ -407,11 +419,12 @@ int alloc_renderer_buffers(crt_renderer *r)
     RA(&b.occ, P);
     RA(&b.thr, P);
     RA(&b.rad, P);
-    RA(&b.raw, 3 * NP);
-    RA(&b.aov_alb, NP);
-    RA(&b.aov_nrm, NP);
-    RA(&b.aov_dep, NP);
-    RA(&r->display, NP);
+    if (image) {
+        RA(&b.raw, 3 * NP);
+        RA(&b.aov_alb, NP);
+        RA(&b.aov_nrm, NP);
+        RA(&b.aov_dep, NP);
+    }
     RA(&b.cnt, (size_t)kCntWords);
     RA(&b.stats, (size_t)4 * kStatSlots);
     if (T) {
@@ -421,6 +434,44 @@ int alloc_renderer_buffers(crt_renderer *r)
         RA(&b.tq, T * S * 13);
         RA(&b.tt, T * S * 3);
     }
+    if (r->ctx->opt.flags & CRT_FLAG_SORT_MATERIALS) {
+        RA(&b.shade_perm, P);
+        RA(&b.shade_bcount, (size_t)kShadeClasses * ((P + 255) / 256));
+    }
+    if (r->ctx->opt.flags & CRT_FLAG_TRAVERSAL_STATS) RA(&b.tstats, (size_t)8);
+#undef RA
+    if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
+    hipStream_t s = r->ctx->stream;
+    HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+    if (b.tstats) HIP_CHECK(hipMemsetAsync(b.tstats, 0, sizeof(unsigned long long) * 8, s));
+    return CRT_OK;
+}
+
+int alloc_renderer_buffers(crt_renderer *r)
+{
+    free_renderer_buffers(r);
+    const uint32_t npix = r->W * r->H;
+    // a wavefront carries up to kmax sample passes of the image (about wavefront_paths camera
+    // paths: a traversal launch's fixed cost, its ramp-up and the tail of its longest rays,
+    // is then spread over more rays), then train_paths cache-training paths per pass
+    const uint64_t wf = r->ctx->opt.wavefront_paths ? r->ctx->opt.wavefront_paths
+                                                    : (r->nrc ? kWavefrontPathsCache : kWavefrontPaths);
+    r->kmax = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(kMaxWavefrontPasses, wf / npix));
+    r->two_sets = (r->ctx->opt.flags & (CRT_FLAG_SORT_RAYS | CRT_FLAG_SORT_MATERIALS)) == 0;
+    int st = alloc_path_set(r, r->b, true);
+    if (st) return st;
+    if (r->two_sets) {
+        if ((st = alloc_path_set(r, r->b2, false))) return st;
+        r->b2.raw     = r->b.raw;
+        r->b2.aov_alb = r->b.aov_alb;
+        r->b2.aov_nrm = r->b.aov_nrm;
+        r->b2.aov_dep = r->b.aov_dep;
+    }
+    const size_t P = r->b.P, NP = npix;
+#define RA(ptr, n)                          \
+    if ((st = ralloc(r, ptr, n)) != CRT_OK) return st
+    RA(&r->display, NP);
     r->sort = RaySortScratch{};
     if (r->ctx->opt.flags & CRT_FLAG_SORT_RAYS) {
         RA(&r->sort.keys, P);
@@ -431,18 +482,10 @@ int alloc_renderer_buffers(crt_renderer *r)
         RA(reinterpret_cast<uint8_t **>(&r->sort.temp), std::max<size_t>(r->sort.temp_bytes, 1));
         r->sort.capacity = (uint32_t)P;
     }
-    if (r->ctx->opt.flags & CRT_FLAG_SORT_MATERIALS) {
-        RA(&b.shade_perm, P);
-        RA(&b.shade_bcount, (size_t)kShadeClasses * ((P + 255) / 256));
-    }
-    if (r->ctx->opt.flags & CRT_FLAG_TRAVERSAL_STATS) RA(&b.tstats, (size_t)8);
 #undef RA
-    if ((st = alloc_nrc_queries(r->allocs, b, P))) return st;
-    hipStream_t s = r->ctx->stream;
+    const PathBuffers &b = r->b;
+    hipStream_t        s = r->ctx->stream;
     HIP_CHECK(hipMemsetAsync(b.raw, 0, sizeof(float) * 3 * NP, s));
-    HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
-    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
-    if (b.tstats) HIP_CHECK(hipMemsetAsync(b.tstats, 0, sizeof(unsigned long long) * 8, s));
     // image.h:30-33: new images are filled with FLT_MAX
     HIP_CHECK(launch_fill(b.aov_alb, NP, 3.402823466e+38f, s));
     HIP_CHECK(launch_fill(b.aov_nrm, NP, 3.402823466e+38f, s));
@@ -479,7 +522,9 @@ TraceQueue make_queue(const RaySoA &rays, const HitSoA &hits, const uint32_t *co
 
 // A sample pass in flight: the event set it records into and what finishing it needs.
 struct PassTicket {
-    int      set    = 0;
+    int      set    = 0;          // event set (two, so a wavefront can be enqueued while one runs)
+    int      bset   = 0;          // path buffer set and stream (0 or 1; always 0 with one set)
+    RenderParams rp{};            // of the wavefront (its tail is enqueued after its traversal)
     bool     warmup = false;
     bool     timing = false;
     uint32_t units  = 1;   // passes the wavefront counts for: its samples, or 1 warm-up pass
@@ -500,19 +545,28 @@ struct PassTicket {
 // launch's epilogue clears the counters the next steps append to, so a bounce is two or
 // three kernels.  With timing on, one event is recorded between consecutive phases (into
 // `ev`, slots 1 ..) and read after the pass's final sync.
+PathBuffers &path_set(crt_renderer *r, int set) { return set ? r->b2 : r->b; }
+hipStream_t  set_stream(const crt_renderer *r, int set) { return set ? r->ctx->stream2 : r->ctx->stream; }
+
 int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hipEvent_t *ev, PassTicket &t)
 {
     crt_context    *ctx = r->ctx;
     const SceneDev &sc  = r->scene->dev;
-    hipStream_t     s   = ctx->stream;
+    const int       set = t.bset;
+    hipStream_t     s   = set_stream(r, set);
     int             nev = 1;   // next free event slot
     auto mark = [&](int &slot) -> hipError_t {
         slot = nev++;
         return hipEventRecord(ev[slot], s);
     };
     t.timed = 0;
-    if (int st = ensure_spill(ctx)) return st;   // the autotuner may have resized the grid
+    if (int st = ensure_spill(ctx, set)) return st;   // the autotuner may have resized the grid
+    uint32_t *spill = set ? ctx->spill2 : ctx->spill;
+    // AOVs come from the last sample pass (renderer.cpp:367-369): a wavefront writes them
+    // (bounce-0 shading, or generate when there are no bounces) after the previous one did
+    if (rp.max_bounces == 0) HIP_CHECK(hipStreamWaitEvent(s, r->ev_aov[set ^ 1], 0));
     HIP_CHECK(launch_generate(sc, b, rp, s));
+    if (rp.max_bounces == 0) HIP_CHECK(hipEventRecord(r->ev_aov[set], s));
     if (t.timing) HIP_CHECK(mark(t.ev_gen));
     uint32_t cur = 0;
     // bounces to run: camera paths stop at max_bounces, or at term + 1 at the latest when
@@ -551,8 +605,9 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
         q.clear0     = b.cnt + (cur ^ 1u);
         q.clear1     = b.cnt + kCntShadow;
         q.stat       = b.stats + 0;
-        HIP_CHECK(launch_trace_closest(sc, q, ctx->spill, tc, s));
+        HIP_CHECK(launch_trace_closest(sc, q, spill, tc, s));
         if (tm) HIP_CHECK(mark(t.ev_ext[i]));
+        if (i == 0) HIP_CHECK(hipStreamWaitEvent(s, r->ev_aov[set ^ 1], 0));
         if (b.shade_perm) {
             // material-sorted shading: the queue's hits grouped by material class
             HIP_CHECK(launch_shade_sort(sc, b, cur, s));
@@ -560,6 +615,7 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
         } else {
             HIP_CHECK(launch_shade(sc, b, rp, i, cur, s));
         }
+        if (i == 0) HIP_CHECK(hipEventRecord(r->ev_aov[set], s));
         if (tm) HIP_CHECK(mark(t.ev_shade[i]));
         if (tm) t.ev_shadow[i] = -1;
         if (sc.sun_enabled) {
@@ -568,14 +624,14 @@ int run_paths(crt_renderer *r, const PathBuffers &b, const RenderParams &rp, hip
                 qs.tstats     = b.tstats;
                 qs.done       = b.cnt + kCntDoneShadow;
                 qs.stat       = b.stats + 1;
-                HIP_CHECK(launch_trace_any(sc, qs, b.occ, ctx->spill, tc, s));
+                HIP_CHECK(launch_trace_any(sc, qs, b.occ, spill, tc, s));
                 HIP_CHECK(launch_shadow_resolve(b, s));
             } else {
                 // closest hit + alpha pass-through, renderer.cpp:330-345 (bounded at 64 steps)
                 HIP_CHECK(launch_shadow_prep(b, s));
                 for (int step = 0; step < 64; ++step) {
                     TraceQueue qs = make_queue(b.sq, b.shits, b.cnt + kCntShadow, b.cnt + kCntWorkShadow, b.cnt + kCntErr);
-                    HIP_CHECK(launch_trace_closest(sc, qs, ctx->spill, tc, s));
+                    HIP_CHECK(launch_trace_closest(sc, qs, spill, tc, s));
                     HIP_CHECK(launch_shadow_alpha_step(sc, b, s));
                     uint32_t left = 0;
                     HIP_CHECK(hipMemcpyAsync(&left, b.cnt + kCntShadow, sizeof left, hipMemcpyDeviceToHost, s));
@@ -611,7 +667,7 @@ NrcParams nrc_params(const crt_renderer *r)
     return np;
 }
 
-int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count);
+int nrc_train_step(crt_renderer *r, const PathBuffers &t, uint32_t mult, uint32_t count);
 int nrc_wait_training(crt_nrc *n);
 int nrc_wait_locked(crt_nrc *n);   // nrc_wait_training with n->mu held by the caller
 
@@ -620,26 +676,22 @@ int train_after(crt_renderer *r, const PassTicket &t)
 {
     if (!t.train) return CRT_OK;
     const auto t0 = std::chrono::steady_clock::now();
-    const int  st = nrc_train_step(r, std::max(1u, t.nsamp), t.count);
+    const int  st = nrc_train_step(r, path_set(r, t.bset), std::max(1u, t.nsamp), t.count);
     r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return st;
 }
 
-
-// Enqueues one wavefront of `nsamp` sample passes (renderer.cpp:121-125, the pass barrier
-// being stream order) from sample index `sample` on, into event set `set` -- or one cache
-// warm-up pass while those are due.  Nothing waits for the GPU here.  `train_prev`: a
-// finished wavefront whose cache training is enqueued here, once this wavefront's
-// traversal is (host launches of the training steps then overlap its GPU work), and
-// before its inference, which needs the trained weights.
-int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, PassTicket &t,
-                 const PassTicket *train_prev = nullptr)
+// Enqueues the traversal part of one wavefront of `nsamp` sample passes (renderer.cpp:121-125,
+// the pass barrier being stream order) from sample index `sample` on -- or of one cache
+// warm-up pass while those are due -- on the stream of its buffer set: generate, then every
+// bounce's extend / shade / shadow.  Nothing waits for the GPU here.
+int enqueue_trace(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, int bset, PassTicket &t)
 {
     crt_context       *ctx = r->ctx;
-    const PathBuffers &b   = r->b;
-    hipStream_t        s   = ctx->stream;
+    const PathBuffers &b   = path_set(r, bset);
+    hipStream_t        s   = set_stream(r, bset);
     hipEvent_t        *ev  = r->ev[set];
-    RenderParams       rp;
+    RenderParams      &rp  = t.rp;
     rp.cam         = to_camera_dev(r->scene->cam);
     rp.aspect      = r->aspect;
     rp.max_bounces = r->bounces;
@@ -651,9 +703,10 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     rp.nrc         = nrc_params(r);
     // cache warm-up pass: only the training paths (their own Philox stream), no image
     t.set    = set;
-    t.warmup = rp.nrc.enabled && r->nrc_warm_done < r->nopt.warmup_passes;
+    t.bset   = bset;
+    t.warmup = rp.nrc.enabled && r->warm_enq < r->nopt.warmup_passes;
     if (t.warmup) {
-        rp.sample = 0x40000000u + r->nrc_warm_done;
+        rp.sample = 0x40000000u + r->warm_enq++;
         rp.nsamp  = 0;
     }
     t.nsamp = rp.nsamp;
@@ -662,9 +715,18 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
     if (rp.nrc.enabled) rp.nrc.train_live = r->nopt.train_paths * std::max(1u, rp.nsamp);
     t.timing = (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) != 0 && !t.warmup;
     HIP_CHECK(hipEventRecord(ev[0], s));
-    int st = run_paths(r, b, rp, ev, t);
-    if (st) return st;
-    if (train_prev && (st = train_after(r, *train_prev))) return st;
+    return run_paths(r, b, rp, ev, t);
+}
+
+// Enqueues the rest of the wavefront: the cache's inference (after the training enqueued so
+// far), the accumulation (after the previous wavefront's, so each pixel's samples are summed
+// in sample order), the cache-training targets and the error flag.
+int enqueue_tail(crt_renderer *r, PassTicket &t)
+{
+    const PathBuffers  &b  = path_set(r, t.bset);
+    hipStream_t         s  = set_stream(r, t.bset);
+    hipEvent_t         *ev = r->ev[t.set];
+    const RenderParams &rp = t.rp;
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries, with the weights of
         // every training step enqueued so far
@@ -678,7 +740,11 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
             HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvAcc], s));
         }
     }
-    if (!t.warmup) HIP_CHECK(launch_accumulate(b, rp.nsamp, s));
+    if (!t.warmup) {
+        HIP_CHECK(hipStreamWaitEvent(s, r->ev_accdone[t.bset ^ 1], 0));
+        HIP_CHECK(launch_accumulate(b, rp.nsamp, s));
+        HIP_CHECK(hipEventRecord(r->ev_accdone[t.bset], s));
+    }
     t.train = rp.nrc.enabled && (r->nopt.train || t.warmup);
     if (t.train) {
         // training targets from the wavefront's training paths; their count goes to pinned
@@ -686,10 +752,10 @@ int enqueue_pass(crt_renderer *r, uint64_t sample, uint32_t nsamp, int set, Pass
         const uint32_t S = r->nopt.train_suffix, live = rp.nrc.train_live;
         HIP_CHECK(hipMemsetAsync(b.cnt + kCntTrain, 0, sizeof(uint32_t), s));
         HIP_CHECK(launch_nrc_targets(b, S, live * S, live, s));
-        HIP_CHECK(hipMemcpyAsync(r->err_host + 2 + set, b.cnt + kCntTrain, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(r->err_host + 2 + t.set, b.cnt + kCntTrain, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(r->nrc->ev_targets, s));
     }
-    HIP_CHECK(hipMemcpyAsync(r->err_host + set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(r->err_host + t.set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
     return CRT_OK;
 }
@@ -734,52 +800,56 @@ int finish_pass(crt_renderer *r, PassTicket &t)
 }
 
 // Up to `n` passes under one hold of the context, as wavefronts of up to kmax sample passes
-// each (cache warm-up passes one at a time).  Without the neural cache (whose training
-// reads a wavefront's results on the host) the next wavefront is enqueued before the
-// previous one is waited for, so the GPU does not idle between them; stream order and the
-// per-pixel sample order of the accumulation keep the image identical to one pass at a
-// time.  Stops early when `r->stop` is set; `done` receives the number of passes counted.
+// each (cache warm-up passes one at a time).  Two wavefronts are in flight: the next one's
+// traversal is enqueued -- on the other stream with its own buffers, so that it runs beside
+// this one's last, draining launches -- before this one is waited for; then this one's cache
+// training (which needs its target count on the host) and the next one's tail (inference,
+// accumulation) follow.  Accumulation in wavefront order, and AOVs from the last one, keep
+// the image identical to one pass at a time.  Stops early when `r->stop` is set; `done`
+// receives the number of passes counted.
 int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
 {
     crt_context *ctx = r->ctx;
-    // the context's stream and spill area are shared with queries, batches and the autotuner
+    // the context's streams and spill areas are shared with queries, batches and the autotuner
     std::lock_guard<std::mutex> ctx_lock(ctx->mu);
     done = 0;
     if (r->needs_reset)
         return fail(CRT_ERR_STATE, "a previous pass failed; reset (or start) the renderer before rendering again");
-    const bool overlap = r->nrc == nullptr;
-    PassTicket inflight, trained;
+    r->warm_enq = r->nrc_warm_done;
+    PassTicket inflight;
     bool       busy   = false;
-    bool       due    = false;   // `trained` still needs its cache training enqueued
     uint32_t   queued = 0;
     int        st     = CRT_OK;
+    int        next_set = 0;
     while (queued < n && st == CRT_OK && !r->stop.load()) {
         PassTicket     t;
-        const uint32_t k = std::min(r->kmax, n - queued);
-        st  = enqueue_pass(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, busy ? inflight.set ^ 1 : 0, t,
-                           due ? &trained : nullptr);
-        due = false;
+        const uint32_t k     = std::min(r->kmax, n - queued);
+        const int      bset  = r->two_sets ? next_set : 0;
+        st = enqueue_trace(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, next_set, bset, t);
         if (st == CRT_OK && busy) {
             st   = finish_pass(r, inflight);
             busy = false;
-            if (st == CRT_OK) done += inflight.units;
+            if (st == CRT_OK) {
+                done += inflight.units;
+                st = train_after(r, inflight);
+            }
         }
+        if (st == CRT_OK) st = enqueue_tail(r, t);
         if (st != CRT_OK) break;
         queued += t.units;
-        if (overlap) {
-            inflight = t;
-            busy     = true;
-        } else if ((st = finish_pass(r, t)) == CRT_OK) {
-            done += t.units;
-            trained = t;
-            due     = t.train;
-        }
+        inflight = t;
+        busy     = true;
+        next_set ^= 1;
     }
-    if (due && st == CRT_OK) st = train_after(r, trained);
-    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) done += inflight.units;
+    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) {
+        done += inflight.units;
+        st = train_after(r, inflight);
+    }
     if (st != CRT_OK) {
-        (void)hipStreamSynchronize(ctx->stream);   // leave nothing running on failure
-        r->needs_reset = true;                     // accumulation and sample count may disagree
+        // leave nothing running on failure
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream2);
+        r->needs_reset = true;   // accumulation and sample count may disagree
     }
     if (r->nrc) {
         const int sw = nrc_wait_training(r->nrc);
@@ -792,10 +862,9 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
 // ncam .., train_paths per pass) traced train_suffix bounces past the query vertex,
 // recording every vertex from term_bounce on, and terminated into the cache (self-training)
 // give targets L_v = (L_path - L_before_v) / T_v; train_steps Adam steps per pass follow.
-int nrc_train_step(crt_renderer *r, uint32_t mult, uint32_t count)
+int nrc_train_step(crt_renderer *r, const PathBuffers &t, uint32_t mult, uint32_t count)
 {
     crt_nrc           *n  = r->nrc;
-    const PathBuffers &t  = r->b;
     const uint32_t     S  = r->nopt.train_suffix;
     count = std::min(count, r->nopt.train_paths * mult * S);
     const uint32_t steps = std::max<uint32_t>(1u, r->nopt.train_steps) * mult;
@@ -881,10 +950,14 @@ int join_worker(crt_renderer *r)
 int reset_accum(crt_renderer *r)
 {
     hipStream_t s = r->ctx->stream;
+    HIP_CHECK(hipStreamSynchronize(r->ctx->stream2));
     HIP_CHECK(hipMemsetAsync(r->b.raw, 0, sizeof(float) * 3 * (size_t)r->b.npix, s));
-    HIP_CHECK(hipMemsetAsync(r->b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
-    if (r->b.tstats) HIP_CHECK(hipMemsetAsync(r->b.tstats, 0, sizeof(unsigned long long) * 8, s));
-    HIP_CHECK(hipMemsetAsync(r->b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    for (PathBuffers *b : {&r->b, &r->b2}) {
+        if (!b->stats) continue;
+        HIP_CHECK(hipMemsetAsync(b->stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+        if (b->tstats) HIP_CHECK(hipMemsetAsync(b->tstats, 0, sizeof(unsigned long long) * 8, s));
+        HIP_CHECK(hipMemsetAsync(b->cnt, 0, sizeof(uint32_t) * kCntWords, s));
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     r->samples.store(0);
     r->needs_reset = false;
@@ -1032,7 +1105,9 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     ctx->device = o.device;
     ctx->opt    = o;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return fail(CRT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
@@ -1044,6 +1119,7 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     e = noalloc_scratch(o.device, &na);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream2);
         delete ctx;
         return fail(CRT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     }
@@ -1058,7 +1134,9 @@ int crt_context_destroy(crt_context *ctx)
     if (ctx->live.load() > 0) return fail(CRT_ERR_STATE, "context still has scenes, renderers or caches");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
     dfree(ctx->spill);
+    dfree(ctx->spill2);
     for (float *p : ctx->st_ray) dfree(p);
     dfree(ctx->st_t);
     dfree(ctx->st_u);
@@ -1072,6 +1150,7 @@ int crt_context_destroy(crt_context *ctx)
     dfree(ctx->st_occ);
     dfree(ctx->st_cnt);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return CRT_OK;
 }
@@ -1752,6 +1831,10 @@ static void destroy_renderer_events(crt_renderer *r)
     for (auto &set : r->ev)
         for (hipEvent_t e : set)
             if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k) {
+        if (r->ev_accdone[k]) (void)hipEventDestroy(r->ev_accdone[k]);
+        if (r->ev_aov[k]) (void)hipEventDestroy(r->ev_aov[k]);
+    }
     if (r->err_host) (void)hipHostFree(r->err_host);
     r->err_host = nullptr;
 }
@@ -1775,6 +1858,12 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
     for (auto &set : r->ev)
         for (hipEvent_t &e : set)
             if (st == CRT_OK && hipEventCreate(&e) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
+    for (int k = 0; k < 2; ++k) {
+        if (st == CRT_OK && hipEventCreateWithFlags(&r->ev_accdone[k], hipEventDisableTiming) != hipSuccess)
+            st = fail(CRT_ERR_HIP, "hipEventCreate");
+        if (st == CRT_OK && hipEventCreateWithFlags(&r->ev_aov[k], hipEventDisableTiming) != hipSuccess)
+            st = fail(CRT_ERR_HIP, "hipEventCreate");
+    }
     if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 4 * sizeof(uint32_t)) != hipSuccess)
         st = fail(CRT_ERR_OUT_OF_MEMORY, "pinned host allocation");
     if (st == CRT_OK) st = alloc_renderer_buffers(r);
@@ -2001,10 +2090,13 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
 {
     if (!r || !stats) return fail(CRT_ERR_INVALID_ARG, "null argument");
     std::vector<unsigned long long> slots(4 * kStatSlots);
-    HIP_CHECK(hipMemcpy(slots.data(), r->b.stats, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
     unsigned long long s[4] = {0, 0, 0, 0};
-    for (uint32_t i = 0; i < kStatSlots; ++i)
-        for (int k = 0; k < 4; ++k) s[k] += slots[4 * i + k];
+    for (const PathBuffers *b : {&r->b, &r->b2}) {
+        if (!b->stats) continue;
+        HIP_CHECK(hipMemcpy(slots.data(), b->stats, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < kStatSlots; ++i)
+            for (int k = 0; k < 4; ++k) s[k] += slots[4 * i + k];
+    }
     std::memset(stats, 0, sizeof *stats);
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - r->t0).count();
     stats->running_time       = t;
@@ -2023,14 +2115,18 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->extend_launches = r->extend_launches;
     stats->ms_nrc_train    = r->ms_nrc_train;
     if (r->b.tstats) {
-        unsigned long long ts[8];
         HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
-        HIP_CHECK(hipMemcpy(ts, r->b.tstats, sizeof ts, hipMemcpyDeviceToHost));
-        stats->node_visits     = ts[0];
-        stats->leaf_visits     = ts[1];
-        stats->tri_tests       = ts[2];
-        stats->wave_node_steps = ts[3];
-        stats->wave_leaf_steps = ts[4];
+        HIP_CHECK(hipStreamSynchronize(r->ctx->stream2));
+        for (const PathBuffers *b : {&r->b, &r->b2}) {
+            if (!b->tstats) continue;
+            unsigned long long ts[8];
+            HIP_CHECK(hipMemcpy(ts, b->tstats, sizeof ts, hipMemcpyDeviceToHost));
+            stats->node_visits     += ts[0];
+            stats->leaf_visits     += ts[1];
+            stats->tri_tests       += ts[2];
+            stats->wave_node_steps += ts[3];
+            stats->wave_leaf_steps += ts[4];
+        }
     }
     if (r->nrc && r->loss_tiles) {
         if (int wst = nrc_wait_training(r->nrc)) return wst;
@@ -2728,7 +2824,9 @@ int crt_autotune(crt_context *ctx, const crt_job *jobs, uint32_t njobs, uint32_t
     auto set_grid = [&](uint32_t blocks) {
         std::lock_guard<std::mutex> lock(ctx->mu);   // no pass or query may use the old spill area
         dfree(ctx->spill);   // sized by the grid; renderers re-allocate it on demand
-        ctx->spill = nullptr;
+        dfree(ctx->spill2);
+        ctx->spill  = nullptr;
+        ctx->spill2 = nullptr;
         ctx->grid  = trace_grid(ctx->device, blocks);
         ctx->opt.trace_waves_per_cu = 4 * blocks;
     };

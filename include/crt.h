@@ -122,8 +122,10 @@ typedef struct crt_options {
     uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy (8 waves per CU) */
     uint32_t flags;         /* CRT_FLAG_* */
     uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 32), 8, 16 or 32 */
-    uint32_t wavefront_paths; /* camera paths per renderer wavefront: 0 = 2^23; a wavefront
-                                 carries floor(wavefront_paths / (W*H)) sample passes (1 to 8) */
+    uint32_t wavefront_paths; /* camera paths per renderer wavefront: 0 = 2^24 (2^23 for a
+                                 renderer with a radiance cache); a wavefront carries
+                                 floor(wavefront_paths / (W*H)) sample passes (1 to 8).  Two
+                                 wavefronts are in flight (two sets of path buffers, two streams). */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
