@@ -224,7 +224,7 @@ def nrc_equal_time(ctx, sc, ref_spp: int) -> dict:
         "plain": [{"spp": a, "seconds": b, "mse": c} for a, b, c in pc],
         "cache": [{"spp": a, "seconds": b, "mse": c} for a, b, c in nc],
         "cache_warmup_seconds": t_warm,
-        "cache_train_ms_per_pass": st["ms_nrc_train"] / max(1, warm + 64),
+        "cache_train_gpu_ms_per_pass": st["ms_nrc_train"] / max(1, warm + 64),   # GPU time of the training steps
         "equal_time_seconds": T,
         "plain_mse_at_T": target,
         "cache_seconds_to_plain_mse": t_match,

@@ -203,7 +203,8 @@ struct crt_renderer {
     // end of pass.  Two sets, so that a pass can be enqueued while the previous one runs.
     static constexpr int kTimedBounces = 15, kEvAcc = 62, kEvEnd = 63;
     hipEvent_t ev[2][64] = {};
-    uint32_t  *err_host = nullptr;     // pinned: per event set, the traversal error flag [set] and
+    uint32_t  *err_host = nullptr;     // pinned: per event set, the traversal error flag [set], the
+                                       // diverged-cache flag [4 + set] and
                                        // the cache-training target count [2 + set]
     double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
     uint64_t extend_launches = 0;
@@ -213,6 +214,14 @@ struct crt_renderer {
     double                 ms_nrc_train = 0.0;
     uint32_t               loss_tiles = 0;   // tiles of the last training step (loss partials)
     uint32_t               nrc_warm_done = 0;   // warm-up passes run since set_nrc
+    // the cache's predictions became non-finite: it is not used again until set_nrc, and the
+    // sample passes that used it are dropped and rendered again
+    bool                   nrc_failed = false;
+    uint32_t               nrc_discarded = 0;
+    // GPU time of the cache's training steps: events around a training group on the cache's
+    // stream, read once the group is known to be done
+    hipEvent_t             ev_train[2] = {};
+    bool                   train_timed = false;   // ev_train holds an unread group
     // A failed pass may have added samples to the accumulation without counting them (or the
     // reverse): passes are refused until reset_accum (start / reset / resize) clears it.
     bool                   needs_reset = false;
@@ -530,6 +539,7 @@ struct PassTicket {
     uint32_t units  = 1;   // passes the wavefront counts for: its samples, or 1 warm-up pass
     uint32_t nsamp  = 0;   // sample passes of the image (0: cache warm-up)
     bool     train  = false;   // cache training targets were computed for this wavefront
+    bool     discard = false;  // rendered with a diverged cache: not accumulated, not counted
     uint32_t count  = 0;       // their number (read when the wavefront is finished)
     // phase events (timing): each phase ends at its event and starts at the one recorded
     // before it; -1 where a phase did not run
@@ -656,7 +666,7 @@ NrcParams nrc_params(const crt_renderer *r)
     np.bmin = sc.bmin;
     const f3 ext = sc.bmax - sc.bmin;
     np.binv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
-    if (r->nrc && r->bounces > 0) {
+    if (r->nrc && r->bounces > 0 && !r->nrc_failed) {
         np.enabled     = 1;
         np.term        = r->nopt.term_bounce;
         np.footprint_c = r->nopt.footprint_c;
@@ -668,6 +678,7 @@ NrcParams nrc_params(const crt_renderer *r)
 }
 
 int nrc_train_step(crt_renderer *r, const PathBuffers &t, uint32_t mult, uint32_t count);
+void harvest_train_time(crt_renderer *r, bool wait);
 int nrc_wait_training(crt_nrc *n);
 int nrc_wait_locked(crt_nrc *n);   // nrc_wait_training with n->mu held by the caller
 
@@ -675,10 +686,19 @@ int nrc_wait_locked(crt_nrc *n);   // nrc_wait_training with n->mu held by the c
 int train_after(crt_renderer *r, const PassTicket &t)
 {
     if (!t.train) return CRT_OK;
-    const auto t0 = std::chrono::steady_clock::now();
-    const int  st = nrc_train_step(r, path_set(r, t.bset), std::max(1u, t.nsamp), t.count);
-    r->ms_nrc_train += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return st;
+    return nrc_train_step(r, path_set(r, t.bset), std::max(1u, t.nsamp), t.count);
+}
+
+// Adds the GPU time of the last timed training group once it is done (wait: block for it).
+void harvest_train_time(crt_renderer *r, bool wait)
+{
+    if (!r->train_timed) return;
+    if (wait) (void)hipEventSynchronize(r->ev_train[1]);
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r->ev_train[0], r->ev_train[1]) == hipSuccess) {
+        r->ms_nrc_train += ms;
+        r->train_timed = false;
+    }
 }
 
 // Enqueues the traversal part of one wavefront of `nsamp` sample passes (renderer.cpp:121-125,
@@ -727,6 +747,15 @@ int enqueue_tail(crt_renderer *r, PassTicket &t)
     hipStream_t         s  = set_stream(r, t.bset);
     hipEvent_t         *ev = r->ev[t.set];
     const RenderParams &rp = t.rp;
+    if (rp.nrc.enabled && r->nrc_failed) {
+        // traced with the cache, which diverged meanwhile: nothing of this wavefront is kept
+        t.discard = true;
+        t.train   = false;
+        HIP_CHECK(hipMemsetAsync(r->err_host + 4 + t.set, 0, sizeof(uint32_t), s));
+        HIP_CHECK(hipMemcpyAsync(r->err_host + t.set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
+        return CRT_OK;
+    }
     if (rp.nrc.enabled) {
         // one inference for the camera and the training paths' queries, with the weights of
         // every training step enqueued so far
@@ -756,6 +785,7 @@ int enqueue_tail(crt_renderer *r, PassTicket &t)
         HIP_CHECK(hipEventRecord(r->nrc->ev_targets, s));
     }
     HIP_CHECK(hipMemcpyAsync(r->err_host + t.set, b.cnt + kCntErr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(r->err_host + 4 + t.set, b.cnt + kCntNrcBad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(ev[crt_renderer::kEvEnd], s));
     return CRT_OK;
 }
@@ -791,6 +821,17 @@ int finish_pass(crt_renderer *r, PassTicket &t)
     }
     if (r->err_host[t.set]) return fail(CRT_ERR_HIP, "traversal stack overflow");
     t.count = r->err_host[2 + t.set];
+    if (!t.discard && r->err_host[4 + t.set]) {
+        // the cache diverged: this wavefront was not accumulated; the renderer goes on
+        // without the cache (automatic fallback to the plain tracer)
+        r->nrc_failed = true;
+        t.discard     = true;
+        t.train       = false;
+    }
+    if (t.discard) {
+        r->nrc_discarded += t.nsamp;
+        return CRT_OK;
+    }
     if (t.warmup) {
         r->nrc_warm_done++;
         return CRT_OK;
@@ -829,19 +870,21 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
         if (st == CRT_OK && busy) {
             st   = finish_pass(r, inflight);
             busy = false;
-            if (st == CRT_OK) {
+            if (st == CRT_OK && inflight.discard) {
+                queued -= inflight.units;   // its passes are rendered again
+            } else if (st == CRT_OK) {
                 done += inflight.units;
                 st = train_after(r, inflight);
             }
         }
         if (st == CRT_OK) st = enqueue_tail(r, t);
         if (st != CRT_OK) break;
-        queued += t.units;
+        if (!t.discard) queued += t.units;
         inflight = t;
         busy     = true;
         next_set ^= 1;
     }
-    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK) {
+    if (busy && st == CRT_OK && (st = finish_pass(r, inflight)) == CRT_OK && !inflight.discard) {
         done += inflight.units;
         st = train_after(r, inflight);
     }
@@ -875,6 +918,9 @@ int nrc_train_step(crt_renderer *r, const PathBuffers &t, uint32_t mult, uint32_
     // for ev_trained, and its targets (which overwrite tq / tt) come after that inference
     hipStream_t s = n->ts;
     HIP_CHECK(hipStreamWaitEvent(s, n->ev_targets, 0));
+    // the previous group is done (this wavefront's inference waited for it)
+    harvest_train_time(r, true);
+    HIP_CHECK(hipEventRecord(r->ev_train[0], s));
     for (uint32_t k = 0; k < steps; ++k) {
         const float *q  = t.tq + (size_t)k * batch * 13;
         const float *tt = t.tt + (size_t)k * batch * 3;
@@ -883,6 +929,8 @@ int nrc_train_step(crt_renderer *r, const PathBuffers &t, uint32_t mult, uint32_
         HIP_CHECK(nrc_launch_adam(n->partial, nrc_partial_rows(batch), n->params, n->m, n->v, n->W, n->WT, n->opt.lr,
                                   n->opt.beta1, n->opt.beta2, n->opt.eps, n->step, 1, s));
     }
+    HIP_CHECK(hipEventRecord(r->ev_train[1], s));
+    r->train_timed = true;
     HIP_CHECK(hipEventRecord(n->ev_trained, s));
     n->pending = true;
     // the loss of the last step (sum of its per-tile partials) is read when stats ask for it
@@ -1832,6 +1880,7 @@ static void destroy_renderer_events(crt_renderer *r)
         for (hipEvent_t e : set)
             if (e) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k) {
+        if (r->ev_train[k]) (void)hipEventDestroy(r->ev_train[k]);
         if (r->ev_accdone[k]) (void)hipEventDestroy(r->ev_accdone[k]);
         if (r->ev_aov[k]) (void)hipEventDestroy(r->ev_aov[k]);
     }
@@ -1859,12 +1908,13 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
         for (hipEvent_t &e : set)
             if (st == CRT_OK && hipEventCreate(&e) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
     for (int k = 0; k < 2; ++k) {
+        if (st == CRT_OK && hipEventCreate(&r->ev_train[k]) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
         if (st == CRT_OK && hipEventCreateWithFlags(&r->ev_accdone[k], hipEventDisableTiming) != hipSuccess)
             st = fail(CRT_ERR_HIP, "hipEventCreate");
         if (st == CRT_OK && hipEventCreateWithFlags(&r->ev_aov[k], hipEventDisableTiming) != hipSuccess)
             st = fail(CRT_ERR_HIP, "hipEventCreate");
     }
-    if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 4 * sizeof(uint32_t)) != hipSuccess)
+    if (st == CRT_OK && hipHostMalloc((void **)&r->err_host, 8 * sizeof(uint32_t)) != hipSuccess)
         st = fail(CRT_ERR_OUT_OF_MEMORY, "pinned host allocation");
     if (st == CRT_OK) st = alloc_renderer_buffers(r);
     if (st != CRT_OK) {
@@ -2113,7 +2163,10 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->ms_accumulate = r->ms_phase[4];
     stats->ms_render     = r->ms_phase[5];
     stats->extend_launches = r->extend_launches;
+    if (r->nrc) harvest_train_time(r, true);
     stats->ms_nrc_train    = r->ms_nrc_train;
+    stats->nrc_detached    = r->nrc_failed ? 1u : 0u;
+    stats->nrc_discarded_passes = r->nrc_discarded;
     if (r->b.tstats) {
         HIP_CHECK(hipStreamSynchronize(r->ctx->stream));
         HIP_CHECK(hipStreamSynchronize(r->ctx->stream2));
@@ -2236,6 +2289,9 @@ extern "C" int crt_renderer_set_nrc(crt_renderer *r, crt_nrc *nrc, const crt_nrc
     r->nopt          = o;
     r->nrc_warm_done = 0;
     r->loss_tiles    = 0;
+    r->nrc_failed    = false;
+    r->nrc_discarded = 0;
+    r->train_timed   = false;
     if ((st = alloc_renderer_buffers(r))) return st;
     r->samples.store(0);
     return reset_accum(r);

@@ -295,6 +295,7 @@ constexpr uint32_t kStatSlots = 64;
 // layout of PathBuffers::cnt
 enum : uint32_t {
     kCntQ0 = 0, kCntQ1 = 1, kCntShadow = 2, kCntShadow2 = 3, kCntErr = 4, kCntNrcQ = 5, kCntTrain = 6,
+    kCntNrcBad = 7,   // a cache prediction of the wavefront was not finite: its accumulation is skipped
     kCntWorkExt = 8, kCntWorkShadow = 16, kCntDoneExt = 24, kCntDoneShadow = 25, kCntWords = 32
 };
 

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void k_generate(SceneDev sc, PathBuffers b, Re
         if (threadIdx.x == 0) {
             b.cnt[kCntQ0]          = rp.max_bounces > 0 ? ncl + ntrain : 0u;
             b.cnt[kCntNrcQ]        = 0;
+            b.cnt[kCntNrcBad]      = 0;
             b.cnt[kCntDoneExt]     = 0;
             b.cnt[kCntDoneShadow]  = 0;
         }
@@ -695,7 +696,7 @@ __global__ void k_shadow_swap(PathBuffers b)
 __global__ __launch_bounds__(256) void k_accumulate(PathBuffers b, uint32_t nsamp)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= b.npix) return;
+    if (p >= b.npix || b.cnt[kCntNrcBad]) return;
     const uint32_t x = p % b.W, y = p / b.W;
     const uint32_t q = (b.W - 1 - x) + (b.H - 1 - y) * b.W;
     float sx = b.raw[3 * (size_t)q + 0], sy = b.raw[3 * (size_t)q + 1], sz = b.raw[3 * (size_t)q + 2];
@@ -768,9 +769,13 @@ __global__ __launch_bounds__(256) void k_nrc_apply(PathBuffers b)
     const uint32_t p = b.nq_path[k];
     const float4   t = b.nq_thr[k];
     float4         r = b.rad[p];
-    r.x += t.x * fmaxf(b.nq_out[3 * (size_t)k + 0], 0.0f);
-    r.y += t.y * fmaxf(b.nq_out[3 * (size_t)k + 1], 0.0f);
-    r.z += t.z * fmaxf(b.nq_out[3 * (size_t)k + 2], 0.0f);
+    const float    y0 = b.nq_out[3 * (size_t)k + 0], y1 = b.nq_out[3 * (size_t)k + 1], y2 = b.nq_out[3 * (size_t)k + 2];
+    // a diverged cache (non-finite or exploding weights) shows here first: flag the wavefront
+    // (its accumulation is skipped and the renderer continues without the cache)
+    if (!(fabsf(y0) < 1e30f) || !(fabsf(y1) < 1e30f) || !(fabsf(y2) < 1e30f)) atomicOr(&b.cnt[kCntNrcBad], 1u);
+    r.x += t.x * fmaxf(y0, 0.0f);
+    r.y += t.y * fmaxf(y1, 0.0f);
+    r.z += t.z * fmaxf(y2, 0.0f);
     b.rad[p] = r;
 }
 

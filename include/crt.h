@@ -153,13 +153,17 @@ typedef struct crt_stats {
     double   ms_generate, ms_extend, ms_shade, ms_shadow, ms_accumulate; /* with CRT_FLAG_PHASE_TIMING */
     double   ms_render;          /* GPU time of all passes (HIP events) */
     uint64_t extend_launches;    /* closest-hit launches covered by ms_extend */
-    double   ms_nrc_train;       /* host time of the cache's training passes */
+    double   ms_nrc_train;       /* GPU time of the cache's training steps (events on its stream) */
     float    nrc_loss;           /* training loss of the last step (one tile) */
     uint64_t node_visits;        /* with CRT_FLAG_TRAVERSAL_STATS, over all traced rays: */
     uint64_t leaf_visits;        /*   inner nodes tested, leaves entered, */
     uint64_t tri_tests;          /*   ray-triangle tests, */
     uint64_t wave_node_steps;    /*   wave iterations of the inner-node loop, */
     uint64_t wave_leaf_steps;    /*   wave iterations of the leaf loop */
+    uint32_t nrc_detached;       /* 1: the cache's predictions became non-finite and the renderer
+                                    went on with the plain tracer (until the next set_nrc) */
+    uint32_t nrc_discarded_passes; /* sample passes rendered with the diverged cache, dropped
+                                      from the image and rendered again without it */
 } crt_stats;
 
 /* BVH summary returned by crt_scene_bvh_info */

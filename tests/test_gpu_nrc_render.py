@@ -104,3 +104,39 @@ def test_equal_time_mse_gain(setup):
           f"{n_nrc} spp MSE {mse_nrc:.3e}; ratio {mse_plain / max(mse_nrc, 1e-30):.2f}")
     # the cache must at least halve the error of the plain tracer at equal time
     assert mse_nrc < 0.5 * mse_plain
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_diverged_cache_falls_back_to_the_plain_tracer(setup, bad):
+    """Non-finite weights (set through crt_nrc_set_params) make the cache's predictions
+    non-finite: the renderer drops the wavefronts that used them, detaches the cache and
+    renders those passes again with the plain tracer, so the image equals, bit for bit, what
+    a renderer without the cache accumulates over the same passes."""
+    ctx, sc, spec = setup[:3]
+    print(f"diverged cache ({bad})", flush=True)
+    plain = _renderer(ctx, sc, spec, seed=31)
+    plain.render(6)
+    ref = plain.raw()
+    plain.close()
+    nrc = Nrc(ctx, seed=3)
+    w = nrc.params()
+    w[::7] = bad
+    nrc.set_params(w)
+    r = _renderer(ctx, sc, spec, seed=31)
+    r.set_nrc(nrc)
+    r.render(6)
+    img = r.raw()
+    st = r.current_stats()
+    assert r.current_sample_count() == 6
+    assert np.isfinite(img).all()
+    assert np.array_equal(img, ref)
+    assert st["nrc_detached"] == 1 and st["nrc_discarded_passes"] >= 6
+    # attaching a healthy cache again clears the fallback
+    nrc2 = Nrc(ctx, seed=4)
+    r.set_nrc(nrc2)
+    r.render(2)
+    st = r.current_stats()
+    assert st["nrc_detached"] == 0 and np.isfinite(r.raw()).all()
+    r.close()
+    nrc.close()
+    nrc2.close()
