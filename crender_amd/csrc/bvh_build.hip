@@ -42,31 +42,39 @@ __device__ __forceinline__ float area3(float lx, float ly, float lz, float hx, f
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
-// Bin index of centroid coordinate c for a segment with centroid range [lo, hi].  Both
-// the binning and the partition call this, so they agree bit for bit.
-__device__ __forceinline__ int bin_of(float c, float lo, float hi)
+// Bin index of centroid coordinate c for a segment with centroid range [lo, hi]: the scale
+// k = 16 (1 - 1e-6) / (hi - lo) is formed once per segment and axis (0 for an empty range)
+// and bin_at applies it per primitive.  The binning and the partition both use these, so
+// they agree bit for bit.
+__device__ __forceinline__ float bin_scale(float lo, float hi)
 {
     const float ext = hi - lo;
-    if (!(ext > 0.0f)) return 0;
-    const float k = (float)kNumBins * (1.0f - 1e-6f) / ext;
+    return ext > 0.0f ? (float)kNumBins * (1.0f - 1e-6f) / ext : 0.0f;
+}
+
+__device__ __forceinline__ int bin_at(float c, float lo, float k)
+{
+    if (!(k > 0.0f)) return 0;
     int b = (int)((c - lo) * k);
     return b < 0 ? 0 : (b >= kNumBins ? kNumBins - 1 : b);
 }
 
-__device__ __forceinline__ void bins_init(int *bins, int tid, int nthreads)
+__device__ __forceinline__ int bin_of(float c, float lo, float hi) { return bin_at(c, lo, bin_scale(lo, hi)); }
+
+__device__ __forceinline__ float4 scale4(const float *lo, const float *hi)
 {
-    for (int i = tid; i < kBinWords; i += nthreads) {
-        const int w = i % 13;
-        bins[i] = w == 0 ? 0 : ((w >= 1 && w <= 3) || (w >= 7 && w <= 9)) ? f2ord(kInf) : f2ord(-kInf);
-    }
+    return make_float4(bin_scale(lo[0], hi[0]), bin_scale(lo[1], hi[1]), bin_scale(lo[2], hi[2]), 0.0f);
 }
 
-__device__ __forceinline__ void bins_add(int *bins, float4 c, float4 lo, float4 hi, const float *cblo,
-                                         const float *cbhi)
+// The centroid range is passed by value (its low corner and the per-axis scales): a pointer
+// into a task held in registers would put the task in scratch memory.
+__device__ __forceinline__ void bins_add(int *bins, float4 c, float4 lo, float4 hi, float4 cblo, float4 kscale)
 {
     const float cc[3] = {c.x, c.y, c.z};
+    const float bl[3] = {cblo.x, cblo.y, cblo.z}, ks[3] = {kscale.x, kscale.y, kscale.z};
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int b = bin_of(cc[a], cblo[a], cbhi[a]);
+        const int b = bin_at(cc[a], bl[a], ks[a]);
         int *p = bins + (a * kNumBins + b) * 13;
         atomicAdd(p + 0, 1);
         atomicMin(p + 1, f2ord(lo.x));
@@ -81,6 +89,23 @@ __device__ __forceinline__ void bins_add(int *bins, float4 c, float4 lo, float4 
         atomicMax(p + 10, f2ord(c.x));
         atomicMax(p + 11, f2ord(c.y));
         atomicMax(p + 12, f2ord(c.z));
+    }
+}
+
+__device__ __forceinline__ float4 lo4(const float *v) { return make_float4(v[0], v[1], v[2], 0.0f); }
+__device__ __forceinline__ float pick(float x, float y, float z, int a) { return a == 0 ? x : (a == 1 ? y : z); }
+
+// One thread per (axis, bin) record of 13 words: count, box low / high, centroid low / high.
+__device__ __forceinline__ void bins_init(int *bins, int tid, int nthreads)
+{
+    const int lo = f2ord(kInf), hi = f2ord(-kInf);
+    for (int r = tid; r < 3 * kNumBins; r += nthreads) {
+        int *p = bins + r * 13;
+        p[0] = 0;
+        p[1] = p[2] = p[3] = lo;
+        p[4] = p[5] = p[6] = hi;
+        p[7] = p[8] = p[9] = lo;
+        p[10] = p[11] = p[12] = hi;
     }
 }
 
@@ -115,26 +140,32 @@ __device__ __forceinline__ BinVal bv_join(const BinVal &a, const BinVal &b)
     return r;
 }
 
-template <int kDir>  // 0: shfl_up, 1: shfl_down
-__device__ __forceinline__ BinVal bv_shfl(const BinVal &v, int d)
+// Moves a value d lanes within its row of 16 lanes with a DPP row shift (a VALU operand
+// modifier, no LDS traffic): kDir 0 takes lane - d (row_shr), 1 takes lane + d (row_shl).
+// Lanes whose source falls outside the row keep their own value (callers ignore them).
+template <int kDir, int d>
+__device__ __forceinline__ uint32_t row_shift(uint32_t v)
+{
+    constexpr int ctrl = (kDir == 0 ? 0x110 : 0x100) + d;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, ctrl, 0xF, 0xF, false);
+}
+template <int kDir, int d>
+__device__ __forceinline__ float row_shift(float v)
+{
+    return __uint_as_float(row_shift<kDir, d>(__float_as_uint(v)));
+}
+
+template <int kDir, int d>  // 0: from lane - d, 1: from lane + d, within rows of 16 lanes
+__device__ __forceinline__ BinVal bv_shfl(const BinVal &v)
 {
     BinVal r;
-    if (kDir == 0) {
-        r.n = __shfl_up(v.n, d, 16);
-        for (int k = 0; k < 3; ++k) {
-            r.lo[k]  = __shfl_up(v.lo[k], d, 16);
-            r.hi[k]  = __shfl_up(v.hi[k], d, 16);
-            r.clo[k] = __shfl_up(v.clo[k], d, 16);
-            r.chi[k] = __shfl_up(v.chi[k], d, 16);
-        }
-    } else {
-        r.n = __shfl_down(v.n, d, 16);
-        for (int k = 0; k < 3; ++k) {
-            r.lo[k]  = __shfl_down(v.lo[k], d, 16);
-            r.hi[k]  = __shfl_down(v.hi[k], d, 16);
-            r.clo[k] = __shfl_down(v.clo[k], d, 16);
-            r.chi[k] = __shfl_down(v.chi[k], d, 16);
-        }
+    r.n = row_shift<kDir, d>(v.n);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k]  = row_shift<kDir, d>(v.lo[k]);
+        r.hi[k]  = row_shift<kDir, d>(v.hi[k]);
+        r.clo[k] = row_shift<kDir, d>(v.clo[k]);
+        r.chi[k] = row_shift<kDir, d>(v.chi[k]);
     }
     return r;
 }
@@ -168,16 +199,21 @@ __device__ void sah_wave(const int *bins, SahOut *out)
             }
         }
     }
+    // segmented (16-lane) inclusive prefix and suffix joins
     BinVal pre = v, suf = v;
-    for (int d = 1; d < 16; d <<= 1) {
-        const BinVal o = bv_shfl<0>(pre, d);
-        if (b >= d) pre = bv_join(pre, o);
+#define SCAN_STEP(d)                                       \
+    {                                                      \
+        const BinVal o = bv_shfl<0, d>(pre);               \
+        if (b >= d) pre = bv_join(pre, o);                 \
+        const BinVal q = bv_shfl<1, d>(suf);               \
+        if (b + d < 16) suf = bv_join(suf, q);             \
     }
-    for (int d = 1; d < 16; d <<= 1) {
-        const BinVal o = bv_shfl<1>(suf, d);
-        if (b + d < 16) suf = bv_join(suf, o);
-    }
-    const BinVal rs = bv_shfl<1>(suf, 1);
+    SCAN_STEP(1)
+    SCAN_STEP(2)
+    SCAN_STEP(4)
+    SCAN_STEP(8)
+#undef SCAN_STEP
+    const BinVal rs = bv_shfl<1, 1>(suf);
     float cost = kInf;
     if (lane < 48 && b < 15 && pre.n > 0u && rs.n > 0u) {
         cost = area3(pre.lo[0], pre.lo[1], pre.lo[2], pre.hi[0], pre.hi[1], pre.hi[2]) * (float)pre.n +
@@ -356,21 +392,50 @@ __global__ void k_root_task(const int *root, uint32_t n, BuildTask *large, Build
 }
 
 // ---- stage 1a: segments larger than one chunk --------------------------------------------
-__global__ void k_make_chunks(const BuildTask *tasks, uint32_t ntasks, uint32_t *chunk_task,
-                              uint32_t *chunk_begin, uint32_t *task_chunk0, uint32_t *nchunks)
+// Chunks of kChunk primitives of every large task, in task order: one block of 1024 threads,
+// a scan of the per-task chunk counts, then one chunk per thread.
+constexpr uint32_t kChunkBlock = 1024;
+
+__global__ __launch_bounds__(kChunkBlock) void k_make_chunks(const BuildTask *tasks, uint32_t ntasks,
+                                                             uint32_t *chunk_task, uint32_t *chunk_begin,
+                                                             uint32_t *task_chunk0, uint32_t *nchunks)
 {
-    if (threadIdx.x != 0) return;
-    uint32_t c = 0;
-    for (uint32_t t = 0; t < ntasks; ++t) {
-        task_chunk0[t] = c;
-        for (uint32_t b = tasks[t].begin; b < tasks[t].end; b += kChunk) {
-            chunk_task[c]  = t;
-            chunk_begin[c] = b;
-            ++c;
-        }
+    __shared__ uint32_t part[kChunkBlock];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (ntasks + kChunkBlock - 1) / kChunkBlock;
+    const uint32_t t0 = min(tid * per, ntasks), t1 = min(t0 + per, ntasks);
+    auto nch = [&](uint32_t t) { return (tasks[t].end - tasks[t].begin + kChunk - 1) / kChunk; };
+    uint32_t local = 0;
+    for (uint32_t t = t0; t < t1; ++t) local += nch(t);
+    part[tid] = local;
+    __syncthreads();
+    for (uint32_t d = 1; d < kChunkBlock; d <<= 1) {   // inclusive scan
+        const uint32_t v = tid >= d ? part[tid - d] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
     }
-    task_chunk0[ntasks] = c;
-    *nchunks            = c;
+    uint32_t run = part[tid] - local;
+    for (uint32_t t = t0; t < t1; ++t) {
+        task_chunk0[t] = run;
+        run += nch(t);
+    }
+    const uint32_t total = part[kChunkBlock - 1];
+    if (tid == 0) {
+        task_chunk0[ntasks] = total;
+        *nchunks            = total;
+    }
+    __syncthreads();   // task_chunk0 written by the block is visible to it
+    for (uint32_t c = tid; c < total; c += kChunkBlock) {
+        uint32_t lo = 0, hi = ntasks;   // the task t with task_chunk0[t] <= c < task_chunk0[t + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (task_chunk0[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        chunk_task[c]  = lo;
+        chunk_begin[c] = tasks[lo].begin + (c - task_chunk0[lo]) * kChunk;
+    }
 }
 
 __global__ void k_clear_bins(int *bins, uint32_t ntasks)
@@ -391,9 +456,10 @@ __global__ __launch_bounds__(256) void k_bin_large(const BuildTask *tasks, const
     const uint32_t  e = min(b + kChunk, t.end);
     bins_init(bins, threadIdx.x, blockDim.x);
     __syncthreads();
+    const float4 cblo = lo4(t.cblo), ks = scale4(t.cblo, t.cbhi);
     for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint32_t id = idx[i];
-        bins_add(bins, cen[id], plo[id], phi[id], t.cblo, t.cbhi);
+        bins_add(bins, cen[id], plo[id], phi[id], cblo, ks);
     }
     __syncthreads();
     int *g = gbins + (size_t)chunk_task[c] * kBinWords;
@@ -640,9 +706,12 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
         if (cur.e == 0xFFFFFFFFu) break;
         bins_init(bins, threadIdx.x, blockDim.x);
         __syncthreads();
-        for (uint32_t i = cur.b + threadIdx.x; i < cur.e; i += blockDim.x) {
-            const uint32_t id = ids[i];
-            bins_add(bins, cen[id], plo[id], phi[id], cur.cblo, cur.cbhi);
+        {
+            const float4 cblo = lo4(cur.cblo), ks = scale4(cur.cblo, cur.cbhi);
+            for (uint32_t i = cur.b + threadIdx.x; i < cur.e; i += blockDim.x) {
+                const uint32_t id = ids[i];
+                bins_add(bins, cen[id], plo[id], phi[id], cblo, ks);
+            }
         }
         __syncthreads();
         if (w == 0) sah_wave(bins, &s);
@@ -659,7 +728,8 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
         __syncthreads();
         if (!dec_leaf && dec_axis >= 0) {
             uint32_t lcount = 0, rcount = 0;
-            const int ax = dec_axis;
+            const int   ax  = dec_axis;
+            const float alo = cur.cblo[ax], aks = bin_scale(alo, cur.cbhi[ax]);
             for (uint32_t base = cur.b; base < cur.e; base += blockDim.x) {
                 const uint32_t i  = base + threadIdx.x;
                 const bool     in = i < cur.e;
@@ -669,7 +739,7 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                     id = ids[i];
                     const float4 c  = cen[id];
                     const float  cc = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
-                    f = bin_of(cc, cur.cblo[ax], cur.cbhi[ax]) <= s.bin;
+                    f = bin_at(cc, alo, aks) <= s.bin;
                 }
                 const uint64_t bal  = __ballot(f);
                 const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -712,10 +782,16 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                 }
                 // larger child first so the smaller is processed next: stack depth <= log2(n)+1;
                 // children of at most kWaveTask primitives go to the wave list while it has room
-                const LocalTask ch[2] = {l.e - l.b >= r.e - r.b ? l : r, l.e - l.b >= r.e - r.b ? r : l};
-                for (int j = 0; j < 2; ++j) {
-                    if (ch[j].e - ch[j].b <= kWaveTask && wcount < kMaxWTasks) wtasks[wcount++] = WTask{ch[j].b, ch[j].e, ch[j].slot};
-                    else stack[sp++] = ch[j];
+                auto put = [&](const LocalTask &ch) {
+                    if (ch.e - ch.b <= kWaveTask && wcount < kMaxWTasks) wtasks[wcount++] = WTask{ch.b, ch.e, ch.slot};
+                    else stack[sp++] = ch;
+                };
+                if (l.e - l.b >= r.e - r.b) {
+                    put(l);
+                    put(r);
+                } else {
+                    put(r);
+                    put(l);
                 }
             }
             if (sp == 0) cur.e = 0xFFFFFFFFu;
@@ -763,9 +839,10 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
         for (;;) {
             bins_init(wb, lane, 64);
             wave_sync();
+            const float4 cblo = lo4(c.cblo), ks = scale4(c.cblo, c.cbhi);
             for (uint32_t i = c.b + lane; i < c.e; i += 64) {
                 const uint32_t id = ids[i];
-                bins_add(wb, cen[id], plo[id], phi[id], c.cblo, c.cbhi);
+                bins_add(wb, cen[id], plo[id], phi[id], cblo, ks);
             }
             wave_sync();
             sah_wave(wb, ws);
@@ -779,6 +856,8 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
             }
             if (!leaf && ax >= 0) {
                 const int      bin = ws->bin;
+                const float    alo = pick(c.cblo[0], c.cblo[1], c.cblo[2], ax);
+                const float    aks = bin_scale(alo, pick(c.cbhi[0], c.cbhi[1], c.cbhi[2], ax));
                 uint32_t       lcount = 0, rcount = 0;
                 for (uint32_t base = c.b; base < c.e; base += 64) {
                     const uint32_t i  = base + lane;
@@ -789,7 +868,7 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                         id = ids[i];
                         const float4 cc4 = cen[id];
                         const float  cc  = ax == 0 ? cc4.x : (ax == 1 ? cc4.y : cc4.z);
-                        f = bin_of(cc, c.cblo[ax], c.cbhi[ax]) <= bin;
+                        f = bin_at(cc, alo, aks) <= bin;
                     }
                     const uint64_t bal  = __ballot(f);
                     const uint32_t rank = __popcll(bal & lt);
@@ -803,27 +882,46 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                 wave_sync();
             }
             if (!leaf) {
-                LocalTask l, r;
-                l.b    = c.b;
-                l.e    = c.b + nl;
-                l.slot = c.slot + 1;
-                l.pad  = 0;
-                r.b    = c.b + nl;
-                r.e    = c.e;
-                r.slot = c.slot + 2 * nl;
-                r.pad  = 0;
+                // children [b, b + nl) at slot + 1 and [b + nl, e) at slot + 2 nl; the larger
+                // waits, the smaller (at most half of c) is next: every push halves the size, so
+                // at most log2(kWaveTask) = 7 entries are ever held.  (Field-wise selects keep
+                // the tasks in registers.)
+                const uint32_t lb = c.b, le = c.b + nl, rb = c.b + nl, re = c.e;
+                const uint32_t lsl = c.slot + 1, rsl = c.slot + 2 * nl;
+                const bool     lbig = le - lb >= re - rb;
+                float lcl[3], lch[3], rcl[3], rch[3];
+                const bool byax = ax >= 0;
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    l.cblo[k] = ax >= 0 ? ws->lcblo[k] : c.cblo[k];
-                    l.cbhi[k] = ax >= 0 ? ws->lcbhi[k] : c.cbhi[k];
-                    r.cblo[k] = ax >= 0 ? ws->rcblo[k] : c.cblo[k];
-                    r.cbhi[k] = ax >= 0 ? ws->rcbhi[k] : c.cbhi[k];
+                    // values, not addresses, are selected (a selected address would put c in memory)
+                    const float a0 = ws->lcblo[k], a1 = ws->lcbhi[k], a2 = ws->rcblo[k], a3 = ws->rcbhi[k];
+                    const float c0 = c.cblo[k], c1 = c.cbhi[k];
+                    lcl[k] = byax ? a0 : c0;
+                    lch[k] = byax ? a1 : c1;
+                    rcl[k] = byax ? a2 : c0;
+                    rch[k] = byax ? a3 : c1;
                 }
-                // the larger child waits, the smaller (at most half of c) is next: every push
-                // halves the size, so at most log2(kWaveTask) = 7 entries are ever held
-                const bool lbig = l.e - l.b >= r.e - r.b;
-                if (lane == 0) wst[wsp] = lbig ? l : r;
+                if (lane == 0) {
+                    LocalTask &d = wst[wsp];
+                    d.b    = lbig ? lb : rb;
+                    d.e    = lbig ? le : re;
+                    d.slot = lbig ? lsl : rsl;
+                    d.pad  = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        d.cblo[k] = lbig ? lcl[k] : rcl[k];
+                        d.cbhi[k] = lbig ? lch[k] : rch[k];
+                    }
+                }
                 ++wsp;
-                c = lbig ? r : l;
+                c.b    = lbig ? rb : lb;
+                c.e    = lbig ? re : le;
+                c.slot = lbig ? rsl : lsl;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c.cblo[k] = lbig ? rcl[k] : lcl[k];
+                    c.cbhi[k] = lbig ? rch[k] : lch[k];
+                }
                 wave_sync();
                 continue;
             }
@@ -1117,7 +1215,7 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     HIP_TRY(hipStreamSynchronize(s));
     uint32_t nlarge = hc[0];
     while (nlarge > 0) {
-        k_make_chunks<<<1, 64, 0, s>>>(large_a, nlarge, chunk_task, chunk_begin, task_chunk0, counts + 2);
+        k_make_chunks<<<1, kChunkBlock, 0, s>>>(large_a, nlarge, chunk_task, chunk_begin, task_chunk0, counts + 2);
         k_clear_bins<<<nlarge, 256, 0, s>>>(gbins, nlarge);
         HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
