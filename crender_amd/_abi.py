@@ -156,6 +156,13 @@ class Hits(C.Structure):
                 ("t", "u", "v", "prim_id", "geom_id", "ng_x", "ng_y", "ng_z", "inst_id")]
 
 
+RECORD_FIELDS = ("distance", "prim_id", "geom_id", "material", "u", "v", "n_x", "n_y", "n_z", "p_x", "p_y", "p_z")
+
+
+class Records(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in RECORD_FIELDS]
+
+
 P = C.c_void_p
 SIGNATURES = {
     "crt_last_error": (C.c_char_p, []),
@@ -179,6 +186,7 @@ SIGNATURES = {
     "crt_scene_bvh_export": (C.c_int, [P, P, C.POINTER(C.c_uint64), P, C.POINTER(C.c_uint32)]),
     "crt_intersect": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, C.POINTER(Hits), C.c_int]),
     "crt_intersect_noalloc": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, C.POINTER(Hits)]),
+    "crt_cast_rays": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, C.POINTER(Records), C.c_int]),
     "crt_occluded": (C.c_int, [P, C.POINTER(Rays), C.c_uint32, P, C.c_int]),
     "crt_renderer_create": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(P)]),
     "crt_renderer_destroy": (C.c_int, [P]),

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <limits>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -1118,6 +1119,86 @@ int query(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits, ui
     return CRT_OK;
 }
 
+// scene::cast_ray over a batch (crt_cast_rays).  Host-memory calls stage the rays and reuse
+// the staging buffers for the records: the records kernel reads each element's inputs
+// before it writes that element's outputs.
+int cast_records(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_records *rec, int memory)
+{
+    if (!scene || !rays || !rec) return fail(CRT_ERR_INVALID_ARG, "null argument");
+    if (!scene->committed) return fail(CRT_ERR_STATE, "scene not committed");
+    if (memory != CRT_MEM_HOST && memory != CRT_MEM_DEVICE) return fail(CRT_ERR_INVALID_ARG, "bad memory kind");
+    const float *in[8] = {rays->org_x, rays->org_y, rays->org_z, rays->dir_x, rays->dir_y, rays->dir_z, rays->tnear, rays->tfar};
+    for (int k = 0; k < 6; ++k)
+        if (!in[k] && n) return fail(CRT_ERR_INVALID_ARG, "null ray origin or direction component");
+    if (!rec->distance && n) return fail(CRT_ERR_INVALID_ARG, "records.distance is required");
+    if (n == 0) return CRT_OK;
+    crt_context *ctx = scene->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int st = stage_reserve(ctx, n);
+    if (st) return st;
+    if ((st = ensure_spill(ctx))) return st;
+    hipStream_t s   = ctx->stream;
+    const bool  dev = memory == CRT_MEM_DEVICE;
+    RaySoA      rs;
+    float     **dst[8] = {&rs.ox, &rs.oy, &rs.oz, &rs.dx, &rs.dy, &rs.dz, &rs.tn, &rs.tf};
+    const float dflt[2] = {1e-5f, std::numeric_limits<float>::infinity()};   // model.cpp:22-23
+    for (int k = 0; k < 8; ++k) {
+        if (!in[k]) {
+            HIP_CHECK(launch_fill_f32(ctx->st_ray[k], n, dflt[k - 6], s));
+            *dst[k] = ctx->st_ray[k];
+        } else if (!dev) {
+            HIP_CHECK(hipMemcpyAsync(ctx->st_ray[k], in[k], sizeof(float) * n, hipMemcpyHostToDevice, s));
+            *dst[k] = ctx->st_ray[k];
+        } else {
+            *dst[k] = const_cast<float *>(in[k]);
+        }
+    }
+    HIP_CHECK(hipMemsetAsync(ctx->st_cnt, 0, sizeof(uint32_t) * 32, s));
+    HIP_CHECK(hipMemcpyAsync(ctx->st_cnt, &n, sizeof n, hipMemcpyHostToDevice, s));
+    HitSoA h{};
+    h.t    = ctx->st_t;
+    h.u    = ctx->st_u;
+    h.v    = ctx->st_v;
+    h.tri  = ctx->st_tri;
+    h.inst = ctx->st_inst;
+    TraceQueue q = make_queue(rs, h, ctx->st_cnt, ctx->st_cnt + 8, ctx->st_cnt + 16, true);
+    HIP_CHECK(launch_trace_closest(scene->dev, q, ctx->spill, trace_cfg(ctx), s));
+    // where each field lands on the device, and where it goes on the host
+    struct Field {
+        void  *user;
+        void  *stage;
+        size_t elem;
+    };
+    Field f[12] = {{rec->distance, ctx->st_t, 4},     {rec->prim_id, ctx->st_prim, 4}, {rec->geom_id, ctx->st_geom, 4},
+                   {rec->material, ctx->st_inst_out, 4}, {rec->u, ctx->st_u, 4},        {rec->v, ctx->st_v, 4},
+                   {rec->n_x, ctx->st_ng[0], 4},      {rec->n_y, ctx->st_ng[1], 4},    {rec->n_z, ctx->st_ng[2], 4},
+                   {rec->p_x, ctx->st_ray[0], 4},     {rec->p_y, ctx->st_ray[1], 4},   {rec->p_z, ctx->st_ray[2], 4}};
+    void *o[12];
+    for (int k = 0; k < 12; ++k) o[k] = f[k].user ? (dev ? f[k].user : f[k].stage) : nullptr;
+    RecordsDev out;
+    out.dist = static_cast<float *>(o[0]);
+    out.prim = static_cast<uint32_t *>(o[1]);
+    out.geom = static_cast<uint32_t *>(o[2]);
+    out.mat  = static_cast<uint32_t *>(o[3]);
+    out.u    = static_cast<float *>(o[4]);
+    out.v    = static_cast<float *>(o[5]);
+    out.nx   = static_cast<float *>(o[6]);
+    out.ny   = static_cast<float *>(o[7]);
+    out.nz   = static_cast<float *>(o[8]);
+    out.px   = static_cast<float *>(o[9]);
+    out.py   = static_cast<float *>(o[10]);
+    out.pz   = static_cast<float *>(o[11]);
+    HIP_CHECK(launch_hit_records(scene->dev, rs, h, n, out, s));
+    if (!dev)
+        for (int k = 0; k < 12; ++k)
+            if (f[k].user) HIP_CHECK(hipMemcpyAsync(f[k].user, f[k].stage, f[k].elem * n, hipMemcpyDeviceToHost, s));
+    uint32_t err = 0;
+    HIP_CHECK(hipMemcpyAsync(&err, ctx->st_cnt + 16, sizeof err, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    return CRT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1866,6 +1947,11 @@ int crt_intersect_noalloc(crt_scene *scene, const crt_rays *rays, uint32_t n, cr
     HIP_CHECK(hipStreamSynchronize(s));
     if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
     return CRT_OK;
+}
+
+int crt_cast_rays(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_records *records, int memory)
+{
+    return cast_records(scene, rays, n, records, memory);
 }
 
 int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory)

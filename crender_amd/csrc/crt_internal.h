@@ -378,6 +378,15 @@ hipError_t launch_shade_multi(const BatchDev &bd, const PathBuffers &b, uint32_t
 hipError_t launch_shadow_alpha_step_multi(const BatchDev &bd, const PathBuffers &b, hipStream_t s);
 hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uint32_t max_acc, hipStream_t s);
 hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s);
+// Intersection records (crt_records): every output pointer may be null except dist.
+struct RecordsDev {
+    float    *dist;
+    uint32_t *prim, *geom, *mat;
+    float    *u, *v, *nx, *ny, *nz, *px, *py, *pz;
+};
+hipError_t launch_hit_records(const SceneDev &sc, const RaySoA &rs, const HitSoA &h, uint32_t n, const RecordsDev &out,
+                              hipStream_t s);
+hipError_t launch_fill_f32(float *p, uint32_t n, float v, hipStream_t s);
 hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, uint32_t *prim, uint32_t *geom,
                               float *ngx, float *ngy, float *ngz, uint32_t *inst, hipStream_t s);
 

@@ -761,6 +761,54 @@ __global__ void k_hits_to_api(SceneDev sc, HitSoA h, uint32_t n, uint32_t *prim,
     if (ngz) ngz[i] = ng.z;
 }
 
+// model.cpp:27-47, and model.cpp:114-118 for a hit on an instance: the intersection record
+// of each traced ray.  Every input of element i is read before any output is written, so the
+// outputs may alias the inputs of the same element (host-staged calls reuse the buffers).
+__global__ void k_hit_records(SceneDev sc, RaySoA rs, HitSoA h, uint32_t n, RecordsDev out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tri  = h.tri[i];
+    const f3       o    = mk3(rs.ox[i], rs.oy[i], rs.oz[i]);
+    const f3       d    = mk3(rs.dx[i], rs.dy[i], rs.dz[i]);
+    const float    t    = h.t[i], bu = h.u[i], bv = h.v[i];
+    const uint32_t inst = sc.instanced && h.inst ? h.inst[i] : kInvalidRef;
+    float    dist = __builtin_huge_valf(), u = 0.0f, v = 0.0f;
+    uint32_t pid = kInvalidRef, gid = kInvalidRef, mat = kInvalidRef;
+    f3       nrm = mk3(0, 0, 0), pt = mk3(0, 0, 0);
+    if (tri != kInvalidRef) {
+        const HitRec r = hit_record(sc, o, d, t, tri, bu, bv, inst, false);
+        dist = r.dist;
+        pt   = r.point;
+        nrm  = r.normal;
+        u    = r.u;
+        v    = r.v;
+        mat  = sc.mat_idx[r.prim];
+        gid  = 0;
+        for (uint32_t m = 1; m < sc.nmodels; ++m)
+            if (r.prim >= sc.model_first[m]) gid = m;
+        pid = r.prim - sc.model_first[gid];
+    }
+    out.dist[i] = dist;
+    if (out.prim) out.prim[i] = pid;
+    if (out.geom) out.geom[i] = gid;
+    if (out.mat) out.mat[i] = mat;
+    if (out.u) out.u[i] = u;
+    if (out.v) out.v[i] = v;
+    if (out.nx) out.nx[i] = nrm.x;
+    if (out.ny) out.ny[i] = nrm.y;
+    if (out.nz) out.nz[i] = nrm.z;
+    if (out.px) out.px[i] = pt.x;
+    if (out.py) out.py[i] = pt.y;
+    if (out.pz) out.pz[i] = pt.z;
+}
+
+__global__ void k_fill_f32(float *p, uint32_t n, float v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // rad[path] += throughput * max(prediction, 0) for every cache query
 __global__ __launch_bounds__(256) void k_nrc_apply(PathBuffers b)
 {
@@ -960,6 +1008,19 @@ hipError_t launch_hits_to_api(const SceneDev &sc, const HitSoA &h, uint32_t n, u
                               float *ngx, float *ngy, float *ngz, uint32_t *inst, hipStream_t s)
 {
     if (n) k_hits_to_api<<<blocks(n), 256, 0, s>>>(sc, h, n, prim, geom, ngx, ngy, ngz, inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_hit_records(const SceneDev &sc, const RaySoA &rs, const HitSoA &h, uint32_t n, const RecordsDev &out,
+                              hipStream_t s)
+{
+    if (n) k_hit_records<<<blocks(n), 256, 0, s>>>(sc, rs, h, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float *p, uint32_t n, float v, hipStream_t s)
+{
+    if (n) k_fill_f32<<<blocks(n), 256, 0, s>>>(p, n, v);
     return hipGetLastError();
 }
 

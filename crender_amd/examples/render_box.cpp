@@ -78,6 +78,11 @@ int main(int argc, char **argv)
         const crt_bvh_info bi = sc.bvh_info();
         std::printf("bvh: width %u, %u nodes, depth %u, built in %.3f ms\n", bi.width, bi.node_count, bi.depth,
                     bi.build_ms);
+        const float eye[3] = {0.0f, 1.0f, -3.4f}, ahead[3] = {0.0f, 0.0f, 2.0f};
+        const crt::intersection_record hit = sc.cast_ray(eye, ahead);   // the back wall at z = 1
+        std::printf("cast_ray: distance %.4f material %u normal %.1f %.1f %.1f point %.2f %.2f %.2f\n", hit.distance,
+                    hit.material, hit.normal[0], hit.normal[1], hit.normal[2], hit.intersection_point[0],
+                    hit.intersection_point[1], hit.intersection_point[2]);
 
         crt::renderer r(ctx, sc, 256, 256, 8);   // starts rendering, like cr::renderer
         r.update([&] {

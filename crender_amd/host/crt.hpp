@@ -66,13 +66,16 @@ struct model_data {
     std::vector<uint32_t>     texture_indices; // 3 per triangle or empty
 };
 
-struct hit {
-    float    distance = __builtin_huge_valf();   // intersection_record::distance (ray.h:12-20)
+// cr::ray::intersection_record (ray.h:12-20); the material is an index into the scene's
+// material table instead of a pointer.
+struct intersection_record {
     uint32_t prim_id  = 0xFFFFFFFFu;
-    uint32_t geom_id  = 0xFFFFFFFFu;
-    uint32_t inst_id  = 0xFFFFFFFFu;
-    float    u = 0.0f, v = 0.0f;
-    float    ng[3] = {0, 0, 0};
+    float    distance = __builtin_huge_valf();
+    uint32_t material = 0xFFFFFFFFu;
+    float    uv[2]    = {0, 0};
+    float    normal[3] = {0, 0, 0};
+    float    intersection_point[3] = {0, 0, 0};
+    uint32_t geom_id  = 0xFFFFFFFFu;   // model of the hit
 };
 
 class scene {
@@ -144,16 +147,30 @@ public:
         return i;
     }
 
-    // scene::cast_ray (scene.cpp:79-98) for one ray; batches go through intersect()
-    hit cast_ray(const float origin[3], const float direction[3])
+    // scene::cast_ray (scene.cpp:79-98) for one ray; batches go through cast_rays() or, for
+    // Embree-style hit records, intersect()
+    intersection_record cast_ray(const float origin[3], const float direction[3])
     {
-        const float tn = 1e-5f, tf = __builtin_huge_valf();
-        crt_rays r{&origin[0], &origin[1], &origin[2], &direction[0], &direction[1], &direction[2], &tn, &tf};
-        hit      out;
-        crt_hits h{&out.distance, &out.u,     &out.v,     &out.prim_id, &out.geom_id,
-                   &out.ng[0],    &out.ng[1], &out.ng[2], &out.inst_id};
-        check(crt_intersect(h_, &r, 1, &h, CRT_MEM_HOST));
+        crt_rays r{&origin[0], &origin[1], &origin[2], &direction[0], &direction[1], &direction[2], nullptr, nullptr};
+        intersection_record out;
+        crt_records         rec{&out.distance,
+                        &out.prim_id,
+                        &out.geom_id,
+                        &out.material,
+                        &out.uv[0],
+                        &out.uv[1],
+                        &out.normal[0],
+                        &out.normal[1],
+                        &out.normal[2],
+                        &out.intersection_point[0],
+                        &out.intersection_point[1],
+                        &out.intersection_point[2]};
+        check(crt_cast_rays(h_, &r, 1, &rec, CRT_MEM_HOST));
         return out;
+    }
+    void cast_rays(const crt_rays &rays, uint32_t n, crt_records &records, int memory = CRT_MEM_HOST)
+    {
+        check(crt_cast_rays(h_, &rays, n, &records, memory));
     }
     void intersect(const crt_rays &rays, uint32_t n, crt_hits &hits, int memory = CRT_MEM_HOST)
     {

@@ -231,6 +231,34 @@ class Scene:
         check(self._L.crt_intersect(self._h, C.byref(rays), n, C.byref(hits), A.CRT_MEM_HOST))
         return out
 
+    def cast_ray_records(self, org, dirs, tnear=None, tfar=None) -> dict:
+        """scene::cast_ray (scene.cpp:79-98) over a batch through crt_cast_rays: the
+        intersection record of each ray (distance, prim_id, geom_id, material, u, v,
+        n_x..n_z, p_x..p_z).  tnear / tfar default to the reference's 1e-5 and +inf."""
+        org = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = org.shape[0]
+        comps = [np.ascontiguousarray(org[:, k]) for k in range(3)] + [np.ascontiguousarray(dirs[:, k]) for k in range(3)]
+        lim = []
+        for v in (tnear, tfar):
+            lim.append(None if v is None else np.ascontiguousarray(
+                np.broadcast_to(np.float32(v), (n,)) if np.isscalar(v) else v, dtype=np.float32))
+        rays = A.Rays(*[vptr(a) for a in comps], *[vptr(a) if a is not None else None for a in lim])
+        out = {k: np.empty(n, np.uint32 if k in ("prim_id", "geom_id", "material") else np.float32)
+               for k in A.RECORD_FIELDS}
+        rec = A.Records(*[vptr(out[k]) for k in A.RECORD_FIELDS])
+        check(self._L.crt_cast_rays(self._h, C.byref(rays), n, C.byref(rec), A.CRT_MEM_HOST))
+        return out
+
+    def cast_ray_records_device(self, rays: dict, records: dict, n: int) -> None:
+        """crt_cast_rays on device memory: `rays` maps org_x .. dir_z (and optionally tnear,
+        tfar) and `records` maps the crt_records fields to device addresses (ints; missing
+        fields are not written, distance is required)."""
+        rk = ("org_x", "org_y", "org_z", "dir_x", "dir_y", "dir_z", "tnear", "tfar")
+        r = A.Rays(*[C.c_void_p(rays[k]) if rays.get(k) else None for k in rk])
+        rec = A.Records(*[C.c_void_p(records[k]) if records.get(k) else None for k in A.RECORD_FIELDS])
+        check(self._L.crt_cast_rays(self._h, C.byref(r), n, C.byref(rec), A.CRT_MEM_DEVICE))
+
     def cast_rays_device(self, rays: dict, hits: dict, n: int, noalloc: bool = False) -> None:
         """crt_intersect on device memory (or crt_intersect_noalloc): `rays` maps org_x ..
         dir_z, tnear, tfar and `hits` maps t, u, v, prim_id, geom_id, ng_x, ng_y, ng_z, inst_id

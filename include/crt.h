@@ -193,6 +193,18 @@ typedef struct crt_hits {     /* SoA; any pointer except t/prim_id may be NULL *
     uint32_t *inst_id;        /* instance of the model (crt_scene_set_instances order), 0xFFFFFFFF on miss */
 } crt_hits;
 
+/* cr::ray::intersection_record (ray.h:12-20) for a batch of rays.  SoA; any pointer except
+ * distance may be NULL.  On a miss: distance +inf, the IDs 0xFFFFFFFF, the rest 0. */
+typedef struct crt_records {
+    float    *distance;           /* |P - origin| (model.cpp:118 for instances) */
+    uint32_t *prim_id;            /* primitive within its model */
+    uint32_t *geom_id;            /* model index */
+    uint32_t *material;           /* index into the scene's material table (materials.indices[primID]) */
+    float    *u, *v;              /* texture coordinates interpolated at the hit (rtcInterpolate0); 0 without uv */
+    float    *n_x, *n_y, *n_z;    /* normalize(Ng); object space for a hit on an instance (model.cpp:36-37) */
+    float    *p_x, *p_y, *p_z;    /* intersection point in world space */
+} crt_records;
+
 typedef struct crt_context  crt_context;
 typedef struct crt_scene    crt_scene;
 typedef struct crt_renderer crt_renderer;
@@ -247,6 +259,11 @@ int crt_intersect(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *
  * crt_intersect's with CRT_MEM_DEVICE. */
 #define CRT_NOALLOC_MAX_RAYS 1048576u
 int crt_intersect_noalloc(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_hits *hits);
+/* scene::cast_ray (scene.cpp:79-98) for a batch: the closest hit of each ray as the
+ * intersection record model.cpp:27-47 fills in.  Each ray is traced along its normalised
+ * direction, as the per-instance transform does (model.cpp:107-110); tnear / tfar default to
+ * the reference's 1e-5 and +inf (model.cpp:22-23) where rays->tnear / rays->tfar are NULL. */
+int crt_cast_rays(crt_scene *scene, const crt_rays *rays, uint32_t n, crt_records *records, int memory);
 /* any hit in (tnear, tfar]; occluded[i] = 1/0 */
 int crt_occluded(crt_scene *scene, const crt_rays *rays, uint32_t n, uint8_t *occluded, int memory);
 

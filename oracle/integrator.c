@@ -187,6 +187,34 @@ int orc_render_f64(const orc_scene *scene, const orc_settings *s, double *raw, f
     return render_common(scene, s, raw, 1, albedo, normal, depth, stats);
 }
 
+/* scene::cast_ray (scene.cpp:79-98) for a batch of rays in FP32, through the integrator's own
+ * cast (model.cpp:27-47): org / dir / uv / nrm / pt interleaved xyz (uv: xy).  A miss gives
+ * distance +inf, prim and material -1 and zeros elsewhere. */
+void orc_cast_records_f32(const orc_scene *scene, uint32_t n, const float *org, const float *dir, float *dist,
+                          int32_t *prim, int32_t *mat, float *uv, float *nrm, float *pt)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        hit_f32       h;
+        const vec_f32 o = mk_f32(org[3 * i], org[3 * i + 1], org[3 * i + 2]);
+        const vec_f32 d = mk_f32(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        if (!cast_f32(scene, o, d, &h)) {
+            dist[i] = INFINITY;
+            prim[i] = -1;
+            mat[i]  = -1;
+            uv[2 * i] = uv[2 * i + 1] = 0.0f;
+            for (int k = 0; k < 3; ++k) nrm[3 * i + k] = pt[3 * i + k] = 0.0f;
+            continue;
+        }
+        dist[i]       = h.t;
+        prim[i]       = h.prim;
+        mat[i]        = (int32_t)scene->mat_idx[h.prim];
+        uv[2 * i]     = h.uvx;
+        uv[2 * i + 1] = h.uvy;
+        nrm[3 * i] = h.normal.x, nrm[3 * i + 1] = h.normal.y, nrm[3 * i + 2] = h.normal.z;
+        pt[3 * i] = h.point.x, pt[3 * i + 1] = h.point.y, pt[3 * i + 2] = h.point.z;
+    }
+}
+
 /* GGX pieces in FP64 for the sampling / quadrature checks (sampling.h:93-140). */
 double orc_ggx_d(double noh, double alpha) { return ggx_d_f64(noh, alpha); }
 double orc_ggx_vis(double nov, double nol, double alpha) { return ggx_vis_f64(nov, nol, alpha); }

@@ -172,6 +172,11 @@ int orc_render_f32(const orc_scene *scene, const orc_settings *s, float *raw,
 int orc_render_f64(const orc_scene *scene, const orc_settings *s, double *raw,
                    float *albedo, float *normal, float *depth, orc_stats *stats);
 
+/* scene::cast_ray (scene.cpp:79-98) for n rays, FP32: distance, primID, material index,
+ * interpolated uv (n x 2), normalize(Ng) and the point (n x 3); -1 / +inf on a miss. */
+void orc_cast_records_f32(const orc_scene *scene, uint32_t n, const float *org, const float *dir, float *dist,
+                          int32_t *prim, int32_t *mat, float *uv, float *nrm, float *pt);
+
 /* One camera ray (camera.cpp:14-39) for jittered pixel coords x,y in [0,1]. */
 void orc_camera_ray(const orc_camera *cam, float x, float y, float aspect, float o[3], float d[3]);
 

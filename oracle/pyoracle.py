@@ -121,6 +121,8 @@ def lib() -> C.CDLL:
             getattr(L, name).argtypes = [C.POINTER(Scene), C.c_uint32, C.c_int32, fp, fp, C.c_float, C.c_float, fp,
                                          fp, fp, C.POINTER(ScatterOut)]
         L.orc_sky_f32.argtypes = [C.POINTER(Scene), fp, fp]
+        ip = C.POINTER(C.c_int32)
+        L.orc_cast_records_f32.argtypes = [C.POINTER(Scene), C.c_uint32, fp, fp, fp, ip, ip, fp, fp, fp]
         _L = L
     return _L
 
@@ -293,6 +295,23 @@ def render(spec, res=None, spp=1, max_bounces=None, rr_start=None, mode=None, se
     return {"raw": raw, "albedo": aov[0], "normal": aov[1], "depth": aov[2],
             "stats": {"ext_rays": stats.ext_rays, "shadow_rays": stats.shadow_rays,
                       "ref_segments": stats.ref_segments, "seconds": stats.seconds}}
+
+
+def cast_records(spec, org, dirs, trav=BRUTE64, bvh: Bvh | None = None) -> dict:
+    """scene::cast_ray (scene.cpp:79-98) per ray in FP32 through the integrator's own cast:
+    dict(distance, prim_id, material (-1 / +inf on a miss), uv (n, 2), normal, point (n, 3))."""
+    org = np.ascontiguousarray(org, dtype=np.float32).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    n = org.shape[0]
+    b = SceneBinding(spec, trav=trav, bvh=bvh)
+    out = {"distance": np.empty(n, np.float32), "prim_id": np.empty(n, np.int32), "material": np.empty(n, np.int32),
+           "uv": np.empty((n, 2), np.float32), "normal": np.empty((n, 3), np.float32),
+           "point": np.empty((n, 3), np.float32)}
+    ip = C.POINTER(C.c_int32)
+    lib().orc_cast_records_f32(C.byref(b.c), n, _f(org), _f(dirs), _f(out["distance"]),
+                               out["prim_id"].ctypes.data_as(ip), out["material"].ctypes.data_as(ip),
+                               _f(out["uv"]), _f(out["normal"]), _f(out["point"]))
+    return out
 
 
 def display(raw: np.ndarray, n: int) -> np.ndarray:

@@ -270,5 +270,6 @@ def test_cpp_host_example_runs():
     out = subprocess.run([str(exe), "8"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "samples 8," in out.stdout
+    assert "cast_ray: distance 4.4000 material 2 normal 0.0 0.0 -1.0 point 0.00 1.00 1.00" in out.stdout
     mean = float(out.stdout.strip().split("mean display ")[-1])
     assert 0.05 < mean < 1.0

@@ -6,49 +6,15 @@ results for the same device rays; plus the size limit, and a two-level (instance
 Device buffers come from the HIP runtime through ctypes (the test allocates; the entry
 point does not).
 """
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from crender_amd import Context, CrtError, lib, scene_from_spec, scenes
+from _devmem import DeviceArrays
+from crender_amd import Context, CrtError, scene_from_spec, scenes
 
 pytestmark = pytest.mark.gpu
 
 MAX_RAYS = 1 << 20
-H2D, D2H = 1, 2
-
-
-class DeviceArrays:
-    """Device buffers from the HIP runtime the library itself uses (the test allocates; the
-    entry point under test does not)."""
-
-    def __init__(self):
-        lib()   # libcrt.so first: the runtime it links is the one looked up here by soname
-        self.hip = C.CDLL("libamdhip64.so.7")
-        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.hip.hipFree.argtypes = [C.c_void_p]
-        self.ptrs = []
-
-    def put(self, a):
-        a = np.ascontiguousarray(a)
-        p = C.c_void_p()
-        assert self.hip.hipMalloc(C.byref(p), max(a.nbytes, 4)) == 0
-        assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, H2D) == 0
-        self.ptrs.append(p)
-        return p.value
-
-    def get(self, ptr, n, dtype):
-        out = np.empty(n, dtype)
-        assert self.hip.hipMemcpy(out.ctypes.data, C.c_void_p(ptr), out.nbytes, D2H) == 0
-        return out
-
-    def free(self):
-        for p in self.ptrs:
-            self.hip.hipFree(p)
-        self.ptrs = []
-
 
 HIT_TYPES = dict(t=np.float32, u=np.float32, v=np.float32, prim_id=np.int32, geom_id=np.int32, ng_x=np.float32,
                  ng_y=np.float32, ng_z=np.float32, inst_id=np.int32)
