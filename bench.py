@@ -13,12 +13,14 @@ kernel's mean launch time measured with HIP events on the library's stream.
 ``cpu_baseline`` times the CPU oracle's restatement of the reference loop (FP32
 binned-SAH BVH, one task per row, barrier per pass) on a band of rows of the same frame.
 
-Two more measurements ride on the same line (skip with --no-nrc / --no-batch):
+Three more measurements ride on the same line (skip with --no-nrc / --no-batch / --no-terrain):
 ``nrc`` -- the metric's second half: image MSE against a 4096-spp plain render of the same
 scene at 1920x1080 (pbr, depth 16), plain tracer vs neural radiance cache, as MSE-vs-time
 curves; ``time_ratio`` = the cache's time to reach the plain tracer's MSE at T over T.
 ``batch`` -- 64 independent scenes rendered by one batched call after the autotuner:
 aggregate Mrays/s and per-scene latency.
+``terrain`` -- the 14.8M-triangle terrain with objects at 1920x1080, 64 spp, pbr, sun + sky,
+depth 8: Mrays/s and ms per spp.
 """
 from __future__ import annotations
 
@@ -256,6 +258,44 @@ def traversal_counts(spec, res, bounces, args) -> dict:
             "leaf_loop_lanes": st["leaf_visits"] / max(1, 64 * st["wave_leaf_steps"])}
 
 
+def terrain_bench(args) -> dict:
+    """The terrain with objects (2047^2 x 2 terrain triangles + 20k icospheres, 14.8M
+    triangles), 1920x1080, pbr, sun + sky, depth 8, Russian roulette from bounce 3: a 64-spp
+    render after an 8-spp warm-up, geometry resident in HBM."""
+    from crender_amd import Context, Renderer, scene_from_spec, scenes
+
+    spec = scenes.terrain()
+    W, H = spec.res
+    ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
+                  trace_stack=args.stack, wavefront_paths=args.wavefront_paths)
+    sc = scene_from_spec(ctx, spec)
+    info = sc.bvh_info()
+    r = Renderer(ctx, sc, W, H, spec.max_bounces)
+    r.set_resolution(W, H)
+    r.set_options(mode=spec.mode, rr_start=spec.rr_start, seed=0)
+    r.render(8)
+    s0 = r.current_stats()
+    t0 = time.perf_counter()
+    r.render(spec.spp)
+    wall = time.perf_counter() - t0
+    s1 = r.current_stats()
+    r.close()
+    sc.close()
+    ctx.close()
+    ext = s1["extension_rays"] - s0["extension_rays"]
+    shadow = s1["shadow_rays"] - s0["shadow_rays"]
+    return {
+        "workload": f"terrain with objects, {spec.ntri} tris, {W}x{H}, {spec.spp} spp, pbr, sun + sky, "
+                    f"depth {spec.max_bounces}, roulette from bounce {spec.rr_start}",
+        "mrays_per_s": (ext + shadow) / wall / 1e6,
+        "ms_per_spp": 1e3 * wall / spec.spp,
+        "extension_rays": ext,
+        "shadow_rays": shadow,
+        "bvh_build_ms": info["build_ms"],
+        "bvh_mtris_per_s": spec.ntri / (info["build_ms"] * 1e-3) / 1e6,
+    }
+
+
 def batch_bench(seed: int = 7) -> dict:
     """64 independent scenes (soup / terrain, 1k-500k tris, 128^2-1024^2, 1-64 spp, depth 6)
     rendered by crt_render_batch after crt_autotune."""
@@ -305,6 +345,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work of the 3 timed baseline runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-rendering measurement")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the 14.8M-triangle terrain measurement")
     ap.add_argument("--no-nrc", action="store_true", help="skip the neural radiance cache equal-time MSE")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -352,6 +393,7 @@ def main(argv=None) -> int:
     segs = s1["total_rays"] - s0["total_rays"]
     rays = ext + shadow
     ms_ext = s1["ms_extend"] - s0["ms_extend"]
+    ms_busy = s1["ms_extend_busy"] - s0["ms_extend_busy"]
     launches = s1["extend_launches"] - s0["extend_launches"]
     gpu_ms = s1["ms_render"] - s0["ms_render"]
     achieved = BYTES_PER_RAY * ext / (ms_ext * 1e-3) / 1e9 if ms_ext > 0 else 0.0
@@ -374,7 +416,7 @@ def main(argv=None) -> int:
                         f"(depth 4), compat estimator, sun off, constant sky",
             "bvh_width": info["width"],
             "waves_per_cu": args.waves_per_cu or 8,
-            "trace_stack": args.stack or 32,
+            "trace_stack": args.stack or 16,
             # sample passes per wavefront: floor(wavefront paths / (W*H)) in [1, 8], 2^24 paths by default
             "passes_per_wavefront": max(1, min(8, (args.wavefront_paths or (1 << 24)) // (res[0] * res[1]))),
             "wavefronts_in_flight": 2,
@@ -393,6 +435,11 @@ def main(argv=None) -> int:
             "kernel": "k_trace (closest hit)",
             "bytes_per_launch": BYTES_PER_RAY * ext / max(launches, 1),
             "mean_launch_ms": ms_ext / max(launches, 1),
+            # two wavefronts are in flight on two streams, so closest-hit launches overlap and
+            # each one's span covers time shared with the other: the same bytes over the time
+            # during which any closest-hit launch ran
+            "busy_ms": ms_busy,
+            "achieved_busy": BYTES_PER_RAY * ext / (ms_busy * 1e-3) / 1e9 if ms_busy > 0 else 0.0,
         },
         "rays": {"extension": ext, "shadow": shadow, "ref_counting_segments": segs},
         "samples_per_s": res[0] * res[1] * args.steps / wall,
@@ -408,6 +455,8 @@ def main(argv=None) -> int:
     sc.close()
     ctx.close()
     out["traversal"] = traversal_counts(spec, res, bounces, args)
+    if not args.no_terrain:
+        out["terrain"] = terrain_bench(args)
     if not args.no_batch:
         out["batch"] = batch_bench()
     if not args.no_cpu_baseline:

@@ -94,7 +94,8 @@ class Stats(C.Structure):
                 ("extend_launches", C.c_uint64), ("ms_nrc_train", C.c_double), ("nrc_loss", C.c_float),
                 ("node_visits", C.c_uint64), ("leaf_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("wave_node_steps", C.c_uint64), ("wave_leaf_steps", C.c_uint64),
-                ("nrc_detached", C.c_uint32), ("nrc_discarded_passes", C.c_uint32)]
+                ("nrc_detached", C.c_uint32), ("nrc_discarded_passes", C.c_uint32),
+                ("ms_extend_busy", C.c_double)]
 
 
 class BvhInfo(C.Structure):

@@ -209,6 +209,12 @@ struct crt_renderer {
                                        // the cache-training target count [2 + set]
     double ms_phase[6] = {0, 0, 0, 0, 0, 0};  // generate, extend, shade, shadow, accumulate, render
     uint64_t extend_launches = 0;
+    // closest-hit launch intervals of the current render_passes call (ms after ev_origin,
+    // recorded on the first stream before anything else of the call), merged into
+    // ms_extend_busy when the call ends
+    hipEvent_t                           ev_origin = nullptr;
+    std::vector<std::pair<float, float>> ext_iv;
+    double                               ms_extend_busy = 0.0;
     // neural radiance cache
     crt_nrc               *nrc = nullptr;
     crt_nrc_render_options nopt{};
@@ -811,6 +817,10 @@ int finish_pass(crt_renderer *r, PassTicket &t)
         int prev = t.ev_gen;
         for (uint32_t i = 0; i < t.timed; ++i) {
             HIP_CHECK(add(1, prev, t.ev_ext[i]));
+            float a = 0, b = 0;
+            if (hipEventElapsedTime(&a, r->ev_origin, ev[prev]) == hipSuccess &&
+                hipEventElapsedTime(&b, r->ev_origin, ev[t.ev_ext[i]]) == hipSuccess)
+                r->ext_iv.emplace_back(a, b);
             HIP_CHECK(add(2, t.ev_ext[i], t.ev_shade[i]));
             prev = t.ev_shade[i];
             if (t.ev_shadow[i] >= 0) {
@@ -858,14 +868,23 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
     if (r->needs_reset)
         return fail(CRT_ERR_STATE, "a previous pass failed; reset (or start) the renderer before rendering again");
     r->warm_enq = r->nrc_warm_done;
+    r->ext_iv.clear();
+    if (ctx->opt.flags & CRT_FLAG_PHASE_TIMING) HIP_CHECK(hipEventRecord(r->ev_origin, ctx->stream));
     PassTicket inflight;
     bool       busy   = false;
     uint32_t   queued = 0;
     int        st     = CRT_OK;
     int        next_set = 0;
+    // Balanced wavefronts, an even number of them when two are in flight and there is more
+    // than one: 20 passes at 8 per wavefront run as 5 + 5 + 5 + 5 rather than 8 + 8 + 4, so
+    // that the last wavefront does not run alone for long.
+    uint32_t waves = (n + r->kmax - 1) / r->kmax;
+    if (r->two_sets && waves > 1 && (waves & 1u)) ++waves;
     while (queued < n && st == CRT_OK && !r->stop.load()) {
         PassTicket     t;
-        const uint32_t k     = std::min(r->kmax, n - queued);
+        const uint32_t rem   = n - queued;
+        const uint32_t left  = std::max<uint32_t>(1u, std::min(waves, rem));
+        const uint32_t k     = std::min(r->kmax, (rem + left - 1) / left);
         const int      bset  = r->two_sets ? next_set : 0;
         st = enqueue_trace(r, r->samples.load() + (busy ? inflight.nsamp : 0u), k, next_set, bset, t);
         if (st == CRT_OK && busy) {
@@ -881,6 +900,7 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
         if (st == CRT_OK) st = enqueue_tail(r, t);
         if (st != CRT_OK) break;
         if (!t.discard) queued += t.units;
+        if (waves > 1) --waves;
         inflight = t;
         busy     = true;
         next_set ^= 1;
@@ -895,6 +915,21 @@ int render_passes(crt_renderer *r, uint32_t n, uint32_t &done)
         (void)hipStreamSynchronize(ctx->stream2);
         r->needs_reset = true;   // accumulation and sample count may disagree
     }
+    // length of the union of the closest-hit intervals of this call
+    std::sort(r->ext_iv.begin(), r->ext_iv.end());
+    double span = 0.0, lo = -std::numeric_limits<double>::infinity(), hi = lo;
+    for (const auto &iv : r->ext_iv) {
+        if (iv.first > hi) {
+            if (hi > lo) span += hi - lo;
+            lo = iv.first;
+            hi = iv.second;
+        } else if (iv.second > hi) {
+            hi = iv.second;
+        }
+    }
+    if (hi > lo) span += hi - lo;
+    r->ms_extend_busy += span;
+    r->ext_iv.clear();
     if (r->nrc) {
         const int sw = nrc_wait_training(r->nrc);
         if (st == CRT_OK) st = sw;
@@ -1012,6 +1047,7 @@ int reset_accum(crt_renderer *r)
     r->needs_reset = false;
     for (double &m : r->ms_phase) m = 0.0;
     r->extend_launches = 0;
+    r->ms_extend_busy  = 0.0;
     r->ms_nrc_train    = 0.0;
     r->t0 = std::chrono::steady_clock::now();
     return CRT_OK;
@@ -1965,6 +2001,7 @@ static void destroy_renderer_events(crt_renderer *r)
     for (auto &set : r->ev)
         for (hipEvent_t e : set)
             if (e) (void)hipEventDestroy(e);
+    if (r->ev_origin) (void)hipEventDestroy(r->ev_origin);
     for (int k = 0; k < 2; ++k) {
         if (r->ev_train[k]) (void)hipEventDestroy(r->ev_train[k]);
         if (r->ev_accdone[k]) (void)hipEventDestroy(r->ev_accdone[k]);
@@ -1993,6 +2030,7 @@ int crt_renderer_create(crt_context *ctx, crt_scene *scene, uint32_t res_x, uint
     for (auto &set : r->ev)
         for (hipEvent_t &e : set)
             if (st == CRT_OK && hipEventCreate(&e) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
+    if (st == CRT_OK && hipEventCreate(&r->ev_origin) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
     for (int k = 0; k < 2; ++k) {
         if (st == CRT_OK && hipEventCreate(&r->ev_train[k]) != hipSuccess) st = fail(CRT_ERR_HIP, "hipEventCreate");
         if (st == CRT_OK && hipEventCreateWithFlags(&r->ev_accdone[k], hipEventDisableTiming) != hipSuccess)
@@ -2249,6 +2287,7 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
     stats->ms_accumulate = r->ms_phase[4];
     stats->ms_render     = r->ms_phase[5];
     stats->extend_launches = r->extend_launches;
+    stats->ms_extend_busy  = r->ms_extend_busy;
     if (r->nrc) harvest_train_time(r, true);
     stats->ms_nrc_train    = r->ms_nrc_train;
     stats->nrc_detached    = r->nrc_failed ? 1u : 0u;

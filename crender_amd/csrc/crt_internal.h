@@ -232,7 +232,10 @@ uint32_t trace_grid(int device, uint32_t waves_per_cu);
 uint64_t trace_spill_bytes(uint32_t grid);
 constexpr uint32_t kTraceBlock = 256;
 constexpr uint32_t kXcds       = 8;
-constexpr uint32_t kStackLds   = 32;    // default LDS stack entries per lane
+// default LDS stack entries per lane: 16 entries (34.3 KB per 256-lane block) let the
+// traversals of both wavefronts in flight hold their blocks on a CU at once (4 x 34.3 KB of
+// 160 KB), where 32 entries (67.6 KB per block) admit only one traversal's two blocks
+constexpr uint32_t kStackLds   = 16;
 constexpr uint32_t kSpillDepth = 256;   // global spill entries per lane (multiple of kStackLds)
 
 // ---- allocation-free queries (noalloc.hip) ---------------------------------------------

@@ -121,7 +121,7 @@ typedef struct crt_options {
     uint32_t max_leaf;      /* 0 = default (4) */
     uint32_t trace_waves_per_cu; /* 0 = default persistent occupancy (8 waves per CU) */
     uint32_t flags;         /* CRT_FLAG_* */
-    uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 32), 8, 16 or 32 */
+    uint32_t trace_stack;   /* LDS stack entries per traversal lane: 0 (= 16), 8, 16 or 32 */
     uint32_t wavefront_paths; /* camera paths per renderer wavefront: 0 = 2^24 (2^23 for a
                                  renderer with a radiance cache); a wavefront carries
                                  floor(wavefront_paths / (W*H)) sample passes (1 to 8).  Two
@@ -164,6 +164,10 @@ typedef struct crt_stats {
                                     went on with the plain tracer (until the next set_nrc) */
     uint32_t nrc_discarded_passes; /* sample passes rendered with the diverged cache, dropped
                                       from the image and rendered again without it */
+    double   ms_extend_busy;     /* with CRT_FLAG_PHASE_TIMING: time during which at least one
+                                    closest-hit launch ran (the union of their intervals over both
+                                    wavefronts in flight); ms_extend sums the launches' own spans,
+                                    which overlap */
 } crt_stats;
 
 /* BVH summary returned by crt_scene_bvh_info */
