@@ -126,6 +126,8 @@ def cpu_baseline(spec, res, bounces, seconds_target: float) -> dict:
 TRAVERSAL_SOURCES = ("crender_amd/csrc/traverse.hip", "crender_amd/csrc/crt_math.h", "crender_amd/csrc/crt_internal.h",
                      "crender_amd/csrc/bvh_build.hip", "crender_amd/csrc/integrator.hip")
 PMC_FILE = ROOT / "profiles" / "trace_closest_soup1m_1080p_pmc.json"
+# the closest-hit traversal the headline workload launches (default width, stack, no instances)
+TRACE_KERNEL = "k_trace<8, false, false, 16u, false, false>"
 
 
 def traversal_source_hash() -> str:
@@ -136,9 +138,10 @@ def traversal_source_hash() -> str:
     return h.hexdigest()[:16]
 
 
-def _pmc_traffic():
-    """HBM bytes per closest-hit launch from the committed counter summary, if it was
-    measured on the current traversal sources; else (None, why)."""
+def _pmc_traffic(rays_per_launch: float):
+    """HBM bytes per closest-hit launch (bytes per traced ray from the committed counter
+    summary times the rays of an average timed launch), if the summary was measured on the
+    current traversal sources; else (None, why)."""
     if not PMC_FILE.exists():
         return None, "no counter summary"
     pmc = json.loads(PMC_FILE.read_text())
@@ -146,7 +149,101 @@ def _pmc_traffic():
     if pmc.get("source_hash") != now:
         return None, (f"stale: {PMC_FILE.name} was measured on traversal sources {pmc.get('source_hash')}, "
                       f"the current ones hash to {now}")
-    return pmc.get("hbm_bytes_per_launch"), f"{PMC_FILE.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE, source {now})"
+    return (pmc["hbm_bytes_per_ray"] * rays_per_launch,
+            f"{PMC_FILE.name} (rocprofv3 FETCH_SIZE + WRITE_SIZE per traced ray, copy-calibrated, "
+            f"commit {pmc.get('commit')}, source {now})")
+
+
+def _counter_rows(d: Path) -> list:
+    import csv
+
+    files = sorted(d.rglob("*counter_collection.csv"))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    return [r for f in files for r in csv.DictReader(open(f))]
+
+
+def _per_dispatch(rows, counter: str, match=None) -> dict:
+    """{dispatch id: (kernel name, summed counter value)} of the dispatches whose kernel
+    name contains `match`."""
+    out = {}
+    for r in rows:
+        if r["Counter_Name"] != counter or (match and match not in r["Kernel_Name"]):
+            continue
+        k = int(r["Dispatch_Id"])
+        name, v = out.get(k, (r["Kernel_Name"], 0.0))
+        out[k] = (name, v + float(r["Counter_Value"]))
+    return dict(sorted(out.items()))
+
+
+def _scale(rows, counter: str, known_bytes: float, match: str, launches: int) -> float:
+    """Bytes moved per byte the counter reports: the `launches` largest dispatches of the
+    calibration kernel each move `known_bytes`."""
+    vals = sorted((v for _, v in _per_dispatch(rows, counter, match).values()), reverse=True)[:launches]
+    if len(vals) < launches:
+        raise SystemExit(f"calibration: {len(vals)} dispatches of '{match}' with {counter}")
+    return known_bytes / (1024.0 * sum(vals) / len(vals))
+
+
+def pmc_summary(d: Path) -> int:
+    """Turns the counter runs under `d` into the traffic summary bench.py reads
+    (profiles/trace_closest_soup1m_1080p_pmc.json) and prints a markdown table.  Expects
+    d/fetch, d/write (FETCH_SIZE / WRITE_SIZE of a --warmup 0 headline run whose JSON line is
+    d/fetch.json), d/copy_fetch, d/copy_write (--copy-calibration, JSON d/copy.json) and
+    d/gather_fetch (--gather-calibration, JSON d/gather.json)."""
+    run = json.loads((d / "fetch.json").read_text().strip().splitlines()[-1])
+    if run.get("warmup") != 0:
+        raise SystemExit("the counter run must have --warmup 0 (all its traversal launches are counted)")
+    copy = json.loads((d / "copy.json").read_text().strip().splitlines()[-1])
+    gather = json.loads((d / "gather.json").read_text().strip().splitlines()[-1])
+    fs = _scale(_counter_rows(d / "copy_fetch"), "FETCH_SIZE", copy["copy_bytes_read"], "", copy["launches"])
+    ws = _scale(_counter_rows(d / "copy_write"), "WRITE_SIZE", copy["copy_bytes_written"], "", copy["launches"])
+    # scattered reads: the 128-byte lines the gathered 64-byte rows lie in, plus the index
+    g_known = gather["unique_lines"] * gather["line_bytes"] + gather["index_bytes"]
+    gs = _scale(_counter_rows(d / "gather_fetch"), "FETCH_SIZE", g_known, "gather", gather["launches"])
+    fetch = _per_dispatch(_counter_rows(d / "fetch"), "FETCH_SIZE", TRACE_KERNEL)
+    write = _per_dispatch(_counter_rows(d / "write"), "WRITE_SIZE", TRACE_KERNEL)
+    if len(fetch) != len(write) or not fetch:
+        raise SystemExit(f"{len(fetch)} FETCH_SIZE and {len(write)} WRITE_SIZE dispatches of {TRACE_KERNEL}")
+    rays = run["rays"]["extension"]
+    kib_f = sum(v for _, v in fetch.values())
+    kib_w = sum(v for _, v in write.values())
+    total = 1024.0 * (kib_f * fs + kib_w * ws)
+    total_g = 1024.0 * (kib_f * gs + kib_w * ws)
+    try:
+        commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short", "HEAD"], capture_output=True,
+                                text=True, timeout=10).stdout.strip() or None
+    except (OSError, subprocess.SubprocessError):
+        commit = None
+    summary = {
+        "kernel": TRACE_KERNEL,
+        "workload": run["config"]["workload"],
+        "source_hash": traversal_source_hash(),
+        "commit": commit,
+        "dispatches": len(fetch),
+        "extension_rays": rays,
+        "fetch_scale": fs,
+        "write_scale": ws,
+        "gather_fetch_scale": gs,
+        "hbm_bytes_per_ray": total / rays,
+        # the same counts scaled as scattered 128-byte line reads (the gather calibration): node and
+        # triangle fetches are scattered, so the copy scale may overstate them
+        "hbm_bytes_per_ray_gather_scaled": total_g / rays,
+        "algorithmic_bytes_per_ray": BYTES_PER_RAY,
+    }
+    PMC_FILE.write_text(json.dumps(summary, indent=1) + "\n")
+    print(f"Copy calibration: FETCH_SIZE scale {fs:.3f}, WRITE_SIZE scale {ws:.3f} (bytes moved / bytes "
+          f"counted).  Gather calibration ({gather['gather_rows']} random {gather['row_bytes']}-byte rows, "
+          f"{gather['unique_lines']} distinct 128-byte lines): FETCH_SIZE scale {gs:.3f} (line bytes + index "
+          f"bytes / bytes counted).\n")
+    print("| dispatch | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled MB |\n|---|---|---|---|")
+    for k, (kf, kw) in zip(fetch, zip(fetch.values(), write.values())):
+        print(f"| {k} | {kf[1]:.0f} | {kw[1]:.0f} | {1024 * (kf[1] * fs + kw[1] * ws) / 1e6:.1f} |")
+    print(f"\n{rays} extension rays over {len(fetch)} dispatches: {total / rays:.1f} bytes per ray copy-scaled, "
+          f"{total_g / rays:.1f} gather-scaled (algorithmic {BYTES_PER_RAY}), {total / len(fetch) / 1e6:.1f} MB "
+          f"per dispatch on average copy-scaled; "
+          f"written to {PMC_FILE.relative_to(ROOT)} (source {summary['source_hash']}, commit {commit}).")
+    return 0
 
 
 def _mse(img, ref) -> float:
@@ -351,7 +448,29 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) 5 times, for scaling "
                          "profiler byte counters; prints the bytes moved per launch")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="MIB",
+                    help="only gather 2^22 random 64-byte rows of an MIB-MiB table 5 times (index_select), for "
+                         "scaling profiler byte counters on scattered reads; prints the bytes moved per launch")
+    ap.add_argument("--pmc-summary", type=Path, default=None, metavar="DIR",
+                    help="only summarise the counter runs under DIR into the committed traffic figure")
     args = ap.parse_args(argv)
+    if args.pmc_summary:
+        return pmc_summary(args.pmc_summary)
+    if args.gather_calibration:
+        import torch
+
+        rows = args.gather_calibration * (1 << 20) // 64
+        table = torch.zeros((rows, 16), dtype=torch.float32, device="cuda:0")
+        idx_h = np.random.default_rng(0).integers(0, rows, 1 << 22)
+        idx = torch.from_numpy(idx_h).to("cuda:0")
+        for _ in range(5):
+            out = table.index_select(0, idx)   # reads 8 B of index and one 64-byte row per output row
+        torch.cuda.synchronize()
+        del out
+        print(json.dumps({"gather_rows": 1 << 22, "unique_rows": int(np.unique(idx_h).size), "row_bytes": 64,
+                          "unique_lines": int(np.unique(idx_h // 2).size), "line_bytes": 128,
+                          "index_bytes": 8 << 22, "launches": 5}), flush=True)
+        return 0
     if args.copy_calibration:
         import torch
 
@@ -397,7 +516,7 @@ def main(argv=None) -> int:
     launches = s1["extend_launches"] - s0["extend_launches"]
     gpu_ms = s1["ms_render"] - s0["ms_render"]
     achieved = BYTES_PER_RAY * ext / (ms_ext * 1e-3) / 1e9 if ms_ext > 0 else 0.0
-    traffic, traffic_note = _pmc_traffic()
+    traffic, traffic_note = _pmc_traffic(ext / max(launches, 1))
     out = {
         "metric": METRIC,
         "value": rays / wall / 1e6,

@@ -198,8 +198,9 @@ template <uint32_t S>
 struct Stack {
     uint2    *s;        // LDS [S + 1][kTraceBlock] of (ref, entry distance bits); row S absorbs
                         // the writes of children that missed
-    uint32_t *g;        // global spill, entry-major over all lanes of the grid: [k][2][nlanes]
-    uint32_t  nlanes, gl, tid;
+    uint4    *g;        // this lane's global spill: kSpillDepth (ref, distance) entries, contiguous, so
+                        // that a spill of S entries writes S * 8 bytes of whole lines (uint4 = 2 entries)
+    uint32_t  tid;
     uint32_t  sp, gsp;
 
     uint32_t *err;
@@ -212,11 +213,9 @@ struct Stack {
                 *err = 1u;
                 return;
             }
-            for (uint32_t k = 0; k < S; ++k) {
-                const size_t e = (size_t)(gsp + k);
-                const uint2  v = s[k * kTraceBlock + tid];
-                g[(e * 2 + 0) * nlanes + gl] = v.x;
-                g[(e * 2 + 1) * nlanes + gl] = v.y;
+            for (uint32_t k = 0; k < S; k += 2) {
+                const uint2 a = s[k * kTraceBlock + tid], b = s[(k + 1) * kTraceBlock + tid];
+                g[(gsp + k) / 2] = make_uint4(a.x, a.y, b.x, b.y);
             }
             gsp += S;
             sp = 0;
@@ -253,9 +252,10 @@ struct Stack {
             if (sp == 0) {
                 if (gsp == 0) return kEmpty;
                 gsp -= S;
-                for (uint32_t k = 0; k < S; ++k) {
-                    const size_t e = (size_t)(gsp + k);
-                    s[k * kTraceBlock + tid] = make_uint2(g[(e * 2 + 0) * nlanes + gl], g[(e * 2 + 1) * nlanes + gl]);
+                for (uint32_t k = 0; k < S; k += 2) {
+                    const uint4 v = g[(gsp + k) / 2];
+                    s[k * kTraceBlock + tid]       = make_uint2(v.x, v.y);
+                    s[(k + 1) * kTraceBlock + tid] = make_uint2(v.z, v.w);
                 }
                 sp = S;
             }
@@ -315,9 +315,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
 
     Stack<S> st;
     st.s      = s_stack;
-    st.g      = spill;
-    st.nlanes = gridDim.x * blockDim.x;
-    st.gl     = blockIdx.x * blockDim.x + threadIdx.x;
+    st.g      = reinterpret_cast<uint4 *>(spill) + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) * (kSpillDepth / 2);
     st.tid    = threadIdx.x;
     st.sp = st.gsp = 0;
     st.err = q.error;
