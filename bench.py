@@ -337,7 +337,8 @@ def traversal_counts(spec, res, bounces, args) -> dict:
     from crender_amd import Context, Renderer, scene_from_spec
 
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
-                  trace_stack=args.stack, traversal_stats=True)
+                  trace_stack=args.stack, trace_refill=args.refill, trace_park_min=args.park_min,
+                  traversal_stats=True)
     sc = scene_from_spec(ctx, spec)
     r = Renderer(ctx, sc, res[0], res[1], bounces)
     r.set_resolution(*res)
@@ -352,7 +353,11 @@ def traversal_counts(spec, res, bounces, args) -> dict:
             "tris_per_ray": st["tri_tests"] / rays, "rays": rays,
             # lanes busy per wave iteration of the node / leaf loop (SIMD utilisation)
             "node_loop_lanes": st["node_visits"] / max(1, 64 * st["wave_node_steps"]),
-            "leaf_loop_lanes": st["leaf_visits"] / max(1, 64 * st["wave_leaf_steps"])}
+            "leaf_loop_lanes": st["leaf_visits"] / max(1, 64 * st["wave_leaf_steps"]),
+            # idle lanes of node-loop iterations by cause, as fractions of all lanes
+            "node_loop_idle_no_ray": st["node_idle_no_ray"] / max(1, 64 * st["wave_node_steps"]),
+            "node_loop_idle_done": st["node_idle_done"] / max(1, 64 * st["wave_node_steps"]),
+            "node_loop_idle_leaf_wait": st["node_idle_leaf_wait"] / max(1, 64 * st["wave_node_steps"])}
 
 
 def terrain_bench(args) -> dict:
@@ -364,7 +369,8 @@ def terrain_bench(args) -> dict:
     spec = scenes.terrain()
     W, H = spec.res
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
-                  trace_stack=args.stack, wavefront_paths=args.wavefront_paths)
+                  trace_stack=args.stack, trace_refill=args.refill, trace_park_min=args.park_min,
+                  wavefront_paths=args.wavefront_paths)
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
     r = Renderer(ctx, sc, W, H, spec.max_bounces)
@@ -436,6 +442,9 @@ def main(argv=None) -> int:
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent traversal occupancy (0 = default)")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per traversal lane (0 = default)")
     ap.add_argument("--max-leaf", type=int, default=0, help="BVH leaf size limit (0 = library default)")
+    ap.add_argument("--refill", type=int, default=0, help="idle lanes that make a traversal wave fetch rays (0 = default)")
+    ap.add_argument("--park-min", type=int, default=0, help="searching lanes below which a wave tests its parked leaves "
+                                                            "(0 = default)")
     ap.add_argument("--wavefront-paths", type=int, default=0, help="camera paths per wavefront (0 = default)")
     ap.add_argument("--sort-rays", action="store_true", help="reorder secondary rays before tracing")
     ap.add_argument("--sort-materials", action="store_true", help="shade each bounce's hits grouped by material")
@@ -492,7 +501,8 @@ def main(argv=None) -> int:
     res, bounces = (1920, 1080), 4
     spec = scenes.soup(args.tris, seed=2, res=res)
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, phase_timing=True,
-                  trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack, sort_rays=args.sort_rays,
+                  trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack,
+                  trace_refill=args.refill, trace_park_min=args.park_min, sort_rays=args.sort_rays,
                   sort_materials=args.sort_materials, wavefront_paths=args.wavefront_paths)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)

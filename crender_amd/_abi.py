@@ -77,7 +77,7 @@ class Camera(C.Structure):
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("bvh_width", C.c_uint32), ("max_leaf", C.c_uint32),
                 ("trace_waves_per_cu", C.c_uint32), ("flags", C.c_uint32), ("trace_stack", C.c_uint32),
-                ("wavefront_paths", C.c_uint32)]
+                ("wavefront_paths", C.c_uint32), ("trace_refill", C.c_uint32), ("trace_park_min", C.c_uint32)]
 
 
 class RenderOptions(C.Structure):
@@ -95,7 +95,8 @@ class Stats(C.Structure):
                 ("node_visits", C.c_uint64), ("leaf_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("wave_node_steps", C.c_uint64), ("wave_leaf_steps", C.c_uint64),
                 ("nrc_detached", C.c_uint32), ("nrc_discarded_passes", C.c_uint32),
-                ("ms_extend_busy", C.c_double)]
+                ("node_idle_no_ray", C.c_uint64), ("node_idle_done", C.c_uint64),
+                ("node_idle_leaf_wait", C.c_uint64), ("ms_extend_busy", C.c_double)]
 
 
 class BvhInfo(C.Structure):

@@ -103,6 +103,7 @@ struct crt_context {
     hipStream_t stream2 = nullptr;
     uint32_t   *spill2  = nullptr;
     uint32_t    stack  = kStackLds;   // LDS stack entries per traversal lane (8, 16, 32)
+    uint32_t    refill = 0, park_min = 0;   // traversal thresholds (0: the kernel's defaults)
     uint32_t    batch_paths = 0;   // tuned wavefront size of crt_render_batch (0 = default)
     // staging for crt_intersect / crt_occluded
     uint32_t    stage_cap = 0;
@@ -236,7 +237,7 @@ struct crt_renderer {
 
 namespace {
 
-TraceConfig trace_cfg(const crt_context *ctx) { return TraceConfig{ctx->grid, ctx->stack}; }
+TraceConfig trace_cfg(const crt_context *ctx) { return TraceConfig{ctx->grid, ctx->stack, ctx->refill, ctx->park_min}; }
 
 int ensure_spill(crt_context *ctx, int set = 0)
 {
@@ -1260,6 +1261,8 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     if (opt) o = *opt;
     if (o.bvh_width != 0 && o.bvh_width != 4 && o.bvh_width != 8)
         return fail(CRT_ERR_INVALID_ARG, "bvh_width must be 0, 4 or 8");
+    if (o.trace_refill > 64 || o.trace_park_min > 64)
+        return fail(CRT_ERR_INVALID_ARG, "trace_refill and trace_park_min must be at most 64");
     if (o.trace_stack != 0 && o.trace_stack != 8 && o.trace_stack != 16 && o.trace_stack != 32)
         return fail(CRT_ERR_INVALID_ARG, "trace_stack must be 0, 8, 16 or 32");
     int ndev = 0;
@@ -1278,6 +1281,8 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     }
     ctx->grid  = trace_grid(o.device, o.trace_waves_per_cu ? (o.trace_waves_per_cu + 3) / 4 : 0);
     ctx->stack = o.trace_stack ? o.trace_stack : kStackLds;
+    ctx->refill   = o.trace_refill;
+    ctx->park_min = o.trace_park_min;
     // the allocation-free query's scratch: a device array whose address is looked up once per
     // device, here, so that crt_intersect_noalloc itself makes no setup call
     void *na = nullptr;
@@ -2304,6 +2309,9 @@ int crt_renderer_current_stats(crt_renderer *r, crt_stats *stats)
             stats->tri_tests       += ts[2];
             stats->wave_node_steps += ts[3];
             stats->wave_leaf_steps += ts[4];
+            stats->node_idle_no_ray    += ts[5];
+            stats->node_idle_done      += ts[6];
+            stats->node_idle_leaf_wait += ts[7];
         }
     }
     if (r->nrc && r->loss_tiles) {

@@ -187,12 +187,17 @@ struct TraceQueue {
     uint32_t           *done;
     uint32_t           *clear0, *clear1;
     unsigned long long *stat;
+    // set by the launcher from TraceConfig: idle lanes that trigger a refill, and the number
+    // of lanes still able to park a leaf below which a wave leaves the inner-node phase
+    uint32_t refill_idle, park_min;
 };
-// Launch shape of the persistent traversal: blocks in the grid and LDS stack entries per
-// lane (8, 16 or 32).
+// Launch shape of the persistent traversal: blocks in the grid, LDS stack entries per lane
+// (8, 16 or 32), and the refill / parking thresholds (0 = defaults).
 struct TraceConfig {
     uint32_t grid;
     uint32_t stack;
+    uint32_t refill = 0;
+    uint32_t park_min = 0;
 };
 hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
                                 hipStream_t s);

@@ -18,8 +18,9 @@
 //   * children are ordered by entry distance with a sorting network; the nearest is
 //     visited next and the others pushed far-to-near, with unconditional LDS writes.
 //   * leaf parking: a lane that reaches a leaf keeps it aside (up to two) and continues
-//     down the tree until (nearly) every lane of the wave holds its leaves, so the triangle
-//     phase runs with more lanes busy (Aila and Laine 2009, "speculative traversal").
+//     down the tree until most lanes of the wave hold their leaves (fewer than 16 can still
+//     park one), so the triangle phase runs with more lanes busy (Aila and Laine 2009,
+//     "speculative traversal").
 #include <hip/hip_runtime.h>
 #include "crt_internal.h"
 
@@ -30,8 +31,11 @@ namespace {
 constexpr float kInf = __builtin_huge_valf();
 constexpr uint32_t kEmpty = kInvalidRef;
 constexpr uint32_t kMissKey = 0xFFFFFFFFu;
-constexpr uint32_t kRefillIdle = 32;
-constexpr uint32_t kParkMin    = 4;            // leaf parking ends when fewer lanes search
+constexpr uint32_t kRefillIdle = 32;           // default idle lanes that trigger a refill
+// default: a wave leaves the inner-node phase for its parked leaves once fewer than 16 lanes
+// can still park one.  Measured on the 1M soup / 14.8M terrain at 1080p with two wavefronts in
+// flight: 4 -> 2690 / 3031 Mrays/s, 8 -> 2814 / 3090, 16 -> 2852-2861 / 3079, 32 -> 2758 / 2979.
+constexpr uint32_t kParkMin    = 16;
 constexpr uint32_t kInstExit   = 0xFFFFFFFEu;   // stack marker: leave the current instance
 // Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
 // 1 + 2*gamma(3) keeps FP32 box tests conservative.
@@ -277,8 +281,9 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // The 8-entry stack variant is also the high-occupancy one: its 18 KB of LDS per block
 // leaves room for 8 waves per SIMD, so the register budget is capped to match (64 VGPRs).
 // kStats: the instrumented variant; counts each lane's inner-node tests, leaf visits and
-// triangle tests, and each wave's iterations of the node and leaf loops, and adds them to
-// q.tstats when the wave leaves the kernel.
+// triangle tests, each wave's iterations of the node and leaf loops, and the lanes idle in
+// node-loop iterations by cause (no ray, ray finished but for parked leaves, waiting at a
+// leaf with both parking slots full), and adds them to q.tstats when the wave leaves.
 // The launch's epilogue (TraceQueue::done): every block reports when its waves are done with
 // the work counters; the last one clears the next bounce's counters.  No fence is needed:
 // every work-counter atomic returned (its value was used) before the block's barrier, the
@@ -303,6 +308,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
                                                                       uint32_t *spill)
 {
     uint32_t c_nodes = 0, c_leaves = 0, c_tris = 0, c_wnode = 0, c_wleaf = 0;
+    uint32_t c_noray = 0, c_empty = 0, c_leafwait = 0;   // idle lanes of node-loop iterations, by cause
     __shared__ uint2 s_stack[(S + 1) * kTraceBlock];
     const uint32_t n = *q.count;
     if (n == 0) {
@@ -379,7 +385,7 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
         // ---- refill idle lanes ----
         uint64_t idle = __ballot(!active);
         uint32_t nidle = __popcll(idle);
-        while (!exhausted && nidle >= kRefillIdle) {
+        while (!exhausted && nidle >= q.refill_idle) {
             uint32_t base = 0, got = 0;
             if (lane == 0) {
                 while (next_slice < kXcds) {
@@ -452,10 +458,15 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             const bool inner = active && cur != kEmpty && !ref_is_leaf(cur);
             const uint64_t srch = __ballot(inner && (!kSpec || pend2 == kEmpty));
             if (srch == 0) break;
-            // fewer than kParkMin lanes still searching and some holding a leaf: run the leaf
+            // fewer than park_min lanes still searching and some holding a leaf: run the leaf
             // phase now rather than keep most of the wave waiting
-            if (kSpec && __popcll(srch) < kParkMin && __ballot(active && pend != kEmpty) != 0) break;
-            if constexpr (kStats) ++c_wnode;
+            if (kSpec && __popcll(srch) < q.park_min && __ballot(active && pend != kEmpty) != 0) break;
+            if constexpr (kStats) {
+                ++c_wnode;
+                c_noray += !active;
+                c_empty += active && cur == kEmpty;
+                c_leafwait += active && cur != kEmpty && ref_is_leaf(cur);
+            }
             if (inner) {
                 uint32_t key[W], ref[W];
                 const int nh = NodeOps<W>::expand(nodes, cur, r, key, ref);
@@ -539,11 +550,14 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
         }
     }
     if constexpr (kStats) {
-        unsigned long long a = c_nodes, b = c_leaves, c = c_tris;
+        unsigned long long a = c_nodes, b = c_leaves, c = c_tris, d = c_noray, e = c_empty, f = c_leafwait;
         for (int o = 32; o > 0; o >>= 1) {
             a += __shfl_xor(a, o);
             b += __shfl_xor(b, o);
             c += __shfl_xor(c, o);
+            d += __shfl_xor(d, o);
+            e += __shfl_xor(e, o);
+            f += __shfl_xor(f, o);
         }
         if (lane == 0 && q.tstats) {
             atomicAdd(q.tstats + 0, a);
@@ -551,6 +565,9 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             atomicAdd(q.tstats + 2, c);
             atomicAdd(q.tstats + 3, (unsigned long long)c_wnode);   // wave-uniform counts
             atomicAdd(q.tstats + 4, (unsigned long long)c_wleaf);
+            atomicAdd(q.tstats + 5, d);
+            atomicAdd(q.tstats + 6, e);
+            atomicAdd(q.tstats + 7, f);
         }
     }
     trace_epilogue(q, n);
@@ -579,10 +596,13 @@ namespace {
 
 // Instantiates k_trace for the BVH width and LDS stack depth of the launch.
 template <bool kAny, bool kMulti, bool kInst>
-hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev &sc, const TraceQueue &q, uint8_t *occ,
+hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev &sc, const TraceQueue &qin, uint8_t *occ,
                           uint32_t *spill, hipStream_t s)
 {
     const dim3 g(tc.grid), b(kTraceBlock);
+    TraceQueue q   = qin;
+    q.refill_idle  = tc.refill ? std::min(tc.refill, 64u) : kRefillIdle;
+    q.park_min     = tc.park_min ? std::min(tc.park_min, 64u) : kParkMin;
     if (!kMulti && q.tstats) {
         // instrumented traversal: one stack depth (the counts do not depend on it)
         if (width == 4) k_trace<4, kAny, false, 32, kInst, true><<<g, b, 0, s>>>(sc, q, occ, spill);

@@ -126,6 +126,10 @@ typedef struct crt_options {
                                  renderer with a radiance cache); a wavefront carries
                                  floor(wavefront_paths / (W*H)) sample passes (1 to 8).  Two
                                  wavefronts are in flight (two sets of path buffers, two streams). */
+    uint32_t trace_refill;  /* traversal waves fetch new rays once this many of their 64 lanes
+                               are idle: 0 = default (32), else 1 to 64 */
+    uint32_t trace_park_min;/* the inner-node phase of a wave ends once fewer than this many
+                               lanes can still park a leaf: 0 = default (16), else 1 to 64 */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
@@ -164,6 +168,10 @@ typedef struct crt_stats {
                                     went on with the plain tracer (until the next set_nrc) */
     uint32_t nrc_discarded_passes; /* sample passes rendered with the diverged cache, dropped
                                       from the image and rendered again without it */
+    uint64_t node_idle_no_ray;   /* with CRT_FLAG_TRAVERSAL_STATS: lanes idle in inner-node loop
+                                    iterations because they hold no ray, */
+    uint64_t node_idle_done;     /*   because their ray has nothing left but parked leaves, */
+    uint64_t node_idle_leaf_wait;/*   because they reached a leaf with both parking slots full */
     double   ms_extend_busy;     /* with CRT_FLAG_PHASE_TIMING: time during which at least one
                                     closest-hit launch ran (the union of their intervals over both
                                     wavefronts in flight); ms_extend sums the launches' own spans,
