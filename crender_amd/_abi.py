@@ -77,7 +77,8 @@ class Camera(C.Structure):
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("bvh_width", C.c_uint32), ("max_leaf", C.c_uint32),
                 ("trace_waves_per_cu", C.c_uint32), ("flags", C.c_uint32), ("trace_stack", C.c_uint32),
-                ("wavefront_paths", C.c_uint32), ("trace_refill", C.c_uint32), ("trace_park_min", C.c_uint32)]
+                ("wavefront_paths", C.c_uint32), ("trace_refill", C.c_uint32), ("trace_park_min", C.c_uint32),
+                ("trace_leaf_rounds", C.c_uint32)]
 
 
 class RenderOptions(C.Structure):

@@ -103,7 +103,7 @@ struct crt_context {
     hipStream_t stream2 = nullptr;
     uint32_t   *spill2  = nullptr;
     uint32_t    stack  = kStackLds;   // LDS stack entries per traversal lane (8, 16, 32)
-    uint32_t    refill = 0, park_min = 0;   // traversal thresholds (0: the kernel's defaults)
+    uint32_t    refill = 0, park_min = 0, leaf_rounds = 0;   // traversal thresholds (0: defaults)
     uint32_t    batch_paths = 0;   // tuned wavefront size of crt_render_batch (0 = default)
     // staging for crt_intersect / crt_occluded
     uint32_t    stage_cap = 0;
@@ -237,7 +237,10 @@ struct crt_renderer {
 
 namespace {
 
-TraceConfig trace_cfg(const crt_context *ctx) { return TraceConfig{ctx->grid, ctx->stack, ctx->refill, ctx->park_min}; }
+TraceConfig trace_cfg(const crt_context *ctx)
+{
+    return TraceConfig{ctx->grid, ctx->stack, ctx->refill, ctx->park_min, ctx->leaf_rounds};
+}
 
 int ensure_spill(crt_context *ctx, int set = 0)
 {
@@ -1283,6 +1286,7 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     ctx->stack = o.trace_stack ? o.trace_stack : kStackLds;
     ctx->refill   = o.trace_refill;
     ctx->park_min = o.trace_park_min;
+    ctx->leaf_rounds = o.trace_leaf_rounds;
     // the allocation-free query's scratch: a device array whose address is looked up once per
     // device, here, so that crt_intersect_noalloc itself makes no setup call
     void *na = nullptr;

@@ -190,6 +190,8 @@ struct TraceQueue {
     // set by the launcher from TraceConfig: idle lanes that trigger a refill, and the number
     // of lanes still able to park a leaf below which a wave leaves the inner-node phase
     uint32_t refill_idle, park_min;
+    uint32_t leaf_rounds;   // leaf-loop iterations per leaf phase before the wave resumes the
+                            // inner-node phase (0: until no lane holds a leaf)
 };
 // Launch shape of the persistent traversal: blocks in the grid, LDS stack entries per lane
 // (8, 16 or 32), and the refill / parking thresholds (0 = defaults).
@@ -198,6 +200,7 @@ struct TraceConfig {
     uint32_t stack;
     uint32_t refill = 0;
     uint32_t park_min = 0;
+    uint32_t leaf_rounds = 0;
 };
 hipError_t launch_trace_closest(const SceneDev &sc, const TraceQueue &q, uint32_t *spill, const TraceConfig &tc,
                                 hipStream_t s);

@@ -36,6 +36,11 @@ constexpr uint32_t kRefillIdle = 32;           // default idle lanes that trigge
 // can still park one.  Measured on the 1M soup / 14.8M terrain at 1080p with two wavefronts in
 // flight: 4 -> 2690 / 3031 Mrays/s, 8 -> 2814 / 3090, 16 -> 2852-2861 / 3079, 32 -> 2758 / 2979.
 constexpr uint32_t kParkMin    = 16;
+// default: after 3 leaf-loop rounds, once every parked leaf is tested, a wave resumes the
+// inner-node phase (lanes still at a leaf park it there) instead of following chains of
+// popped leaves with few lanes busy.  Soup / terrain: unlimited 2870 / 3048 Mrays/s, 1 round
+// 2880 / 3055, 2 rounds 2891 / 3105, 3 rounds 2904 / 3093.
+constexpr uint32_t kLeafRounds = 3;
 constexpr uint32_t kInstExit   = 0xFFFFFFFEu;   // stack marker: leave the current instance
 // Ize, "Robust BVH Ray Traversal" (JCGT 2013): scaling the far slab distance by
 // 1 + 2*gamma(3) keeps FP32 box tests conservative.
@@ -481,10 +486,12 @@ __global__ __launch_bounds__(kTraceBlock, S == 8 ? 8 : 1) void k_trace(SceneDev 
             }
         }
         // ---- leaf phase: the parked leaf first, then leaves popped after it ----
-        for (;;) {
+        for (uint32_t round = 0;; ++round) {
             const bool parked = kSpec && active && pend != kEmpty;
             const bool leaf   = active && (parked || (cur != kEmpty && ref_is_leaf(cur)));
             if (__ballot(leaf) == 0) break;
+            // a lane left holding an unparked leaf parks it in the next inner-node phase
+            if (kSpec && q.leaf_rounds && round >= q.leaf_rounds && __ballot(parked) == 0) break;
             if constexpr (kStats) ++c_wleaf;
             if (kInst && leaf && level == 0) {
                 // instance leaf: enter the first instance, keep the rest for later
@@ -603,6 +610,7 @@ hipError_t dispatch_trace(uint32_t width, const TraceConfig &tc, const SceneDev 
     TraceQueue q   = qin;
     q.refill_idle  = tc.refill ? std::min(tc.refill, 64u) : kRefillIdle;
     q.park_min     = tc.park_min ? std::min(tc.park_min, 64u) : kParkMin;
+    q.leaf_rounds  = tc.leaf_rounds ? (tc.leaf_rounds > 64u ? 0u : tc.leaf_rounds) : kLeafRounds;
     if (!kMulti && q.tstats) {
         // instrumented traversal: one stack depth (the counts do not depend on it)
         if (width == 4) k_trace<4, kAny, false, 32, kInst, true><<<g, b, 0, s>>>(sc, q, occ, spill);

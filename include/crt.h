@@ -130,6 +130,9 @@ typedef struct crt_options {
                                are idle: 0 = default (32), else 1 to 64 */
     uint32_t trace_park_min;/* the inner-node phase of a wave ends once fewer than this many
                                lanes can still park a leaf: 0 = default (16), else 1 to 64 */
+    uint32_t trace_leaf_rounds; /* leaf-loop rounds after which a wave whose parked leaves are
+                               all tested resumes the inner-node phase: 0 = default (3), 1 to 64,
+                               or above 64 = until no lane holds a leaf */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
