@@ -2731,29 +2731,41 @@ void plan_batch(const crt_job *jobs, uint32_t njobs, uint32_t P, BatchPlan &plan
     close();
 }
 
+// Two sets of path buffers replay alternate chunks on two streams (set k renders chunks k,
+// k + 2, ...), so one chunk's traversal overlaps the other's; each chunk's accumulation
+// waits for the previous chunk's, so the job sums are added in chunk order.  Per set: a
+// stream, a chunk cursor, a spill area and, with graphs, one graph for the traversal part of
+// a replay and one for its accumulation.
 struct BatchRun {
     std::vector<void *> allocs;
-    hipStream_t         stream = nullptr;
-    hipGraph_t          graph  = nullptr;
-    hipGraphExec_t      exec   = nullptr;
-    std::vector<hipEvent_t> events;
+    hipStream_t         stream[2] = {nullptr, nullptr};
+    hipGraph_t          graph[2][2] = {};      // [set][0: generate + bounces, 1: accumulate + advance]
+    hipGraphExec_t      exec[2][2]  = {};
+    std::vector<hipEvent_t> events;            // [0] start, [c + 1] chunk c accumulated
     ~BatchRun()
     {
-        if (stream) (void)hipStreamSynchronize(stream);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (hipStream_t s : stream)
+            if (s) (void)hipStreamSynchronize(s);
+        for (auto &row : exec)
+            for (hipGraphExec_t e : row)
+                if (e) (void)hipGraphExecDestroy(e);
+        for (auto &row : graph)
+            for (hipGraph_t g : row)
+                if (g) (void)hipGraphDestroy(g);
         for (hipEvent_t e : events)
             if (e) (void)hipEventDestroy(e);
         for (void *p : allocs) dfree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (hipStream_t s : stream)
+            if (s) (void)hipStreamDestroy(s);
     }
 };
 
-// One replay: generate the chunk's paths, trace / shade / shadow for max_bounces bounces,
-// accumulate, advance the chunk cursor.  Only kernel launches (capturable).
-int record_replay(const PathBuffers &b, const BatchDev &bd, uint32_t *cursor, const TraceScene *tscenes,
-                  uint32_t width, uint32_t max_bounces, bool any_sun, bool any_alpha, uint32_t *spill,
-                  const TraceConfig &tc, hipStream_t s)
+// The traversal part of one replay: generate the chunk's paths, trace / shade / shadow for
+// max_bounces bounces.  Only kernel launches (capturable) unless some job's scene has alpha
+// materials under the sun.
+int record_trace(const PathBuffers &b, const BatchDev &bd, const TraceScene *tscenes, uint32_t width,
+                 uint32_t max_bounces, bool any_sun, bool any_alpha, uint32_t *spill, const TraceConfig &tc,
+                 hipStream_t s)
 {
     HIP_CHECK(launch_batch_generate(bd, b, s));
     uint32_t cur = 0;
@@ -2792,8 +2804,15 @@ int record_replay(const PathBuffers &b, const BatchDev &bd, uint32_t *cursor, co
         }
         cur ^= 1u;
     }
+    return CRT_OK;
+}
+
+// The accumulation part of one replay: add the chunk's radiance to the job sums and move the
+// set's cursor to its next chunk.
+int record_accumulate(const PathBuffers &b, const BatchDev &bd, uint32_t *cursor, uint32_t step, hipStream_t s)
+{
     HIP_CHECK(launch_batch_accumulate(bd, b, b.P, s));
-    HIP_CHECK(launch_batch_advance(cursor, s));
+    HIP_CHECK(launch_batch_advance(cursor, step, s));
     return CRT_OK;
 }
 
@@ -2852,20 +2871,24 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
 
     HIP_CHECK(hipSetDevice(ctx->device));
     BatchRun run;
-    HIP_CHECK(hipStreamCreateWithFlags(&run.stream, hipStreamNonBlocking));
-    hipStream_t s = run.stream;
+    const uint32_t nrep  = (uint32_t)plan.chunks.size();
+    const uint32_t nsets = nrep > 1 ? 2u : 1u;
+    for (uint32_t k = 0; k < nsets; ++k) HIP_CHECK(hipStreamCreateWithFlags(&run.stream[k], hipStreamNonBlocking));
+    hipStream_t s = run.stream[0];
     auto &L = run.allocs;
     int st;
-    PathBuffers b{};
-    b.P    = P;
-    b.npix = P;   // no cache-training paths in batches
-    b.ncam = P;
-    if ((st = alloc_path_core(L, b, P))) return st;
-    if ((st = lalloc(L, &b.pjob, P))) return st;
-    if ((st = lalloc(L, &b.ppix, P))) return st;
-    if ((st = lalloc(L, &b.psamp, P))) return st;
-    uint32_t *spill = nullptr;
-    if ((st = lalloc(L, reinterpret_cast<uint8_t **>(&spill), trace_spill_bytes(ctx->grid)))) return st;
+    PathBuffers b[2]{};
+    uint32_t   *spill[2] = {nullptr, nullptr};
+    for (uint32_t k = 0; k < nsets; ++k) {
+        b[k].P    = P;
+        b[k].npix = P;   // no cache-training paths in batches
+        b[k].ncam = P;
+        if ((st = alloc_path_core(L, b[k], P))) return st;
+        if ((st = lalloc(L, &b[k].pjob, P))) return st;
+        if ((st = lalloc(L, &b[k].ppix, P))) return st;
+        if ((st = lalloc(L, &b[k].psamp, P))) return st;
+        if ((st = lalloc(L, reinterpret_cast<uint8_t **>(&spill[k]), trace_spill_bytes(ctx->grid)))) return st;
+    }
     // jobs, per-job traversal views, outputs
     std::vector<JobDev>     hj(njobs);
     std::vector<TraceScene> hts(njobs);
@@ -2903,45 +2926,61 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
     if ((st = lalloc(L, &d_ts, njobs))) return st;
     if ((st = lalloc(L, &d_segs, plan.segs.size()))) return st;
     if ((st = lalloc(L, &d_chunks, plan.chunks.size()))) return st;
-    if ((st = lalloc(L, &d_cursor, 1))) return st;
+    if ((st = lalloc(L, &d_cursor, 2))) return st;   // one per set: chunk k, then k + nsets, ...
     HIP_CHECK(hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_ts, hts.data(), sizeof(TraceScene) * njobs, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_segs, plan.segs.data(), sizeof(BatchSeg) * plan.segs.size(), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_chunks, plan.chunks.data(), sizeof(BatchChunk) * plan.chunks.size(),
                              hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemsetAsync(d_cursor, 0, sizeof(uint32_t), s));
-    HIP_CHECK(hipMemsetAsync(b.cnt, 0, sizeof(uint32_t) * kCntWords, s));
-    HIP_CHECK(hipMemsetAsync(b.stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
-    HIP_CHECK(hipStreamSynchronize(s));   // host staging vectors are pageable
+    const uint32_t cursor0[2] = {0u, 1u};
+    HIP_CHECK(hipMemcpyAsync(d_cursor, cursor0, sizeof cursor0, hipMemcpyHostToDevice, s));
+    for (uint32_t k = 0; k < nsets; ++k) {
+        HIP_CHECK(hipMemsetAsync(b[k].cnt, 0, sizeof(uint32_t) * kCntWords, s));
+        HIP_CHECK(hipMemsetAsync(b[k].stats, 0, sizeof(unsigned long long) * 4 * kStatSlots, s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));   // host staging vectors are pageable; both streams start after this
 
-    BatchDev bd{d_jobs, d_segs, d_chunks, d_cursor};
-    const uint32_t nrep = (uint32_t)plan.chunks.size();
+    BatchDev bd[2];
+    for (uint32_t k = 0; k < 2; ++k) bd[k] = BatchDev{d_jobs, d_segs, d_chunks, d_cursor + k};
     run.events.assign(nrep + 1, nullptr);
-    for (hipEvent_t &e : run.events) HIP_CHECK(hipEventCreate(&e));
+    for (hipEvent_t &e : run.events) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDefault));
+    const TraceConfig tc = trace_cfg(ctx);
+    auto trace_part = [&](uint32_t k) {
+        return record_trace(b[k], bd[k], d_ts, width, max_bounces, any_sun, any_alpha, spill[k], tc, run.stream[k]);
+    };
+    auto acc_part = [&](uint32_t k) { return record_accumulate(b[k], bd[k], d_cursor + k, nsets, run.stream[k]); };
     if (use_graph) {
-        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        st = record_replay(b, bd, d_cursor, d_ts, width, max_bounces, any_sun, any_alpha, spill, trace_cfg(ctx), s);
-        hipGraph_t g = nullptr;
-        const hipError_t ce = hipStreamEndCapture(s, &g);
-        run.graph = g;
-        if (st) return st;
-        HIP_CHECK(ce);
-        HIP_CHECK(hipGraphInstantiate(&run.exec, run.graph, nullptr, nullptr, 0));
+        for (uint32_t k = 0; k < nsets; ++k) {
+            for (int part = 0; part < 2; ++part) {
+                HIP_CHECK(hipStreamBeginCapture(run.stream[k], hipStreamCaptureModeThreadLocal));
+                st = part == 0 ? trace_part(k) : acc_part(k);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(run.stream[k], &g);
+                run.graph[k][part] = g;
+                if (st) return st;
+                HIP_CHECK(ce);
+                HIP_CHECK(hipGraphInstantiate(&run.exec[k][part], g, nullptr, nullptr, 0));
+            }
+        }
     }
     HIP_CHECK(hipEventRecord(run.events[0], s));
     for (uint32_t c = 0; c < nrep; ++c) {
-        if (use_graph) {
-            HIP_CHECK(hipGraphLaunch(run.exec, s));
-        } else if ((st = record_replay(b, bd, d_cursor, d_ts, width, max_bounces, any_sun, any_alpha, spill,
-                                       trace_cfg(ctx), s))) {
-            return st;
-        }
-        HIP_CHECK(hipEventRecord(run.events[c + 1], s));
+        const uint32_t k  = c % nsets;
+        hipStream_t    sk = run.stream[k];
+        if (use_graph) HIP_CHECK(hipGraphLaunch(run.exec[k][0], sk));
+        else if ((st = trace_part(k))) return st;
+        // the job sums take the chunks in order
+        if (c > 0) HIP_CHECK(hipStreamWaitEvent(sk, run.events[c], 0));
+        if (use_graph) HIP_CHECK(hipGraphLaunch(run.exec[k][1], sk));
+        else if ((st = acc_part(k))) return st;
+        HIP_CHECK(hipEventRecord(run.events[c + 1], sk));
     }
-    HIP_CHECK(hipStreamSynchronize(s));
-    uint32_t err = 0;
-    HIP_CHECK(hipMemcpy(&err, b.cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
-    if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    for (uint32_t k = 0; k < nsets; ++k) HIP_CHECK(hipStreamSynchronize(run.stream[k]));
+    for (uint32_t k = 0; k < nsets; ++k) {
+        uint32_t err = 0;
+        HIP_CHECK(hipMemcpy(&err, b[k].cnt + kCntErr, sizeof err, hipMemcpyDeviceToHost));
+        if (err) return fail(CRT_ERR_HIP, "traversal stack overflow");
+    }
     for (uint32_t j = 0; j < njobs; ++j) {
         if (jobs[j].memory != CRT_MEM_DEVICE)
             HIP_CHECK(hipMemcpy(jobs[j].rgb_sum, dev_out[j], sizeof(float) * 3 * jobs[j].width * jobs[j].height,
@@ -2954,10 +2993,12 @@ int crt_render_batch(crt_context *ctx, const crt_job *jobs, uint32_t njobs, cons
     }
     if (stats) {
         std::vector<unsigned long long> sv(4 * kStatSlots);
-        HIP_CHECK(hipMemcpy(sv.data(), b.stats, sizeof(unsigned long long) * sv.size(), hipMemcpyDeviceToHost));
         uint64_t tot[4] = {0, 0, 0, 0};
-        for (uint32_t i = 0; i < kStatSlots; ++i)
-            for (int k = 0; k < 4; ++k) tot[k] += sv[4 * i + k];
+        for (uint32_t set = 0; set < nsets; ++set) {
+            HIP_CHECK(hipMemcpy(sv.data(), b[set].stats, sizeof(unsigned long long) * sv.size(), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < kStatSlots; ++i)
+                for (int k = 0; k < 4; ++k) tot[k] += sv[4 * i + k];
+        }
         float ms = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&ms, run.events[0], run.events[nrep]));
         *stats = crt_batch_stats{};

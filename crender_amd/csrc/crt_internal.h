@@ -388,7 +388,7 @@ hipError_t launch_batch_generate(const BatchDev &bd, const PathBuffers &b, hipSt
 hipError_t launch_shade_multi(const BatchDev &bd, const PathBuffers &b, uint32_t bounce, uint32_t cur, hipStream_t s);
 hipError_t launch_shadow_alpha_step_multi(const BatchDev &bd, const PathBuffers &b, hipStream_t s);
 hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uint32_t max_acc, hipStream_t s);
-hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s);
+hipError_t launch_batch_advance(uint32_t *cursor, uint32_t step, hipStream_t s);
 // Intersection records (crt_records): every output pointer may be null except dist.
 struct RecordsDev {
     float    *dist;

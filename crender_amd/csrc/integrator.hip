@@ -933,9 +933,9 @@ __global__ __launch_bounds__(256) void k_batch_accumulate(BatchDev bd, PathBuffe
     raw[2] = r2;
 }
 
-__global__ void k_batch_advance(uint32_t *cursor)
+__global__ void k_batch_advance(uint32_t *cursor, uint32_t step)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *cursor += 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cursor += step;
 }
 
 }  // namespace
@@ -1050,9 +1050,9 @@ hipError_t launch_batch_accumulate(const BatchDev &bd, const PathBuffers &b, uin
     return hipGetLastError();
 }
 
-hipError_t launch_batch_advance(uint32_t *cursor, hipStream_t s)
+hipError_t launch_batch_advance(uint32_t *cursor, uint32_t step, hipStream_t s)
 {
-    k_batch_advance<<<1, 64, 0, s>>>(cursor);
+    k_batch_advance<<<1, 64, 0, s>>>(cursor, step);
     return hipGetLastError();
 }
 
