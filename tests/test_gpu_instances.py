@@ -2,10 +2,11 @@
 instance loop (scene.cpp:79-98, model.cpp:99-126), and against the same geometry flattened
 into world space.
 
-Bars: model / instance / primitive agree with the oracle (FP64 per-instance brute force) on
->= 99.9% of rays (the rest are near-ties of equal distance); world distances within 1e-4
-relative; occlusion equals "closest hit closer than tfar"; a pbr render of the instanced
-scene matches a render of the flattened scene (world normals; same Philox streams).
+Bars: hit / miss and model / instance / primitive agree with the oracle (FP64 per-instance
+brute force) except at most 1e-5 of the rays (near-tie grazing rays); world distances within
+1e-4 relative (FP32 instance transforms against the oracle's FP64 ones); occlusion equals
+"closest hit closer than tfar"; a pbr render of the instanced scene matches a render of the
+flattened scene (world normals; same Philox streams).
 """
 import numpy as np
 import pytest
@@ -81,13 +82,20 @@ def test_instanced_hits_match_oracle(width):
         hit = m >= 0
         ghit = g["prim_id"] != 0xFFFFFFFF
         assert hit.sum() > 1500
-        assert np.mean(hit == ghit) > 0.999
         both = hit & ghit
         same = (g["geom_id"][both] == m[both]) & (g["inst_id"][both] == i[both]) & (g["prim_id"][both] == p[both])
-        assert same.mean() > 0.999, same.mean()
         dn = np.linalg.norm(dirs.astype(np.float64), axis=1)
-        close = np.isclose(g["t"][both].astype(np.float64) * dn[both], d[both], rtol=1e-4)
-        assert close.mean() > 0.999
+        rel = np.abs(g["t"][both].astype(np.float64) * dn[both] - d[both]) / d[both]
+        print(f"instances: hit/miss disagree {int(np.sum(hit != ghit))}, id disagree {int(np.sum(~same))} of "
+              f"{int(both.sum())} hits, distance rel err max {rel.max():.3e} p99.9 {np.percentile(rel, 99.9):.3e}",
+              flush=True)
+        # hit IDs: at most 1e-5 of the rays (near-tie grazing rays) may disagree
+        allowed = max(1, int(1e-5 * len(hit)))
+        assert int(np.sum(hit != ghit)) <= allowed
+        assert int(np.sum(~same)) <= allowed
+        # distances: the traversal maps rays into object space in FP32 (as the reference's glm
+        # transforms do), the oracle in FP64; that rounding stays below 1e-4 relative
+        assert rel[same].max() <= 1e-4
         # occlusion = closest hit in (tnear, tfar]
         tf = np.where(ghit, g["t"] * 1.5, 5.0).astype(np.float32)
         occ = sc.occluded(org, dirs, tfar=tf)
