@@ -338,7 +338,7 @@ def traversal_counts(spec, res, bounces, args) -> dict:
 
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
                   trace_stack=args.stack, trace_refill=args.refill, trace_park_min=args.park_min,
-                  trace_leaf_rounds=args.leaf_rounds,
+                  trace_leaf_rounds=args.leaf_rounds, bvh_prim_cost=args.prim_cost,
                   traversal_stats=True)
     sc = scene_from_spec(ctx, spec)
     r = Renderer(ctx, sc, res[0], res[1], bounces)
@@ -371,7 +371,7 @@ def terrain_bench(args) -> dict:
     W, H = spec.res
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, trace_waves_per_cu=args.waves_per_cu,
                   trace_stack=args.stack, trace_refill=args.refill, trace_park_min=args.park_min,
-                  trace_leaf_rounds=args.leaf_rounds,
+                  trace_leaf_rounds=args.leaf_rounds, bvh_prim_cost=args.prim_cost,
                   wavefront_paths=args.wavefront_paths)
     sc = scene_from_spec(ctx, spec)
     info = sc.bvh_info()
@@ -445,6 +445,8 @@ def main(argv=None) -> int:
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per traversal lane (0 = default)")
     ap.add_argument("--max-leaf", type=int, default=0, help="BVH leaf size limit (0 = library default)")
     ap.add_argument("--refill", type=int, default=0, help="idle lanes that make a traversal wave fetch rays (0 = default)")
+    ap.add_argument("--prim-cost", type=float, default=0.0,
+                    help="triangle-test cost relative to a wide-node test in the BVH collapse (0 = default)")
     ap.add_argument("--leaf-rounds", type=int, default=0,
                     help="leaf-loop rounds per leaf phase (0 = default, above 64 = unlimited)")
     ap.add_argument("--park-min", type=int, default=0, help="searching lanes below which a wave tests its parked leaves "
@@ -507,7 +509,7 @@ def main(argv=None) -> int:
     ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, phase_timing=True,
                   trace_waves_per_cu=args.waves_per_cu, trace_stack=args.stack,
                   trace_refill=args.refill, trace_park_min=args.park_min,
-                  trace_leaf_rounds=args.leaf_rounds, sort_rays=args.sort_rays,
+                  trace_leaf_rounds=args.leaf_rounds, bvh_prim_cost=args.prim_cost, sort_rays=args.sort_rays,
                   sort_materials=args.sort_materials, wavefront_paths=args.wavefront_paths)
     dev = ctx.device_name()
     sc = scene_from_spec(ctx, spec)

@@ -78,7 +78,7 @@ class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("bvh_width", C.c_uint32), ("max_leaf", C.c_uint32),
                 ("trace_waves_per_cu", C.c_uint32), ("flags", C.c_uint32), ("trace_stack", C.c_uint32),
                 ("wavefront_paths", C.c_uint32), ("trace_refill", C.c_uint32), ("trace_park_min", C.c_uint32),
-                ("trace_leaf_rounds", C.c_uint32)]
+                ("trace_leaf_rounds", C.c_uint32), ("bvh_prim_cost", C.c_float)]
 
 
 class RenderOptions(C.Structure):

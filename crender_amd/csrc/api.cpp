@@ -1264,6 +1264,8 @@ int crt_context_create(const crt_options *opt, crt_context **out)
     if (opt) o = *opt;
     if (o.bvh_width != 0 && o.bvh_width != 4 && o.bvh_width != 8)
         return fail(CRT_ERR_INVALID_ARG, "bvh_width must be 0, 4 or 8");
+    if (!(o.bvh_prim_cost >= 0.0f) || o.bvh_prim_cost > 1e6f)
+        return fail(CRT_ERR_INVALID_ARG, "bvh_prim_cost must be a non-negative number");
     if (o.trace_refill > 64 || o.trace_park_min > 64)
         return fail(CRT_ERR_INVALID_ARG, "trace_refill and trace_park_min must be at most 64");
     if (o.trace_stack != 0 && o.trace_stack != 8 && o.trace_stack != 16 && o.trace_stack != 32)
@@ -1618,7 +1620,8 @@ int build_two_level(crt_scene *scene, const float *d_corners, const std::vector<
         const ModelHost &m = scene->models[k];
         const uint32_t   n = (uint32_t)(m.corners.size() / 9);
         if (n == 0 || m.instances.empty()) continue;
-        hipError_t e = build_bvh(d_corners + 9 * (size_t)model_first[k], n, width, max_leaf, s, &blas[k]);
+        hipError_t e = build_bvh(d_corners + 9 * (size_t)model_first[k], n, width, max_leaf, s, &blas[k],
+                                 scene->ctx->opt.bvh_prim_cost);
         if (e == hipSuccess) e = tri_add_prim_base(blas[k].tri, n, model_first[k], s);
         if (blas[k].invalid) return fail(CRT_ERR_HIP, "model BVH build: the builder's consistency checks failed");
         if (e != hipSuccess) return fail(CRT_ERR_HIP, std::string("model BVH build: ") + hipGetErrorString(e));
@@ -1825,7 +1828,7 @@ int crt_scene_commit(crt_scene *scene)
     BuildResult br{};
     uint32_t    max_stack = 0;
     if (!two_level) {
-        hipError_t be = build_bvh(d_corners, ntri, width, max_leaf, s, &br);
+        hipError_t be = build_bvh(d_corners, ntri, width, max_leaf, s, &br, ctx->opt.bvh_prim_cost);
         dfree(d_corners);
         if (br.invalid) return fail(CRT_ERR_HIP, "BVH build: the builder's consistency checks failed");
         if (be != hipSuccess) return fail(CRT_ERR_HIP, std::string("BVH build: ") + hipGetErrorString(be));

@@ -153,8 +153,10 @@ struct BuildResult {
     uint32_t invalid = 0;   // 1: the builder's own consistency checks failed (hipErrorUnknown)
 };
 // Builds the BVH of ntri triangles (9 floats each, device memory) on stream s.
+// prim_cost: cost of a primitive test relative to a wide-node test in the collapse's cost
+// model (0 = default).
 hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf,
-                     hipStream_t s, BuildResult *out);
+                     hipStream_t s, BuildResult *out, float prim_cost = 0.0f);
 // Moves a wide BVH into a pool: inner child refs += node_base, leaf first += tri_base.
 hipError_t bvh_rebase(void *nodes, uint32_t count, uint32_t width, uint32_t node_base, uint32_t tri_base,
                       hipStream_t s);

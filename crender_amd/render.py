@@ -26,7 +26,8 @@ class Context:
     def __init__(self, device: int = 0, bvh_width: int = 0, max_leaf: int = 0, phase_timing: bool = False,
                  trace_waves_per_cu: int = 0, trace_stack: int = 0, sort_rays: bool = False,
                  sort_materials: bool = False, traversal_stats: bool = False, wavefront_paths: int = 0,
-                 trace_refill: int = 0, trace_park_min: int = 0, trace_leaf_rounds: int = 0):
+                 trace_refill: int = 0, trace_park_min: int = 0, trace_leaf_rounds: int = 0,
+                 bvh_prim_cost: float = 0.0):
         L = A.lib()
         o = A.Options()
         L.crt_options_default(C.byref(o))
@@ -39,6 +40,7 @@ class Context:
         o.trace_refill = trace_refill
         o.trace_park_min = trace_park_min
         o.trace_leaf_rounds = trace_leaf_rounds
+        o.bvh_prim_cost = bvh_prim_cost
         o.flags = ((A.CRT_FLAG_PHASE_TIMING if phase_timing else 0) | (A.CRT_FLAG_SORT_RAYS if sort_rays else 0) |
                    (A.CRT_FLAG_SORT_MATERIALS if sort_materials else 0) |
                    (A.CRT_FLAG_TRAVERSAL_STATS if traversal_stats else 0))

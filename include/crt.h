@@ -133,6 +133,8 @@ typedef struct crt_options {
     uint32_t trace_leaf_rounds; /* leaf-loop rounds after which a wave whose parked leaves are
                                all tested resumes the inner-node phase: 0 = default (3), 1 to 64,
                                or above 64 = until no lane holds a leaf */
+    float    bvh_prim_cost;  /* cost of a triangle test relative to a wide-node test in the
+                               collapse of the binary BVH into wide nodes: 0 = default (1.0) */
 } crt_options;
 
 #define CRT_FLAG_PHASE_TIMING 1u /* record HIP events around every kernel of a pass */
