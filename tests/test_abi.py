@@ -63,6 +63,21 @@ def test_errors_without_gpu_are_reported_not_fatal():
     assert b"null" in L.crt_last_error()
 
 
+@pytest.mark.parametrize("field,value,word", [("bvh_width", 3, b"bvh_width"), ("bvh_prim_cost", -1.0, b"prim_cost"),
+                                             ("bvh_prim_cost", float("nan"), b"prim_cost"),
+                                             ("trace_refill", 65, b"trace_refill"), ("trace_park_min", 100, b"park_min"),
+                                             ("trace_stack", 12, b"trace_stack")])
+def test_context_options_are_checked_before_the_device(field, value, word):
+    """Out-of-range context options are refused with a message before any device is used."""
+    L = _abi.lib()
+    o = _abi.Options()
+    L.crt_options_default(ctypes.byref(o))
+    setattr(o, field, value)
+    h = ctypes.c_void_p()
+    assert L.crt_context_create(ctypes.byref(o), ctypes.byref(h)) == -1
+    assert word in L.crt_last_error() and not h.value
+
+
 def test_struct_layouts_match_header(tmp_path):
     """Every ctypes mirror of a crt.h struct has the C struct's size and field offsets
     (compiled with the host C compiler against the header)."""
@@ -78,6 +93,7 @@ def test_struct_layouts_match_header(tmp_path):
     pairs = {"crt_material": A.Material, "crt_image": A.Image, "crt_model_data": A.ModelData, "crt_sun": A.Sun,
              "crt_camera": A.Camera, "crt_options": A.Options, "crt_render_options": A.RenderOptions,
              "crt_stats": A.Stats, "crt_bvh_info": A.BvhInfo, "crt_rays": A.Rays, "crt_hits": A.Hits,
+             "crt_records": A.Records,
              "crt_nrc_options": A.NrcOptions, "crt_nrc_render_options": A.NrcRenderOptions,
              "crt_post_options": A.PostOptions, "crt_denoise_options": A.DenoiseOptions, "crt_job": A.Job,
              "crt_batch_options": A.BatchOptions, "crt_batch_stats": A.BatchStats, "crt_tune_result": A.TuneResult}

@@ -139,6 +139,27 @@ def test_soup_1m_headline_scale_vs_oracle(soup_1m, stack):
     ctx.close()
 
 
+@pytest.mark.parametrize("opts", [dict(trace_refill=8, trace_park_min=2), dict(trace_leaf_rounds=1),
+                                  dict(trace_leaf_rounds=100, trace_park_min=48),
+                                  dict(bvh_prim_cost=0.2, max_leaf=8), dict(bvh_prim_cost=5.0)],
+                         ids=["refill8-park2", "leaf1", "leafall-park48", "primcost0.2-leaf8", "primcost5"])
+def test_soup_1m_traversal_options_vs_oracle(soup_1m, opts):
+    """The traversal's scheduling thresholds and the collapse's cost model change the order of
+    work and the tree, never the answers: the same bar as the default settings."""
+    spec, tri, o, d, pb, tb, sel, p64, t64 = soup_1m
+    print(f"soup 1M, {opts}", flush=True)
+    ctx = Context(0, **opts)
+    sc = scene_from_spec(ctx, spec)
+    got = sc.cast_rays(o, d)
+    gp = _prims(got)
+    mism = int(np.sum(gp != pb))
+    assert mism <= max(1, int(1e-5 * len(gp))), f"{mism} of {len(gp)} differ from the FP32 oracle"
+    hit = (pb >= 0) & (gp == pb)
+    assert np.max(np.abs(got["t"][hit] - tb[hit]) / np.maximum(tb[hit], 1e-6)) <= 1e-5
+    sc.close()
+    ctx.close()
+
+
 def test_edge_cases(oracle):
     spec = scenes.cornell(2)
     ctx = Context(0)
