@@ -555,15 +555,35 @@ __global__ __launch_bounds__(256) void k_part_count(const BuildTask *tasks, cons
     if (threadIdx.x == 0) chunk_left[c] = cnt;
 }
 
-__global__ void k_part_offsets(uint32_t ntasks, const uint32_t *task_chunk0, const uint32_t *chunk_left,
-                               uint32_t *chunk_loff)
+// Exclusive scan of the chunks' left counts within each task: one 256-thread block per task,
+// 256 chunks at a time (a 14.8M-triangle root task has 7,200 chunks).
+__global__ __launch_bounds__(256) void k_part_offsets(uint32_t ntasks, const uint32_t *task_chunk0,
+                                                      const uint32_t *chunk_left, uint32_t *chunk_loff)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t wsum[4];
+    const uint32_t t = blockIdx.x;
     if (t >= ntasks) return;
-    uint32_t acc = 0;
-    for (uint32_t c = task_chunk0[t]; c < task_chunk0[t + 1]; ++c) {
-        chunk_loff[c] = acc;
-        acc += chunk_left[c];
+    const uint32_t c0 = task_chunk0[t], c1 = task_chunk0[t + 1];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = c0; base < c1; base += blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        const uint32_t v = c < c1 ? chunk_left[c] : 0u;
+        uint32_t incl = v;
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t x = __shfl_up(incl, o);
+            if (lane >= o) incl += x;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t j = 0; j < 4; ++j) {
+            before += j < w ? wsum[j] : 0u;
+            tot += wsum[j];
+        }
+        if (c < c1) chunk_loff[c] = carry + before + incl - v;
+        carry += tot;
+        __syncthreads();   // wsum is rewritten by the next tile
     }
 }
 
@@ -1358,7 +1378,7 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         k_split_large<<<nlarge, 64, 0, s>>>(large_a, nlarge, gbins, split, nodes, nslots, large_b, small, counts,
                                             max_leaf);
         k_part_count<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, idx, cen, chunk_left);
-        k_part_offsets<<<(nlarge + 63) / 64, 64, 0, s>>>(nlarge, task_chunk0, chunk_left, chunk_loff);
+        k_part_offsets<<<nlarge, 256, 0, s>>>(nlarge, task_chunk0, chunk_left, chunk_loff);
         k_part_scatter<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, chunk_loff, idx, cen, tmp);
         k_copy_chunks<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, tmp, idx);
         HIP_TRY(hipGetLastError());
