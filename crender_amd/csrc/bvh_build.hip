@@ -1011,11 +1011,23 @@ __global__ void k_level_down(const Bvh2Node *nodes, uint32_t nslots, uint32_t *l
 {
     const uint32_t n = cnt[L], base = off[L], out = base + n;
     if (blockIdx.x == 0 && threadIdx.x == 0) off[L + 1] = out;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Bvh2Node nd = nodes[list[base + i]];
-        if (b2_leaf(nd) || nd.a >= nslots || nd.b >= nslots) continue;
-        const uint32_t k = atomicAdd(cnt + L + 1, 2u);
-        if (out + k + 1 < nslots) {
+    const uint32_t lane = threadIdx.x & 63u;
+    // whole waves step through the list together, so every wave appends with one atomic
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        Bvh2Node nd{};
+        bool inner = false;
+        if (i < n) {
+            nd    = nodes[list[base + i]];
+            inner = !b2_leaf(nd) && nd.a < nslots && nd.b < nslots;
+        }
+        const uint64_t bal  = __ballot(inner);
+        const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
+        uint32_t first = 0;
+        if (lane == 0 && bal) first = atomicAdd(cnt + L + 1, 2u * (uint32_t)__popcll(bal));
+        first = __shfl(first, 0);
+        const uint32_t k = first + 2u * rank;
+        if (inner && out + k + 1 < nslots) {
             list[out + k]     = nd.a;
             list[out + k + 1] = nd.b;
         }
