@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <math.h>
+#include <functional>
+#include <mutex>
 #include <vector>
 #include "crt_internal.h"
 
@@ -370,7 +372,8 @@ __global__ __launch_bounds__(256) void k_prim_bounds(const float *__restrict__ t
     }
 }
 
-__global__ void k_root_task(const int *root, uint32_t n, BuildTask *large, BuildTask *small, uint32_t *counts)
+__global__ void k_root_task(const int *root, uint32_t n, BuildTask *large, BuildTask *small, uint32_t *nlarge,
+                            uint32_t *counts)
 {
     BuildTask t;
     t.begin = 0;
@@ -383,26 +386,31 @@ __global__ void k_root_task(const int *root, uint32_t n, BuildTask *large, Build
     }
     if (n > kChunk) {
         large[0]  = t;
-        counts[0] = 1;  // large
+        *nlarge   = 1;
         counts[1] = 0;  // small
     } else {
         small[0]  = t;
-        counts[0] = 0;
+        *nlarge   = 0;
         counts[1] = 1;
     }
 }
 
 // ---- stage 1a: segments larger than one chunk --------------------------------------------
+// Every kernel of a level reads its task and chunk counts from device memory and loops over
+// them with a fixed grid, so the host launches several levels before it reads a count.
 // Chunks of kChunk primitives of every large task, in task order: one block of 1024 threads,
-// a scan of the per-task chunk counts, then one chunk per thread.
+// a scan of the per-task chunk counts, then one chunk per thread.  The task count is read
+// from device memory (written by the previous level's split), so the host learns the next
+// level's task and chunk counts with one read.
 constexpr uint32_t kChunkBlock = 1024;
 
-__global__ __launch_bounds__(kChunkBlock) void k_make_chunks(const BuildTask *tasks, uint32_t ntasks,
+__global__ __launch_bounds__(kChunkBlock) void k_make_chunks(const BuildTask *tasks, const uint32_t *ntasks_p,
                                                              uint32_t *chunk_task, uint32_t *chunk_begin,
                                                              uint32_t *task_chunk0, uint32_t *nchunks)
 {
     __shared__ uint32_t part[kChunkBlock];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid    = threadIdx.x;
+    const uint32_t ntasks = *ntasks_p;
     const uint32_t per = (ntasks + kChunkBlock - 1) / kChunkBlock;
     const uint32_t t0 = min(tid * per, ntasks), t1 = min(t0 + per, ntasks);
     auto nch = [&](uint32_t t) { return (tasks[t].end - tasks[t].begin + kChunk - 1) / kChunk; };
@@ -439,57 +447,55 @@ __global__ __launch_bounds__(kChunkBlock) void k_make_chunks(const BuildTask *ta
     }
 }
 
-__global__ void k_clear_bins(int *bins, uint32_t ntasks)
+// Also zeroes the counter the level's split appends the next level's large tasks to.
+__global__ void k_clear_bins(int *bins, const uint32_t *ntasks_p, uint32_t *next_count)
 {
-    const uint32_t t = blockIdx.x;
-    if (t < ntasks) bins_init(bins + (size_t)t * kBinWords, threadIdx.x, blockDim.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0u;
+    const uint32_t n = *ntasks_p;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) bins_init(bins + (size_t)t * kBinWords, threadIdx.x, blockDim.x);
 }
 
 __global__ __launch_bounds__(256) void k_bin_large(const BuildTask *tasks, const uint32_t *chunk_task,
                                                    const uint32_t *chunk_begin, const uint32_t *idx,
                                                    const float4 *plo, const float4 *phi, const float4 *cen,
-                                                   int *gbins)
+                                                   int *gbins, const uint32_t *nchunks_p)
 {
     __shared__ int bins[kBinWords];
-    const uint32_t  c = blockIdx.x;
-    const BuildTask t = tasks[chunk_task[c]];
-    const uint32_t  b = chunk_begin[c];
-    const uint32_t  e = min(b + kChunk, t.end);
-    bins_init(bins, threadIdx.x, blockDim.x);
-    __syncthreads();
-    const float4 cblo = lo4(t.cblo), ks = scale4(t.cblo, t.cbhi);
-    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        const uint32_t id = idx[i];
-        bins_add(bins, cen[id], plo[id], phi[id], cblo, ks);
-    }
-    __syncthreads();
-    int *g = gbins + (size_t)chunk_task[c] * kBinWords;
-    for (int i = threadIdx.x; i < kBinWords; i += blockDim.x) {
-        const int w = i % 13;
-        const int v = bins[i];
-        if (w == 0) {
-            if (v) atomicAdd(g + i, v);
-        } else if ((w >= 1 && w <= 3) || (w >= 7 && w <= 9)) {
-            atomicMin(g + i, v);
-        } else {
-            atomicMax(g + i, v);
+    const uint32_t nch = *nchunks_p;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const BuildTask t = tasks[chunk_task[c]];
+        const uint32_t  b = chunk_begin[c];
+        const uint32_t  e = min(b + kChunk, t.end);
+        bins_init(bins, threadIdx.x, blockDim.x);
+        __syncthreads();
+        const float4 cblo = lo4(t.cblo), ks = scale4(t.cblo, t.cbhi);
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const uint32_t id = idx[i];
+            bins_add(bins, cen[id], plo[id], phi[id], cblo, ks);
         }
+        __syncthreads();
+        int *g = gbins + (size_t)chunk_task[c] * kBinWords;
+        for (int i = threadIdx.x; i < kBinWords; i += blockDim.x) {
+            const int w = i % 13;
+            const int v = bins[i];
+            if (w == 0) {
+                if (v) atomicAdd(g + i, v);
+            } else if ((w >= 1 && w <= 3) || (w >= 7 && w <= 9)) {
+                atomicMin(g + i, v);
+            } else {
+                atomicMax(g + i, v);
+            }
+        }
+        __syncthreads();   // bins are cleared for the next chunk
     }
 }
 
-// one wave per large segment: choose the split, write the node, emit the children
-__global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, uint32_t ntasks, const int *gbins,
-                                                    SplitInfo *split, Bvh2Node *nodes, uint32_t nslots,
-                                                    BuildTask *next_large, BuildTask *small, uint32_t *counts,
-                                                    uint32_t max_leaf)
+// Split decision, node and children of one large segment (next_count: the next level's
+// large-task counter, counts[1]: small tasks, counts[5]: error flag).
+__device__ __forceinline__ void split_one(const SahOut &s, const BuildTask &tk, uint32_t t, SplitInfo *split,
+                                          Bvh2Node *nodes, uint32_t nslots, BuildTask *next_large, BuildTask *small,
+                                          uint32_t *next_count, uint32_t *counts, uint32_t max_leaf)
 {
-    __shared__ SahOut s;
-    const uint32_t t = blockIdx.x;
-    if (t >= ntasks) return;
-    sah_wave(gbins + (size_t)t * kBinWords, &s);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const BuildTask tk = tasks[t];
     uint32_t        nleft = 0;
     int             axis  = -1;
     SplitInfo       si;
@@ -521,8 +527,25 @@ __global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, uint
     }
     const BuildTask ch[2] = {l, r};
     for (int j = 0; j < 2; ++j) {
-        if (ch[j].end - ch[j].begin > kChunk) next_large[atomicAdd(counts + 0, 1u)] = ch[j];
+        if (ch[j].end - ch[j].begin > kChunk) next_large[atomicAdd(next_count, 1u)] = ch[j];
         else small[atomicAdd(counts + 1, 1u)] = ch[j];
+    }
+}
+
+// one wave per large segment: choose the split, write the node, emit the children
+__global__ __launch_bounds__(64) void k_split_large(const BuildTask *tasks, const uint32_t *ntasks_p,
+                                                    const int *gbins, SplitInfo *split, Bvh2Node *nodes,
+                                                    uint32_t nslots, BuildTask *next_large, BuildTask *small,
+                                                    uint32_t *next_count, uint32_t *counts, uint32_t max_leaf)
+{
+    __shared__ SahOut s;
+    const uint32_t n = *ntasks_p;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+        sah_wave(gbins + (size_t)t * kBinWords, &s);
+        __syncthreads();
+        if (threadIdx.x == 0) split_one(s, tasks[t], t, split, nodes, nslots, next_large, small, next_count, counts,
+                                        max_leaf);
+        __syncthreads();   // s is rewritten for the next task
     }
 }
 
@@ -537,53 +560,58 @@ __device__ __forceinline__ bool goes_left(const SplitInfo &si, const BuildTask &
 
 __global__ __launch_bounds__(256) void k_part_count(const BuildTask *tasks, const SplitInfo *split,
                                                     const uint32_t *chunk_task, const uint32_t *chunk_begin,
-                                                    const uint32_t *idx, const float4 *cen, uint32_t *chunk_left)
+                                                    const uint32_t *idx, const float4 *cen, uint32_t *chunk_left,
+                                                    const uint32_t *nchunks_p)
 {
     __shared__ uint32_t cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    const uint32_t  c  = blockIdx.x;
-    const uint32_t  ti = chunk_task[c];
-    const BuildTask t  = tasks[ti];
-    const SplitInfo si = split[ti];
-    const uint32_t  b  = chunk_begin[c];
-    const uint32_t  e  = min(b + kChunk, t.end);
-    uint32_t        local = 0;
-    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) local += goes_left(si, t, i, idx[i], cen) ? 1u : 0u;
-    atomicAdd(&cnt, local);
-    __syncthreads();
-    if (threadIdx.x == 0) chunk_left[c] = cnt;
+    const uint32_t nch = *nchunks_p;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        if (threadIdx.x == 0) cnt = 0;
+        __syncthreads();
+        const uint32_t  ti = chunk_task[c];
+        const BuildTask t  = tasks[ti];
+        const SplitInfo si = split[ti];
+        const uint32_t  b  = chunk_begin[c];
+        const uint32_t  e  = min(b + kChunk, t.end);
+        uint32_t        local = 0;
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) local += goes_left(si, t, i, idx[i], cen) ? 1u : 0u;
+        atomicAdd(&cnt, local);
+        __syncthreads();
+        if (threadIdx.x == 0) chunk_left[c] = cnt;
+        __syncthreads();   // cnt is reset for the next chunk
+    }
 }
 
 // Exclusive scan of the chunks' left counts within each task: one 256-thread block per task,
 // 256 chunks at a time (a 14.8M-triangle root task has 7,200 chunks).
-__global__ __launch_bounds__(256) void k_part_offsets(uint32_t ntasks, const uint32_t *task_chunk0,
+__global__ __launch_bounds__(256) void k_part_offsets(const uint32_t *ntasks_p, const uint32_t *task_chunk0,
                                                       const uint32_t *chunk_left, uint32_t *chunk_loff)
 {
     __shared__ uint32_t wsum[4];
-    const uint32_t t = blockIdx.x;
-    if (t >= ntasks) return;
-    const uint32_t c0 = task_chunk0[t], c1 = task_chunk0[t + 1];
+    const uint32_t ntasks = *ntasks_p;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = c0; base < c1; base += blockDim.x) {
-        const uint32_t c = base + threadIdx.x;
-        const uint32_t v = c < c1 ? chunk_left[c] : 0u;
-        uint32_t incl = v;
-        for (uint32_t o = 1; o < 64; o <<= 1) {
-            const uint32_t x = __shfl_up(incl, o);
-            if (lane >= o) incl += x;
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const uint32_t c0 = task_chunk0[t], c1 = task_chunk0[t + 1];
+        uint32_t carry = 0;
+        for (uint32_t base = c0; base < c1; base += blockDim.x) {
+            const uint32_t c = base + threadIdx.x;
+            const uint32_t v = c < c1 ? chunk_left[c] : 0u;
+            uint32_t incl = v;
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint32_t x = __shfl_up(incl, o);
+                if (lane >= o) incl += x;
+            }
+            if (lane == 63) wsum[w] = incl;
+            __syncthreads();
+            uint32_t before = 0, tot = 0;
+            for (uint32_t j = 0; j < 4; ++j) {
+                before += j < w ? wsum[j] : 0u;
+                tot += wsum[j];
+            }
+            if (c < c1) chunk_loff[c] = carry + before + incl - v;
+            carry += tot;
+            __syncthreads();   // wsum is rewritten by the next tile
         }
-        if (lane == 63) wsum[w] = incl;
-        __syncthreads();
-        uint32_t before = 0, tot = 0;
-        for (uint32_t j = 0; j < 4; ++j) {
-            before += j < w ? wsum[j] : 0u;
-            tot += wsum[j];
-        }
-        if (c < c1) chunk_loff[c] = carry + before + incl - v;
-        carry += tot;
-        __syncthreads();   // wsum is rewritten by the next tile
     }
 }
 
@@ -591,53 +619,58 @@ __global__ __launch_bounds__(256) void k_part_offsets(uint32_t ntasks, const uin
 __global__ __launch_bounds__(256) void k_part_scatter(const BuildTask *tasks, const SplitInfo *split,
                                                       const uint32_t *chunk_task, const uint32_t *chunk_begin,
                                                       const uint32_t *chunk_loff, const uint32_t *idx,
-                                                      const float4 *cen, uint32_t *tmp)
+                                                      const float4 *cen, uint32_t *tmp, const uint32_t *nchunks_p)
 {
     __shared__ uint32_t wsum[4];
-    const uint32_t  c  = blockIdx.x;
-    const uint32_t  ti = chunk_task[c];
-    const BuildTask t  = tasks[ti];
-    const SplitInfo si = split[ti];
-    const uint32_t  b  = chunk_begin[c];
-    const uint32_t  e  = min(b + kChunk, t.end);
-    const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t        lbefore = chunk_loff[c];            // lefts of this task before this chunk
-    uint32_t        rbefore = (b - t.begin) - lbefore;  // rights before this chunk
-    for (uint32_t base = b; base < e; base += blockDim.x) {
-        const uint32_t i  = base + threadIdx.x;
-        const bool     in = i < e;
-        const uint32_t id = in ? idx[i] : 0u;
-        const bool     f  = in && goes_left(si, t, i, id, cen);
-        const uint64_t bal = __ballot(f);
-        const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[w] = __popcll(bal);
-        __syncthreads();
-        uint32_t off = 0, tot = 0;
-        for (int j = 0; j < 4; ++j) {
-            off += j < w ? wsum[j] : 0u;
-            tot += wsum[j];
+    const uint32_t nch = *nchunks_p;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t  ti = chunk_task[c];
+        const BuildTask t  = tasks[ti];
+        const SplitInfo si = split[ti];
+        const uint32_t  b  = chunk_begin[c];
+        const uint32_t  e  = min(b + kChunk, t.end);
+        const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        uint32_t        lbefore = chunk_loff[c];            // lefts of this task before this chunk
+        uint32_t        rbefore = (b - t.begin) - lbefore;  // rights before this chunk
+        for (uint32_t base = b; base < e; base += blockDim.x) {
+            const uint32_t i  = base + threadIdx.x;
+            const bool     in = i < e;
+            const uint32_t id = in ? idx[i] : 0u;
+            const bool     f  = in && goes_left(si, t, i, id, cen);
+            const uint64_t bal = __ballot(f);
+            const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[w] = __popcll(bal);
+            __syncthreads();
+            uint32_t off = 0, tot = 0;
+            for (int j = 0; j < 4; ++j) {
+                off += j < w ? wsum[j] : 0u;
+                tot += wsum[j];
+            }
+            if (in) {
+                const uint32_t lrank = off + rank;
+                const uint32_t pos   = f ? t.begin + lbefore + lrank
+                                         : t.begin + si.nleft + rbefore + (threadIdx.x - lrank);
+                tmp[pos] = id;
+            }
+            const uint32_t m = min((uint32_t)blockDim.x, e - base);
+            lbefore += tot;
+            rbefore += m - tot;
+            __syncthreads();
         }
-        if (in) {
-            const uint32_t lrank = off + rank;
-            const uint32_t pos   = f ? t.begin + lbefore + lrank
-                                     : t.begin + si.nleft + rbefore + (threadIdx.x - lrank);
-            tmp[pos] = id;
-        }
-        const uint32_t m = min((uint32_t)blockDim.x, e - base);
-        lbefore += tot;
-        rbefore += m - tot;
-        __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(256) void k_copy_chunks(const BuildTask *tasks, const uint32_t *chunk_task,
-                                                     const uint32_t *chunk_begin, const uint32_t *tmp, uint32_t *idx)
+                                                     const uint32_t *chunk_begin, const uint32_t *tmp, uint32_t *idx,
+                                                     const uint32_t *nchunks_p)
 {
-    const uint32_t  c = blockIdx.x;
-    const BuildTask t = tasks[chunk_task[c]];
-    const uint32_t  b = chunk_begin[c];
-    const uint32_t  e = min(b + kChunk, t.end);
-    for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) idx[i] = tmp[i];
+    const uint32_t nch = *nchunks_p;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const BuildTask t = tasks[chunk_task[c]];
+        const uint32_t  b = chunk_begin[c];
+        const uint32_t  e = min(b + kChunk, t.end);
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) idx[i] = tmp[i];
+    }
 }
 
 // ---- stage 1b: one workgroup builds a whole small segment -------------------------------
@@ -660,6 +693,47 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Subtrees of at most kSortMax primitives are split by an exact SAH sweep over their
+// primitives sorted by centroid (one row of 16 lanes per axis) instead of binning: most binary
+// nodes are this small, and the sweep costs a fraction of a binned step.
+constexpr uint32_t kSortMax = 16;
+
+// Position of lane pos's key among its row of 16 (ties by lane): count the smaller keys,
+// visiting the row with DPP rotations (row_ror d: lane i reads lane (i - d) mod 16, which
+// precedes i exactly when d <= i).
+template <int d>
+__device__ __forceinline__ uint32_t rank_in_row(float key, int pos, uint32_t r)
+{
+    const float kj = __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(
+        (int)__float_as_uint(key), (int)__float_as_uint(key), 0x120 + d, 0xF, 0xF, false));
+    r += (kj < key || (kj == key && d <= pos)) ? 1u : 0u;
+    if constexpr (d < 15) return rank_in_row<d + 1>(key, pos, r);
+    else return r;
+}
+
+// Inclusive prefix (suffix) box over a row of 16 lanes, Hillis-Steele with DPP row shifts;
+// a lane whose source falls outside the row keeps its own value, which min / max ignore.
+template <int d>
+__device__ __forceinline__ void prefix_box(float *lo, float *hi)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], row_shift<0, d>(lo[k]));
+        hi[k] = fmaxf(hi[k], row_shift<0, d>(hi[k]));
+    }
+    if constexpr (d < 8) prefix_box<2 * d>(lo, hi);
+}
+template <int d>
+__device__ __forceinline__ void suffix_box(float *lo, float *hi)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], row_shift<1, d>(lo[k]));
+        hi[k] = fmaxf(hi[k], row_shift<1, d>(hi[k]));
+    }
+    if constexpr (d < 8) suffix_box<2 * d>(lo, hi);
 }
 
 // One workgroup builds a whole small segment: nodes above kWaveTask primitives block-wide
@@ -858,60 +932,156 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
         }
         int wsp = 0;   // wave-uniform
         for (;;) {
-            bins_init(wb, lane, 64);
-            wave_sync();
-            const float4 cblo = lo4(c.cblo), ks = scale4(c.cblo, c.cbhi);
-            for (uint32_t i = c.b + lane; i < c.e; i += 64) {
-                const uint32_t id = ids[i];
-                bins_add(wb, cen[id], plo[id], phi[id], cblo, ks);
-            }
-            wave_sync();
-            sah_wave(wb, ws);
-            wave_sync();
-            uint32_t nl = 0;
-            int      ax = -1;
-            const int leaf = decide(*ws, max_leaf, nl, ax);   // same result in every lane
-            if (lane == 0) {
-                if (leaf) write_leaf(nodes, nslots, err, c.slot, *ws, gb + c.b, c.e - c.b);
-                else write_inner(nodes, nslots, err, c.slot, *ws, nl);
-            }
-            if (!leaf && ax >= 0) {
-                const int      bin = ws->bin;
-                const float    alo = pick(c.cblo[0], c.cblo[1], c.cblo[2], ax);
-                const float    aks = bin_scale(alo, pick(c.cbhi[0], c.cbhi[1], c.cbhi[2], ax));
-                uint32_t       lcount = 0, rcount = 0;
-                for (uint32_t base = c.b; base < c.e; base += 64) {
-                    const uint32_t i  = base + lane;
-                    const bool     in = i < c.e;
-                    bool           f  = false;
-                    uint32_t       id = 0;
-                    if (in) {
-                        id = ids[i];
-                        const float4 cc4 = cen[id];
-                        const float  cc  = ax == 0 ? cc4.x : (ax == 1 ? cc4.y : cc4.z);
-                        f = bin_at(cc, alo, aks) <= bin;
-                    }
-                    const uint64_t bal  = __ballot(f);
-                    const uint32_t rank = __popcll(bal & lt);
-                    if (in) tmp[f ? c.b + lcount + rank : c.b + nl + rcount + ((uint32_t)lane - rank)] = id;
-                    const uint32_t tot = __popcll(bal);
-                    lcount += tot;
-                    rcount += min(64u, c.e - base) - tot;
+            const uint32_t cn     = c.e - c.b;
+            const bool     sorted = cn <= kSortMax;
+            int            leaf;
+            uint32_t       nl = 0;
+            int            ax = -1;
+            bool           keep_l = true, keep_r = true;   // children still to be built
+            if (sorted) {
+                // ---- exact SAH sweep over the sorted primitives (at most 16) ----
+                const int  row = lane >> 4, pos = lane & 15;   // row a: primitives sorted by axis a
+                const bool val = lane < 48 && (uint32_t)pos < cn;
+                uint32_t   id  = 0;
+                float      key = kInf;
+                float4     blo = make_float4(kInf, kInf, kInf, 0.0f), bhi = make_float4(-kInf, -kInf, -kInf, 0.0f);
+                if (val) {
+                    id = ids[c.b + pos];
+                    const float4 cc = cen[id];
+                    key = pick(cc.x, cc.y, cc.z, row);
+                    blo = plo[id];
+                    bhi = phi[id];
+                }
+                const uint32_t rank = rank_in_row<1>(key, pos, 0u);
+                uint32_t *sid = reinterpret_cast<uint32_t *>(wb);        // [3][16] ids in sorted order
+                float    *sbx = reinterpret_cast<float *>(wb + 48);      // [6][3][16] their boxes
+                if (val) {
+                    const int at = row * 16 + (int)rank;
+                    sid[at]           = id;
+                    sbx[0 * 48 + at]  = blo.x;
+                    sbx[1 * 48 + at]  = blo.y;
+                    sbx[2 * 48 + at]  = blo.z;
+                    sbx[3 * 48 + at]  = bhi.x;
+                    sbx[4 * 48 + at]  = bhi.y;
+                    sbx[5 * 48 + at]  = bhi.z;
                 }
                 wave_sync();
-                for (uint32_t i = c.b + lane; i < c.e; i += 64) ids[i] = tmp[i];
+                float pl[3], ph[3];   // box of sorted positions 0..pos
+                for (int k = 0; k < 3; ++k) {
+                    pl[k] = val ? sbx[k * 48 + lane] : kInf;
+                    ph[k] = val ? sbx[(3 + k) * 48 + lane] : -kInf;
+                }
+                float sl[3] = {pl[0], pl[1], pl[2]}, sh[3] = {ph[0], ph[1], ph[2]};   // positions pos..15
+                prefix_box<1>(pl, ph);
+                suffix_box<1>(sl, sh);
+                float rl[3], rh[3];   // box of positions pos + 1 .. cn - 1: right side of the split after pos
+                for (int k = 0; k < 3; ++k) {
+                    rl[k] = row_shift<1, 1>(sl[k]);
+                    rh[k] = row_shift<1, 1>(sh[k]);
+                }
+                float cost = kInf;
+                if (lane < 48 && (uint32_t)pos + 1u < cn)
+                    cost = area3(pl[0], pl[1], pl[2], ph[0], ph[1], ph[2]) * (float)(pos + 1) +
+                           area3(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]) * (float)(cn - 1u - (uint32_t)pos);
+                float bc = cost;
+                int   bl = lane;
+                for (int o = 32; o > 0; o >>= 1) {
+                    const float oc = __shfl_xor(bc, o);
+                    const int   ol = __shfl_xor(bl, o);
+                    if (oc < bc || (oc == bc && ol < bl)) {
+                        bc = oc;
+                        bl = ol;
+                    }
+                }
+                SahOut so;
+                for (int k = 0; k < 3; ++k) {
+                    so.lo[k] = __shfl(pl[k], (int)cn - 1);
+                    so.hi[k] = __shfl(ph[k], (int)cn - 1);
+                }
+                so.n     = cn;
+                so.cost  = bc;
+                so.axis  = bc == kInf ? -1 : bl >> 4;
+                so.bin   = bc == kInf ? -1 : bl & 15;
+                so.nleft = bc == kInf ? 0u : (uint32_t)(bl & 15) + 1u;
+                leaf     = decide(so, max_leaf, nl, ax);
+                if (lane == 0) {
+                    if (leaf) write_leaf(nodes, nslots, err, c.slot, so, gb + c.b, cn);
+                    else write_inner(nodes, nslots, err, c.slot, so, nl);
+                }
+                if (!leaf && ax >= 0) {
+                    if (row == ax && (uint32_t)pos < cn) ids[c.b + pos] = sid[lane];
+                    // a one-primitive child is a leaf: the splitting lane holds its box
+                    keep_l = nl > 1u;
+                    keep_r = cn - nl > 1u;
+                    if (lane == bl) {
+                        if (!keep_l) {
+                            SahOut lb;
+                            for (int k = 0; k < 3; ++k) lb.lo[k] = pl[k], lb.hi[k] = ph[k];
+                            write_leaf(nodes, nslots, err, c.slot + 1u, lb, gb + c.b, 1u);
+                        }
+                        if (!keep_r) {
+                            SahOut rb;
+                            for (int k = 0; k < 3; ++k) rb.lo[k] = rl[k], rb.hi[k] = rh[k];
+                            write_leaf(nodes, nslots, err, c.slot + 2u * nl, rb, gb + c.b + nl, 1u);
+                        }
+                    }
+                }
                 wave_sync();
+            } else {
+                bins_init(wb, lane, 64);
+                wave_sync();
+                const float4 cblo = lo4(c.cblo), ks = scale4(c.cblo, c.cbhi);
+                for (uint32_t i = c.b + lane; i < c.e; i += 64) {
+                    const uint32_t id = ids[i];
+                    bins_add(wb, cen[id], plo[id], phi[id], cblo, ks);
+                }
+                wave_sync();
+                sah_wave(wb, ws);
+                wave_sync();
+                leaf = decide(*ws, max_leaf, nl, ax);   // same result in every lane
+                if (lane == 0) {
+                    if (leaf) write_leaf(nodes, nslots, err, c.slot, *ws, gb + c.b, cn);
+                    else write_inner(nodes, nslots, err, c.slot, *ws, nl);
+                }
+                if (!leaf && ax >= 0) {
+                    const int      bin = ws->bin;
+                    const float    alo = pick(c.cblo[0], c.cblo[1], c.cblo[2], ax);
+                    const float    aks = bin_scale(alo, pick(c.cbhi[0], c.cbhi[1], c.cbhi[2], ax));
+                    uint32_t       lcount = 0, rcount = 0;
+                    for (uint32_t base = c.b; base < c.e; base += 64) {
+                        const uint32_t i  = base + lane;
+                        const bool     in = i < c.e;
+                        bool           f  = false;
+                        uint32_t       id = 0;
+                        if (in) {
+                            id = ids[i];
+                            const float4 cc4 = cen[id];
+                            const float  cc  = ax == 0 ? cc4.x : (ax == 1 ? cc4.y : cc4.z);
+                            f = bin_at(cc, alo, aks) <= bin;
+                        }
+                        const uint64_t bal  = __ballot(f);
+                        const uint32_t rank = __popcll(bal & lt);
+                        if (in) tmp[f ? c.b + lcount + rank : c.b + nl + rcount + ((uint32_t)lane - rank)] = id;
+                        const uint32_t tot = __popcll(bal);
+                        lcount += tot;
+                        rcount += min(64u, c.e - base) - tot;
+                    }
+                    wave_sync();
+                    for (uint32_t i = c.b + lane; i < c.e; i += 64) ids[i] = tmp[i];
+                    wave_sync();
+                }
             }
-            if (!leaf) {
-                // children [b, b + nl) at slot + 1 and [b + nl, e) at slot + 2 nl; the larger
-                // waits, the smaller (at most half of c) is next: every push halves the size, so
-                // at most log2(kWaveTask) = 7 entries are ever held.  (Field-wise selects keep
-                // the tasks in registers.)
+            if (!leaf && (keep_l || keep_r)) {
+                // children [b, b + nl) at slot + 1 and [b + nl, e) at slot + 2 nl.  With both
+                // still to build, the larger waits and the smaller (at most half of c) is next:
+                // every push halves the size, so at most log2(kWaveTask) = 7 entries are ever
+                // held.  (Field-wise selects keep the tasks in registers.)  Children of a sorted
+                // node are sorted nodes too and need no centroid bounds.
                 const uint32_t lb = c.b, le = c.b + nl, rb = c.b + nl, re = c.e;
                 const uint32_t lsl = c.slot + 1, rsl = c.slot + 2 * nl;
-                const bool     lbig = le - lb >= re - rb;
+                const bool     lbig = keep_l && keep_r ? le - lb >= re - rb : keep_r;
                 float lcl[3], lch[3], rcl[3], rch[3];
-                const bool byax = ax >= 0;
+                const bool byax = ax >= 0 && !sorted;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     // values, not addresses, are selected (a selected address would put c in memory)
@@ -922,19 +1092,22 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                     rcl[k] = byax ? a2 : c0;
                     rch[k] = byax ? a3 : c1;
                 }
-                if (lane == 0) {
-                    LocalTask &d = wst[wsp];
-                    d.b    = lbig ? lb : rb;
-                    d.e    = lbig ? le : re;
-                    d.slot = lbig ? lsl : rsl;
-                    d.pad  = 0;
+                if (keep_l && keep_r) {
+                    if (lane == 0) {
+                        LocalTask &d = wst[wsp];
+                        d.b    = lbig ? lb : rb;
+                        d.e    = lbig ? le : re;
+                        d.slot = lbig ? lsl : rsl;
+                        d.pad  = 0;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        d.cblo[k] = lbig ? lcl[k] : rcl[k];
-                        d.cbhi[k] = lbig ? lch[k] : rch[k];
+                        for (int k = 0; k < 3; ++k) {
+                            d.cblo[k] = lbig ? lcl[k] : rcl[k];
+                            d.cbhi[k] = lbig ? lch[k] : rch[k];
+                        }
                     }
+                    ++wsp;
                 }
-                ++wsp;
+                // next: the smaller child, or the only one left to build (lbig = keep_r then)
                 c.b    = lbig ? rb : lb;
                 c.e    = lbig ? re : le;
                 c.slot = lbig ? rsl : lsl;
@@ -1093,13 +1266,9 @@ __global__ void k_dp_level(const Bvh2Node *nodes, const uint32_t *list, const ui
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Frontier *in, uint32_t nin,
-                                                  Frontier *out, uint32_t *nout, uint32_t *wide_count,
-                                                  void *wide_nodes, CollapseDP dp)
+__device__ __forceinline__ void collapse_node(const Bvh2Node *b2, const Frontier f, Frontier *out, uint32_t *nout,
+                                              uint32_t *wide_count, void *wide_nodes, const CollapseDP &dp)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nin) return;
-    const Frontier f = in[i];
     const Bvh2Node root = b2[f.slot];
     // children: binary slots, each a leaf of its whole range (leafy) or a wide node to be
     uint32_t ch[W];
@@ -1240,6 +1409,18 @@ __global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Fron
     }
 }
 
+// One wide level: the frontier's count is read from device memory (written by the level
+// above), so levels are launched back to back without the host reading each count.
+template <int W>
+__global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Frontier *in, const uint32_t *nin_p,
+                                                  Frontier *out, uint32_t *nout, uint32_t *wide_count,
+                                                  void *wide_nodes, CollapseDP dp)
+{
+    const uint32_t nin = *nin_p;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += gridDim.x * blockDim.x)
+        collapse_node<W>(b2, in[i], out, nout, wide_count, wide_nodes, dp);
+}
+
 // Structural check of the finished wide BVH, so that a bad tree fails the build instead of
 // the traversal: child counts in [1, W]; inner children after their parent (which also rules
 // out cycles) and inside the pool; leaf ranges inside the triangle array; quantised child
@@ -1318,6 +1499,58 @@ __global__ void k_sah_cost(const Bvh2Node *nodes, uint32_t nslots, double *acc, 
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
+// The build's scratch memory comes from a pool of its own per device that keeps up to
+// kPoolKeep bytes mapped between builds: the default pool hands freed memory back to the
+// driver at every synchronisation, and mapping a few hundred MB again costs ~0.2 ms per
+// build.  Scratch beyond kPoolKeep (builds of more than ~5M triangles) is still returned.
+constexpr uint64_t kPoolKeep   = 2ull << 30;
+constexpr int      kMaxDevices = 64;
+
+// The build's scratch buffers, carved out of one allocation in 256-byte aligned pieces.
+struct Arena {
+    std::vector<std::function<void(char *)>> pieces;   // each sets its pointer from the base
+    size_t                                   bytes = 0;
+    template <class T>
+    void add(T **p, size_t n)
+    {
+        const size_t off = bytes;
+        pieces.push_back([p, off](char *base) { *p = static_cast<T *>(static_cast<void *>(base + off)); });
+        bytes += (n + 255) & ~(size_t)255;
+    }
+    void place(void *base)
+    {
+        for (auto &f : pieces) f(static_cast<char *>(base));
+    }
+};
+
+hipError_t scratch_pool(hipMemPool_t *out)
+{
+    static std::mutex   mu;
+    static hipMemPool_t pools[kMaxDevices] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props{};
+        props.allocType     = hipMemAllocationTypePinned;
+        props.handleTypes   = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id   = dev;
+        hipMemPool_t pool   = nullptr;
+        HIP_TRY(hipMemPoolCreate(&pool, &props));
+        uint64_t keep = kPoolKeep;
+        const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (e != hipSuccess) {
+            hipMemPoolDestroy(pool);
+            return e;
+        }
+        pools[dev] = pool;
+    }
+    *out = pools[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf, hipStream_t s,
@@ -1329,6 +1562,8 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     if (ntri >= (1u << 27)) return hipErrorInvalidValue;
     if (max_leaf < 1) max_leaf = 1;
     if (max_leaf > 16) max_leaf = 16;
+    hipMemPool_t pool;
+    HIP_TRY(scratch_pool(&pool));
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
@@ -1340,103 +1575,120 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     int *root;
     Bvh2Node *nodes;
     BuildTask *large_a, *large_b, *small;
-    uint32_t *counts;   // [0] large, [1] small, [2] nchunks, [3] frontier out, [4] wide count
+    uint32_t *counts;   // [1] small tasks, [4] wide count, [5] error flag, [8..24) per-level rings
     const uint32_t max_large = ntri / kChunk + 2;
     const uint32_t max_chunks = ntri / kChunk + 2 * max_large + 2;
     uint32_t *chunk_task, *chunk_begin, *task_chunk0, *chunk_left, *chunk_loff;
     SplitInfo *split;
     int *gbins;
-    HIP_TRY(hipMallocAsync((void **)&plo, sizeof(float4) * ntri, s));
-    HIP_TRY(hipMallocAsync((void **)&phi, sizeof(float4) * ntri, s));
-    HIP_TRY(hipMallocAsync((void **)&cen, sizeof(float4) * ntri, s));
-    HIP_TRY(hipMallocAsync((void **)&idx, sizeof(uint32_t) * ntri, s));
-    HIP_TRY(hipMallocAsync((void **)&tmp, sizeof(uint32_t) * ntri, s));
-    HIP_TRY(hipMallocAsync((void **)&root, sizeof(int) * 12, s));
-    HIP_TRY(hipMallocAsync((void **)&nodes, sizeof(Bvh2Node) * nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&large_a, sizeof(BuildTask) * max_large, s));
-    HIP_TRY(hipMallocAsync((void **)&large_b, sizeof(BuildTask) * max_large, s));
-    HIP_TRY(hipMallocAsync((void **)&small, sizeof(BuildTask) * (ntri + 1), s));
-    HIP_TRY(hipMallocAsync((void **)&counts, sizeof(uint32_t) * 8, s));
-    HIP_TRY(hipMallocAsync((void **)&chunk_task, sizeof(uint32_t) * max_chunks, s));
-    HIP_TRY(hipMallocAsync((void **)&chunk_begin, sizeof(uint32_t) * max_chunks, s));
-    HIP_TRY(hipMallocAsync((void **)&task_chunk0, sizeof(uint32_t) * (max_large + 1), s));
-    HIP_TRY(hipMallocAsync((void **)&chunk_left, sizeof(uint32_t) * max_chunks, s));
-    HIP_TRY(hipMallocAsync((void **)&chunk_loff, sizeof(uint32_t) * max_chunks, s));
-    HIP_TRY(hipMallocAsync((void **)&split, sizeof(SplitInfo) * max_large, s));
-    HIP_TRY(hipMallocAsync((void **)&gbins, sizeof(int) * kBinWords * max_large, s));
+    Arena A;
+    A.add(&plo, sizeof(float4) * ntri);
+    A.add(&phi, sizeof(float4) * ntri);
+    A.add(&cen, sizeof(float4) * ntri);
+    A.add(&idx, sizeof(uint32_t) * ntri);
+    A.add(&tmp, sizeof(uint32_t) * ntri);
+    A.add(&root, sizeof(int) * 12);
+    A.add(&nodes, sizeof(Bvh2Node) * nslots);
+    A.add(&large_a, sizeof(BuildTask) * max_large);
+    A.add(&large_b, sizeof(BuildTask) * max_large);
+    A.add(&small, sizeof(BuildTask) * (ntri + 1));
+    A.add(&counts, sizeof(uint32_t) * 32);
+    A.add(&chunk_task, sizeof(uint32_t) * max_chunks);
+    A.add(&chunk_begin, sizeof(uint32_t) * max_chunks);
+    A.add(&task_chunk0, sizeof(uint32_t) * (max_large + 1));
+    A.add(&chunk_left, sizeof(uint32_t) * max_chunks);
+    A.add(&chunk_loff, sizeof(uint32_t) * max_chunks);
+    A.add(&split, sizeof(SplitInfo) * max_large);
+    A.add(&gbins, sizeof(int) * kBinWords * max_large);
+    // scratch of the later stages
+    double   *sah_acc;
+    uint32_t *used;
+    A.add(&sah_acc, sizeof(double));
+    A.add(&used, sizeof(uint32_t));
+    CollapseDP dp{};
+    uint32_t  *lvl_list = nullptr, *lvl_cnt = nullptr, *lvl_off = nullptr;
+    A.add(&dp.cost, sizeof(float) * 8 * (size_t)nslots);
+    A.add(&dp.choice, sizeof(uint32_t) * (size_t)nslots);
+    A.add(&dp.count, sizeof(uint32_t) * (size_t)nslots);
+    A.add(&dp.first, sizeof(uint32_t) * (size_t)nslots);
+    A.add(&lvl_list, sizeof(uint32_t) * (size_t)nslots);
+    A.add(&lvl_cnt, sizeof(uint32_t) * (kMaxLevels + 1));
+    A.add(&lvl_off, sizeof(uint32_t) * (kMaxLevels + 1));
+    const uint32_t max_wide   = ntri + 1;
+    const size_t   node_bytes = 128;
+    void          *wide_tmp;
+    Frontier      *fa, *fb;
+    uint32_t      *fcnt;
+    A.add(&wide_tmp, node_bytes * max_wide);
+    A.add(&fa, sizeof(Frontier) * max_wide);
+    A.add(&fb, sizeof(Frontier) * max_wide);
+    A.add(&fcnt, sizeof(uint32_t) * ((size_t)max_wide + 6));
+    // one allocation for all scratch: a pool allocation costs the host ~10 us, and one made
+    // while kernels are queued has been measured to leave the GPU idle ~0.25 ms
+    void *arena = nullptr;
+    HIP_TRY(hipMallocFromPoolAsync(&arena, A.bytes, pool, s));
+    A.place(arena);
+    float4 *tri_out = nullptr;   // the triangles in leaf order: part of the result
 
-    const int init_root[12] = {f2ord(kInf), f2ord(kInf), f2ord(kInf), f2ord(-kInf), f2ord(-kInf), f2ord(-kInf),
-                               f2ord(kInf), f2ord(kInf), f2ord(kInf), f2ord(-kInf), f2ord(-kInf), f2ord(-kInf)};
-    HIP_TRY(hipMemcpyAsync(root, init_root, sizeof init_root, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * 8, s));
+    // root bounds start empty (order-preserving integer images of +inf / -inf); small
+    // constants are set with memsets, not copied from pageable host memory, which would wait
+    // for the stream to drain
+    const int o_inf = f2ord(kInf), o_ninf = f2ord(-kInf);
+    for (int k = 0; k < 4; ++k) HIP_TRY(hipMemsetD32Async(root + 3 * k, (k & 1) ? o_ninf : o_inf, 3, s));
+    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * 32, s));
     k_mark_unused<<<(nslots + 255) / 256, 256, 0, s>>>(nodes, nslots);
     const uint32_t nb = std::min<uint32_t>((ntri + 255) / 256, 2048u);
     k_prim_bounds<<<nb, 256, 0, s>>>(d_tri, ntri, plo, phi, cen, idx, root);
-    k_root_task<<<1, 1, 0, s>>>(root, ntri, large_a, small, counts);
+    // Per-level counters in rings of kRing entries: tcnt[L % kRing] large tasks and
+    // ccnt[L % kRing] chunks of level L.  Levels are launched kLevelsPerRead at a time with
+    // fixed grids; the host then reads the rings once and stops when the next level has no
+    // large task (a level launched past the last one finds zero tasks and does nothing).
+    constexpr uint32_t kLevelsPerRead = 4, kRing = 8;
+    uint32_t          *tcnt = counts + 8, *ccnt = counts + 8 + kRing;
+    k_root_task<<<1, 1, 0, s>>>(root, ntri, large_a, small, tcnt, counts);
+    k_make_chunks<<<1, kChunkBlock, 0, s>>>(large_a, tcnt, chunk_task, chunk_begin, task_chunk0, ccnt);
     HIP_TRY(hipGetLastError());
-
-    uint32_t hc[8];
-    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    uint32_t nlarge = hc[0];
-    while (nlarge > 0) {
-        k_make_chunks<<<1, kChunkBlock, 0, s>>>(large_a, nlarge, chunk_task, chunk_begin, task_chunk0, counts + 2);
-        k_clear_bins<<<nlarge, 256, 0, s>>>(gbins, nlarge);
-        HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        const uint32_t nchunks = hc[2];
-        k_bin_large<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, idx, plo, phi, cen, gbins);
-        HIP_TRY(hipMemsetAsync(counts + 0, 0, sizeof(uint32_t), s));
-        k_split_large<<<nlarge, 64, 0, s>>>(large_a, nlarge, gbins, split, nodes, nslots, large_b, small, counts,
-                                            max_leaf);
-        k_part_count<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, idx, cen, chunk_left);
-        k_part_offsets<<<nlarge, 256, 0, s>>>(nlarge, task_chunk0, chunk_left, chunk_loff);
-        k_part_scatter<<<nchunks, 256, 0, s>>>(large_a, split, chunk_task, chunk_begin, chunk_loff, idx, cen, tmp);
-        k_copy_chunks<<<nchunks, 256, 0, s>>>(large_a, chunk_task, chunk_begin, tmp, idx);
+    uint32_t       hc[8 + 2 * kRing];
+    const uint32_t gt = std::min<uint32_t>(max_large, 1024u), gc = std::min<uint32_t>(max_chunks, 2048u);
+    uint32_t L = 0;
+    for (;;) {
+        for (const uint32_t upto = L + kLevelsPerRead; L < upto; ++L) {
+            BuildTask     *cur = (L & 1u) ? large_b : large_a, *nxt = (L & 1u) ? large_a : large_b;
+            const uint32_t *nt = tcnt + L % kRing, *nc = ccnt + L % kRing;
+            uint32_t       *nt1 = tcnt + (L + 1) % kRing, *nc1 = ccnt + (L + 1) % kRing;
+            k_clear_bins<<<gt, 256, 0, s>>>(gbins, nt, nt1);
+            k_bin_large<<<gc, 256, 0, s>>>(cur, chunk_task, chunk_begin, idx, plo, phi, cen, gbins, nc);
+            k_split_large<<<gt, 64, 0, s>>>(cur, nt, gbins, split, nodes, nslots, nxt, small, nt1, counts, max_leaf);
+            k_part_count<<<gc, 256, 0, s>>>(cur, split, chunk_task, chunk_begin, idx, cen, chunk_left, nc);
+            k_part_offsets<<<gt, 256, 0, s>>>(nt, task_chunk0, chunk_left, chunk_loff);
+            k_part_scatter<<<gc, 256, 0, s>>>(cur, split, chunk_task, chunk_begin, chunk_loff, idx, cen, tmp, nc);
+            k_copy_chunks<<<gc, 256, 0, s>>>(cur, chunk_task, chunk_begin, tmp, idx, nc);
+            // chunks of the next level (the chunk lists above have been consumed)
+            k_make_chunks<<<1, kChunkBlock, 0, s>>>(nxt, nt1, chunk_task, chunk_begin, task_chunk0, nc1);
+        }
         HIP_TRY(hipGetLastError());
+        // the result's triangle buffer is allocated while the first levels run
+        if (!tri_out) HIP_TRY(hipMalloc((void **)&tri_out, sizeof(float4) * 3 * (size_t)ntri));
         HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        nlarge = hc[0];
-        std::swap(large_a, large_b);
+        if (hc[8 + L % kRing] == 0u) break;
     }
-    const uint32_t nsmall = hc[1];
+    const uint32_t large_levels = L;   // an upper bound: the last batch may end in empty levels
+    const uint32_t nsmall       = hc[1];
     if (nsmall) k_small<<<nsmall, 256, 0, s>>>(small, nsmall, idx, plo, phi, cen, nodes, nslots, counts + 5, max_leaf);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (hc[5] != 0u) {
-        // a node slot or child range fell outside the tree: nothing past this point is safe
-        void *early[] = {plo, phi, cen, idx, tmp, root, nodes, large_a, large_b, small, counts, chunk_task,
-                         chunk_begin, task_chunk0, chunk_left, chunk_loff, split, gbins};
-        for (void *p : early) HIP_TRY(hipFreeAsync(p, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        hipEventDestroy(ev0);
-        hipEventDestroy(ev1);
-        out->invalid = 1;
-        return hipErrorUnknown;
-    }
+    // The SAH cost, the collapse's initialisation and the first batch of level lists are
+    // queued behind k_small without waiting for it: they only read node slots inside the tree
+    // (k_level_down checks every child slot), and k_small's error flag is read with the
+    // level counts, before anything that trusts the tree runs.
 
     // SAH cost + used slots
-    double   *sah_acc;
-    uint32_t *used;
-    HIP_TRY(hipMallocAsync((void **)&sah_acc, sizeof(double), s));
-    HIP_TRY(hipMallocAsync((void **)&used, sizeof(uint32_t), s));
     HIP_TRY(hipMemsetAsync(sah_acc, 0, sizeof(double), s));
     HIP_TRY(hipMemsetAsync(used, 0, sizeof(uint32_t), s));
     k_sah_cost<<<std::min<uint32_t>((nslots + 255) / 256, 1024u), 256, 0, s>>>(nodes, nslots, sah_acc, used);
 
     // SAH-optimal distribution of the binary tree over wide nodes (cost of a primitive test
     // relative to a wide-node test): level lists top-down, costs bottom-up
-    CollapseDP     dp{};
-    uint32_t      *lvl_list = nullptr, *lvl_cnt = nullptr, *lvl_off = nullptr;
-    const uint32_t one_u32  = 1;
-    HIP_TRY(hipMallocAsync((void **)&dp.cost, sizeof(float) * 8 * (size_t)nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&dp.choice, sizeof(uint32_t) * (size_t)nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&dp.count, sizeof(uint32_t) * (size_t)nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&dp.first, sizeof(uint32_t) * (size_t)nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&lvl_list, sizeof(uint32_t) * (size_t)nslots, s));
-    HIP_TRY(hipMallocAsync((void **)&lvl_cnt, sizeof(uint32_t) * (kMaxLevels + 1), s));
-    HIP_TRY(hipMallocAsync((void **)&lvl_off, sizeof(uint32_t) * (kMaxLevels + 1), s));
     // nodes the level lists do not reach (trees deeper than kMaxLevels) keep a split into
     // halves and are never turned into leaves
     HIP_TRY(hipMemsetAsync(dp.choice, 0x44, sizeof(uint32_t) * (size_t)nslots, s));
@@ -1444,17 +1696,32 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     HIP_TRY(hipMemsetAsync(lvl_cnt, 0, sizeof(uint32_t) * (kMaxLevels + 1), s));
     HIP_TRY(hipMemsetAsync(lvl_off, 0, sizeof(uint32_t) * (kMaxLevels + 1), s));
     HIP_TRY(hipMemsetAsync(lvl_list, 0, sizeof(uint32_t), s));   // level 0: the root, slot 0
-    HIP_TRY(hipMemcpyAsync(lvl_cnt, &one_u32, sizeof one_u32, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetD32Async(lvl_cnt, 1, 1, s));
     uint32_t nlevels = 0;
     {
-        const uint32_t g = std::min<uint32_t>((nslots + 255) / 256, 1024u);
+        // level L holds at most 2^L nodes: the grid of a level's launch is sized to that
+        const uint32_t g    = std::min<uint32_t>((nslots + 255) / 256, 1024u);
+        auto           grid = [&](uint32_t L) { return L >= 18u ? g : std::min(g, ((1u << L) + 255u) / 256u); };
         std::vector<uint32_t> lc(kMaxLevels + 1);
-        for (uint32_t L = 0; L < kMaxLevels && nlevels == 0;) {
-            const uint32_t upto = std::min<uint32_t>(L + 16u, kMaxLevels);
-            for (; L < upto; ++L) k_level_down<<<g, 256, 0, s>>>(nodes, nslots, lvl_list, lvl_cnt, lvl_off, L);
+        // first batch: the large levels plus the depth a 2048-primitive segment usually reaches
+        for (uint32_t L = 0, batch = large_levels + 24u; L < kMaxLevels && nlevels == 0; batch = 16u) {
+            const bool     first = L == 0;
+            const uint32_t upto  = std::min<uint32_t>(L + batch, kMaxLevels);
+            for (; L < upto; ++L) k_level_down<<<grid(L), 256, 0, s>>>(nodes, nslots, lvl_list, lvl_cnt, lvl_off, L);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(lc.data(), lvl_cnt, sizeof(uint32_t) * (L + 1), hipMemcpyDeviceToHost, s));
+            if (first) HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
+            if (first && hc[5] != 0u) {
+                // a node slot or child range fell outside the tree: nothing past this point is safe
+                HIP_TRY(hipFreeAsync(arena, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                hipFree(tri_out);
+                hipEventDestroy(ev0);
+                hipEventDestroy(ev1);
+                out->invalid = 1;
+                return hipErrorUnknown;
+            }
             for (uint32_t k = 0; k <= L; ++k)
                 if (lc[k] == 0u) {
                     nlevels = k;
@@ -1465,54 +1732,63 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         const float    cp   = prim_cost > 0.0f ? prim_cost : kPrimCost;
         const uint32_t lmax = std::min<uint32_t>(std::max<uint32_t>(max_leaf, 1u), kMaxLeafPrims);
         for (uint32_t L = nlevels; L-- > 0;) {
-            if (width == 4) k_dp_level<4><<<g, 256, 0, s>>>(nodes, lvl_list, lvl_cnt, lvl_off, L, dp, 1.0f, cp, lmax);
-            else k_dp_level<8><<<g, 256, 0, s>>>(nodes, lvl_list, lvl_cnt, lvl_off, L, dp, 1.0f, cp, lmax);
+            if (width == 4)
+                k_dp_level<4><<<grid(L), 256, 0, s>>>(nodes, lvl_list, lvl_cnt, lvl_off, L, dp, 1.0f, cp, lmax);
+            else k_dp_level<8><<<grid(L), 256, 0, s>>>(nodes, lvl_list, lvl_cnt, lvl_off, L, dp, 1.0f, cp, lmax);
         }
         HIP_TRY(hipGetLastError());
     }
 
     // collapse
-    const uint32_t max_wide = ntri + 1;
-    const size_t   node_bytes = 128;
-    void          *wide_tmp;
-    Frontier      *fa, *fb;
-    HIP_TRY(hipMallocAsync(&wide_tmp, node_bytes * max_wide, s));
-    HIP_TRY(hipMallocAsync((void **)&fa, sizeof(Frontier) * max_wide, s));
-    HIP_TRY(hipMallocAsync((void **)&fb, sizeof(Frontier) * max_wide, s));
-    const Frontier f0{0u, 0u};
-    HIP_TRY(hipMemcpyAsync(fa, &f0, sizeof f0, hipMemcpyHostToDevice, s));
-    const uint32_t one = 1;
-    HIP_TRY(hipMemcpyAsync(counts + 4, &one, sizeof one, hipMemcpyHostToDevice, s));
-    uint32_t nfront = 1, depth = 0;
-    while (nfront > 0) {
-        HIP_TRY(hipMemsetAsync(counts + 3, 0, sizeof(uint32_t), s));
-        if (width == 4)
-            k_collapse<4><<<(nfront + 255) / 256, 256, 0, s>>>(nodes, fa, nfront, fb, counts + 3, counts + 4, wide_tmp,
-                                                                dp);
-        else
-            k_collapse<8><<<(nfront + 255) / 256, 256, 0, s>>>(nodes, fa, nfront, fb, counts + 3, counts + 4, wide_tmp,
-                                                                dp);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        nfront = hc[3];
-        ++depth;
-        std::swap(fa, fb);
+    HIP_TRY(hipMemsetAsync(fa, 0, sizeof(Frontier), s));   // the root: binary slot 0, wide node 0
+    HIP_TRY(hipMemsetD32Async(counts + 4, 1, 1, s));
+    // frontier sizes per wide level: fcnt[d] nodes at depth d (fcnt[0] = 1, the root).  The
+    // depth is at most the number of wide nodes, so fcnt never overflows.  Levels are
+    // launched in batches and the host reads the counts once per batch; a launch past the
+    // last level finds an empty frontier and does nothing.
+    HIP_TRY(hipMemsetD32Async(fcnt, 1, 1, s));
+    uint32_t depth = 0;
+    {
+        const uint32_t gcap = std::min<uint32_t>((max_wide + 255) / 256, 1024u);
+        auto cgrid = [&](uint32_t d) {   // depth d holds at most width^d nodes
+            uint64_t b = 1;
+            for (uint32_t k = 0; k < d && b < 256ull * gcap; ++k) b *= width;
+            return (uint32_t)std::min<uint64_t>(gcap, (b + 255) / 256);
+        };
+        std::vector<uint32_t> fc;
+        // first batch: about the wide depth that the binary depth implies (a wide level
+        // spans ~log2(width) binary levels), then four levels at a time
+        for (uint32_t d = 0, batch = nlevels / (width == 8 ? 3u : 2u) + 2u; depth == 0 && d < max_wide; batch = 4u) {
+            const uint32_t upto = std::min<uint32_t>(d + batch, max_wide);
+            HIP_TRY(hipMemsetAsync(fcnt + d + 1, 0, sizeof(uint32_t) * (upto - d), s));
+            for (; d < upto; ++d) {
+                const Frontier *in = (d & 1u) ? fb : fa;
+                Frontier       *ou = (d & 1u) ? fa : fb;
+                if (width == 4)
+                    k_collapse<4><<<cgrid(d), 256, 0, s>>>(nodes, in, fcnt + d, ou, fcnt + d + 1, counts + 4, wide_tmp, dp);
+                else
+                    k_collapse<8><<<cgrid(d), 256, 0, s>>>(nodes, in, fcnt + d, ou, fcnt + d + 1, counts + 4, wide_tmp, dp);
+            }
+            HIP_TRY(hipGetLastError());
+            fc.resize(d + 1);
+            HIP_TRY(hipMemcpyAsync(fc.data(), fcnt, sizeof(uint32_t) * (d + 1), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            for (uint32_t k = 0; k <= d; ++k)
+                if (fc[k] == 0u) {
+                    depth = k;
+                    break;
+                }
+        }
     }
     const uint32_t nwide = hc[4];
     if (width == 4) k_check_wide<4><<<(nwide + 255) / 256, 256, 0, s>>>(wide_tmp, nwide, ntri, counts + 5);
     else k_check_wide<8><<<(nwide + 255) / 256, 256, 0, s>>>(wide_tmp, nwide, ntri, counts + 5);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const bool invalid = hc[5] != 0u;
-    void *wide;
-    HIP_TRY(hipMalloc(&wide, node_bytes * nwide));
-    HIP_TRY(hipMemcpyAsync(wide, wide_tmp, node_bytes * nwide, hipMemcpyDeviceToDevice, s));
-    float4 *tri_out;
-    HIP_TRY(hipMalloc((void **)&tri_out, sizeof(float4) * 3 * (size_t)ntri));
     k_reorder<<<(ntri + 255) / 256, 256, 0, s>>>(idx, ntri, d_tri, tri_out);
     HIP_TRY(hipGetLastError());
+    void *wide;   // allocated while the two kernels above run
+    HIP_TRY(hipMalloc(&wide, node_bytes * nwide));
+    HIP_TRY(hipMemcpyAsync(wide, wide_tmp, node_bytes * nwide, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipEventRecord(ev1, s));
 
     double   sah_h = 0.0;
@@ -1521,7 +1797,9 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     HIP_TRY(hipMemcpyAsync(&used_h, used, sizeof used_h, hipMemcpyDeviceToHost, s));
     Bvh2Node root_node;
     HIP_TRY(hipMemcpyAsync(&root_node, nodes, sizeof root_node, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const bool invalid = hc[5] != 0u;   // the structure check of the wide tree failed
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
     hipEventDestroy(ev0);
@@ -1530,12 +1808,7 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
                 rdz = root_node.hi[2] - root_node.lo[2];
     const double ra = 2.0 * ((double)rdx * rdy + (double)rdy * rdz + (double)rdz * rdx);
 
-    void *frees[] = {plo,         phi,         cen,        idx,       tmp,      root,      nodes,
-                     large_a,     large_b,     small,      counts,    chunk_task, chunk_begin, task_chunk0,
-                     chunk_left,  chunk_loff,  split,      gbins,     sah_acc,  used,      wide_tmp,
-                     fa,          fb,          dp.cost,    dp.choice, dp.count, dp.first,  lvl_list,
-                     lvl_cnt,     lvl_off};
-    for (void *p : frees) HIP_TRY(hipFreeAsync(p, s));
+    HIP_TRY(hipFreeAsync(arena, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (invalid) {
         hipFree(wide);
