@@ -437,6 +437,30 @@ def batch_bench(seed: int = 7) -> dict:
     }
 
 
+def build_only(args) -> int:
+    """BVH build throughput of the triangle soup: the commit, then rebuilds of the same scene
+    (the first build in a process also maps the builder's scratch pool)."""
+    from crender_amd import Context, scene_from_spec, scenes
+
+    spec = scenes.soup(args.tris, seed=2)
+    ctx = Context(0, bvh_width=args.width, max_leaf=args.max_leaf, bvh_prim_cost=args.prim_cost)
+    sc = scene_from_spec(ctx, spec)
+    ms = [sc.bvh_info()["build_ms"]]
+    for _ in range(args.build_only - 1):
+        sc.commit()
+        ms.append(sc.bvh_info()["build_ms"])
+    info = sc.bvh_info()
+    rebuilds = ms[1:] or ms
+    print(json.dumps({"metric": "BVH build", "tris": args.tris, "build_ms": [round(x, 3) for x in ms],
+                      "first_build_ms": ms[0], "median_rebuild_ms": float(np.median(rebuilds)),
+                      "mtris_per_s": args.tris / (float(np.median(rebuilds)) * 1e-3) / 1e6,
+                      "wide_nodes": info["node_count"], "binary_nodes": info["bvh2_node_count"],
+                      "sah_cost": info["sah_cost"], "depth": info["depth"]}), flush=True)
+    sc.close()
+    ctx.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,6 +495,9 @@ def main(argv=None) -> int:
                          "scaling profiler byte counters on scattered reads; prints the bytes moved per launch")
     ap.add_argument("--pmc-summary", type=Path, default=None, metavar="DIR",
                     help="only summarise the counter runs under DIR into the committed traffic figure")
+    ap.add_argument("--build-only", type=int, default=0, metavar="N",
+                    help="only build the soup's BVH N times (the scene commit, then N - 1 rebuilds) and print "
+                         "each build's time")
     args = ap.parse_args(argv)
     if args.pmc_summary:
         return pmc_summary(args.pmc_summary)
@@ -504,6 +531,8 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box has one GPU)",
               file=sys.stderr)
         return 2
+    if args.build_only:
+        return build_only(args)
 
     from crender_amd import Context, Renderer, scene_from_spec, scenes
 

@@ -983,20 +983,18 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                 if (lane < 48 && (uint32_t)pos + 1u < cn)
                     cost = area3(pl[0], pl[1], pl[2], ph[0], ph[1], ph[2]) * (float)(pos + 1) +
                            area3(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]) * (float)(cn - 1u - (uint32_t)pos);
-                float bc = cost;
-                int   bl = lane;
-                for (int o = 32; o > 0; o >>= 1) {
-                    const float oc = __shfl_xor(bc, o);
-                    const int   ol = __shfl_xor(bl, o);
-                    if (oc < bc || (oc == bc && ol < bl)) {
-                        bc = oc;
-                        bl = ol;
-                    }
-                }
+                // argmin over (cost, lane) as one unsigned key: the cost's bits (non-negative
+                // floats order as integers) with the low 6 bits replaced by the lane, so costs
+                // within 2^-17 of each other go to the lower lane
+                uint32_t bk = (__float_as_uint(cost) & ~63u) | (uint32_t)lane;
+                for (int o = 32; o > 0; o >>= 1) bk = min(bk, (uint32_t)__shfl_xor((int)bk, o));
+                const int   bl = (int)(bk & 63u);
+                const float bc = __shfl(cost, bl);
                 SahOut so;
+                const int last = (int)cn - 1;   // wave-uniform
                 for (int k = 0; k < 3; ++k) {
-                    so.lo[k] = __shfl(pl[k], (int)cn - 1);
-                    so.hi[k] = __shfl(ph[k], (int)cn - 1);
+                    so.lo[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pl[k]), last));
+                    so.hi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ph[k]), last));
                 }
                 so.n     = cn;
                 so.cost  = bc;
@@ -1080,8 +1078,24 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
                 const uint32_t lb = c.b, le = c.b + nl, rb = c.b + nl, re = c.e;
                 const uint32_t lsl = c.slot + 1, rsl = c.slot + 2 * nl;
                 const bool     lbig = keep_l && keep_r ? le - lb >= re - rb : keep_r;
+                if (sorted) {   // wave-uniform: ranges and slots only
+                    if (keep_l && keep_r) {
+                        if (lane == 0) {
+                            LocalTask &d = wst[wsp];
+                            d.b    = lbig ? lb : rb;
+                            d.e    = lbig ? le : re;
+                            d.slot = lbig ? lsl : rsl;
+                        }
+                        ++wsp;
+                    }
+                    c.b    = lbig ? rb : lb;
+                    c.e    = lbig ? re : le;
+                    c.slot = lbig ? rsl : lsl;
+                    wave_sync();
+                    continue;
+                }
                 float lcl[3], lch[3], rcl[3], rch[3];
-                const bool byax = ax >= 0 && !sorted;
+                const bool byax = ax >= 0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     // values, not addresses, are selected (a selected address would put c in memory)
