@@ -1193,13 +1193,17 @@ struct CollapseDP {
 // Level lists: list[off[L] .. off[L] + cnt[L]) holds the slots of depth L.  Launch L appends
 // the children of its inner nodes as level L + 1 (cnt zeroed beforehand, cnt[0] = 1 with
 // list[0] = the root, off[0] = 0).
-__global__ void k_level_down(const Bvh2Node *nodes, uint32_t nslots, uint32_t *list, uint32_t *cnt, uint32_t *off,
-                             uint32_t L)
+constexpr uint32_t kLevelBlock = 256;
+
+__global__ __launch_bounds__(kLevelBlock) void k_level_down(const Bvh2Node *nodes, uint32_t nslots, uint32_t *list,
+                                                            uint32_t *cnt, uint32_t *off, uint32_t L)
 {
     const uint32_t n = cnt[L], base = off[L], out = base + n;
     if (blockIdx.x == 0 && threadIdx.x == 0) off[L + 1] = out;
-    const uint32_t lane = threadIdx.x & 63u;
-    // whole waves step through the list together, so every wave appends with one atomic
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    __shared__ uint32_t wsum[kLevelBlock / 64], bfirst;
+    // whole blocks step through the list together, so every block appends with one atomic
+    // (a wave each made ~8K atomics on one counter per large level, serialised at the L2)
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
         Bvh2Node nd{};
@@ -1210,14 +1214,21 @@ __global__ void k_level_down(const Bvh2Node *nodes, uint32_t nslots, uint32_t *l
         }
         const uint64_t bal  = __ballot(inner);
         const uint32_t rank = __popcll(bal & ((1ull << lane) - 1ull));
-        uint32_t first = 0;
-        if (lane == 0 && bal) first = atomicAdd(cnt + L + 1, 2u * (uint32_t)__popcll(bal));
-        first = __shfl(first, 0);
-        const uint32_t k = first + 2u * rank;
+        if (lane == 0) wsum[w] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t j = 0; j < kLevelBlock / 64; ++j) {
+            before += j < w ? wsum[j] : 0u;
+            tot += wsum[j];
+        }
+        if (threadIdx.x == 0) bfirst = tot ? atomicAdd(cnt + L + 1, 2u * tot) : 0u;
+        __syncthreads();
+        const uint32_t k = bfirst + 2u * (before + rank);
         if (inner && out + k + 1 < nslots) {
             list[out + k]     = nd.a;
             list[out + k + 1] = nd.b;
         }
+        __syncthreads();   // wsum and bfirst are rewritten by the next round
     }
 }
 
@@ -1280,62 +1291,67 @@ __global__ void k_dp_level(const Bvh2Node *nodes, const uint32_t *list, const ui
 }
 
 template <int W>
-__device__ __forceinline__ void collapse_node(const Bvh2Node *b2, const Frontier f, Frontier *out, uint32_t *nout,
-                                              uint32_t *wide_count, void *wide_nodes, const CollapseDP &dp)
+// Called by every lane of a wave together (valid: the lane holds a frontier node); the wave
+// allocates its nodes' inner children with one atomic per counter.
+__device__ __forceinline__ void collapse_node(bool valid, const Bvh2Node *b2, const Frontier f, Frontier *out,
+                                              uint32_t *nout, uint32_t *wide_count, void *wide_nodes,
+                                              const CollapseDP &dp)
 {
-    const Bvh2Node root = b2[f.slot];
     // children: binary slots, each a leaf of its whole range (leafy) or a wide node to be
     uint32_t ch[W];
     bool     leafy[W];
     int      nc = 0;
-    uint32_t c1 = dp.choice[f.slot] & 15u;
-    // a binary leaf, or a root that the cost model makes one leaf (the root is the only
-    // frontier node that can carry a leaf choice)
-    const bool one_leaf = b2_leaf(root) || (c1 == 0u && dp.count[f.slot] <= kMaxLeafPrims);
-    if (!one_leaf && (c1 == 0u || c1 >= (uint32_t)W)) c1 = (uint32_t)W / 2u;
-    if (one_leaf) {
-        ch[nc]      = f.slot;
-        leafy[nc++] = true;
-    } else {
-        uint32_t sslot[W], sk[W];
-        int      sp = 0;
-        sslot[sp] = root.b;
-        sk[sp++]  = (uint32_t)W - c1;
-        sslot[sp] = root.a;
-        sk[sp++]  = c1;
-        while (sp > 0 && nc < W) {
-            --sp;
-            const uint32_t m = sslot[sp], k = sk[sp];
-            const Bvh2Node mn = b2[m];
-            if (b2_leaf(mn)) {
-                ch[nc]      = m;
-                leafy[nc++] = true;
-                continue;
+    if (valid) {
+        const Bvh2Node root = b2[f.slot];
+        uint32_t c1 = dp.choice[f.slot] & 15u;
+        // a binary leaf, or a root that the cost model makes one leaf (the root is the only
+        // frontier node that can carry a leaf choice)
+        const bool one_leaf = b2_leaf(root) || (c1 == 0u && dp.count[f.slot] <= kMaxLeafPrims);
+        if (!one_leaf && (c1 == 0u || c1 >= (uint32_t)W)) c1 = (uint32_t)W / 2u;
+        if (one_leaf) {
+            ch[nc]      = f.slot;
+            leafy[nc++] = true;
+        } else {
+            uint32_t sslot[W], sk[W];
+            int      sp = 0;
+            sslot[sp] = root.b;
+            sk[sp++]  = (uint32_t)W - c1;
+            sslot[sp] = root.a;
+            sk[sp++]  = c1;
+            while (sp > 0 && nc < W) {
+                --sp;
+                const uint32_t m = sslot[sp], k = sk[sp];
+                const Bvh2Node mn = b2[m];
+                if (b2_leaf(mn)) {
+                    ch[nc]      = m;
+                    leafy[nc++] = true;
+                    continue;
+                }
+                const uint32_t cm = dp.choice[m];
+                if (k == 1u) {
+                    ch[nc]      = m;
+                    leafy[nc++] = (cm & 15u) == 0u && dp.count[m] <= kMaxLeafPrims;
+                    continue;
+                }
+                const uint32_t j = (cm >> (4u * (k - 1u))) & 15u;
+                if (j == 0u || j >= k) {
+                    sslot[sp] = m;
+                    sk[sp++]  = k - 1u;
+                } else if (sp + 2 <= W) {
+                    sslot[sp] = mn.b;
+                    sk[sp++]  = k - j;
+                    sslot[sp] = mn.a;
+                    sk[sp++]  = j;
+                } else {   // cannot happen: the slots on the stack sum to at most W
+                    ch[nc]      = m;
+                    leafy[nc++] = false;
+                }
             }
-            const uint32_t cm = dp.choice[m];
-            if (k == 1u) {
-                ch[nc]      = m;
-                leafy[nc++] = (cm & 15u) == 0u && dp.count[m] <= kMaxLeafPrims;
-                continue;
+            while (sp > 0 && nc < W) {   // cannot happen either; keep every subtree in the tree
+                --sp;
+                ch[nc]      = sslot[sp];
+                leafy[nc++] = b2_leaf(b2[sslot[sp]]);
             }
-            const uint32_t j = (cm >> (4u * (k - 1u))) & 15u;
-            if (j == 0u || j >= k) {
-                sslot[sp] = m;
-                sk[sp++]  = k - 1u;
-            } else if (sp + 2 <= W) {
-                sslot[sp] = mn.b;
-                sk[sp++]  = k - j;
-                sslot[sp] = mn.a;
-                sk[sp++]  = j;
-            } else {   // cannot happen: the slots on the stack sum to at most W
-                ch[nc]      = m;
-                leafy[nc++] = false;
-            }
-        }
-        while (sp > 0 && nc < W) {   // cannot happen either; keep every subtree in the tree
-            --sp;
-            ch[nc]      = sslot[sp];
-            leafy[nc++] = b2_leaf(b2[sslot[sp]]);
         }
     }
     uint32_t ref[W];
@@ -1356,11 +1372,28 @@ __device__ __forceinline__ void collapse_node(const Bvh2Node *b2, const Frontier
             ref[j] = ref_leaf(fst, cnt);
         }
     }
-    // a node's inner children get consecutive indices (one allocation), so siblings share
-    // DRAM pages and translation entries
+    // a node's inner children get consecutive indices, so siblings share DRAM pages and
+    // translation entries; the wave's nodes take consecutive ranges (prefix sum of ninner,
+    // at most W < 16, from one ballot per bit)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt   = (1ull << lane) - 1ull;
+    uint32_t       pre = 0, tot = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+        const uint64_t m = __ballot((ninner >> b) & 1u);
+        pre += (uint32_t)__popcll(m & lt) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    uint32_t wbase = 0, fbase = 0;
+    if (lane == 0 && tot) {
+        wbase = atomicAdd(wide_count, tot);
+        fbase = atomicAdd(nout, tot);
+    }
+    wbase = __shfl(wbase, 0);
+    fbase = __shfl(fbase, 0);
+    if (!valid) return;
     if (ninner) {
-        uint32_t wi = atomicAdd(wide_count, ninner);
-        uint32_t fo = atomicAdd(nout, ninner);
+        uint32_t wi = wbase + pre, fo = fbase + pre;
         for (int j = 0; j < nc; ++j) {
             if (!inner[j]) continue;
             ref[j]    = wi;
@@ -1431,8 +1464,12 @@ __global__ __launch_bounds__(256) void k_collapse(const Bvh2Node *b2, const Fron
                                                   void *wide_nodes, CollapseDP dp)
 {
     const uint32_t nin = *nin_p;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nin; i += gridDim.x * blockDim.x)
-        collapse_node<W>(b2, in[i], out, nout, wide_count, wide_nodes, dp);
+    // the loop bound is the same for the whole wave, so its lanes call collapse_node together
+    for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < nin; w0 += gridDim.x * blockDim.x) {
+        const uint32_t i     = w0 + (threadIdx.x & 63u);
+        const bool     valid = i < nin;
+        collapse_node<W>(valid, b2, valid ? in[i] : Frontier{0u, 0u}, out, nout, wide_count, wide_nodes, dp);
+    }
 }
 
 // Structural check of the finished wide BVH, so that a bad tree fails the build instead of
@@ -1721,7 +1758,7 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
         for (uint32_t L = 0, batch = large_levels + 24u; L < kMaxLevels && nlevels == 0; batch = 16u) {
             const bool     first = L == 0;
             const uint32_t upto  = std::min<uint32_t>(L + batch, kMaxLevels);
-            for (; L < upto; ++L) k_level_down<<<grid(L), 256, 0, s>>>(nodes, nslots, lvl_list, lvl_cnt, lvl_off, L);
+            for (; L < upto; ++L) k_level_down<<<grid(L), kLevelBlock, 0, s>>>(nodes, nslots, lvl_list, lvl_cnt, lvl_off, L);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(lc.data(), lvl_cnt, sizeof(uint32_t) * (L + 1), hipMemcpyDeviceToHost, s));
             if (first) HIP_TRY(hipMemcpyAsync(hc, counts, sizeof hc, hipMemcpyDeviceToHost, s));
