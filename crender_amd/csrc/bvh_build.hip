@@ -938,7 +938,57 @@ __global__ __launch_bounds__(256) void k_small(const BuildTask *tasks, uint32_t 
             uint32_t       nl = 0;
             int            ax = -1;
             bool           keep_l = true, keep_r = true;   // children still to be built
-            if (sorted) {
+            if (cn == 2u) {
+                // ---- two primitives: the only split is one against one.  Ordered by centroid
+                // x, ties by position, with the cost summed in that order, it is what the
+                // sweep below decides (its three rows tie, and row x has the lowest lanes) ----
+                uint32_t id = 0;
+                float    kx = 0.0f;
+                float4   blo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bhi = blo;
+                if (lane < 2) {
+                    id  = ids[c.b + lane];
+                    kx  = cen[id].x;
+                    blo = plo[id];
+                    bhi = phi[id];
+                }
+                auto rl = [](float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+                const bool     sw  = rl(kx, 1) < rl(kx, 0);    // the second comes first
+                const int      f0  = sw ? 1 : 0, f1 = sw ? 0 : 1;
+                const uint32_t id0 = (uint32_t)__builtin_amdgcn_readlane((int)id, 0);
+                const uint32_t id1 = (uint32_t)__builtin_amdgcn_readlane((int)id, 1);
+                SahOut a, b, so;
+                const float la[3] = {rl(blo.x, 0), rl(blo.y, 0), rl(blo.z, 0)}, ha[3] = {rl(bhi.x, 0), rl(bhi.y, 0), rl(bhi.z, 0)};
+                const float lb[3] = {rl(blo.x, 1), rl(blo.y, 1), rl(blo.z, 1)}, hb[3] = {rl(bhi.x, 1), rl(bhi.y, 1), rl(bhi.z, 1)};
+                for (int k = 0; k < 3; ++k) {
+                    a.lo[k]  = f0 ? lb[k] : la[k];
+                    a.hi[k]  = f0 ? hb[k] : ha[k];
+                    b.lo[k]  = f1 ? lb[k] : la[k];
+                    b.hi[k]  = f1 ? hb[k] : ha[k];
+                    so.lo[k] = fminf(a.lo[k], b.lo[k]);
+                    so.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+                }
+                so.n     = 2u;
+                so.cost  = area3(a.lo[0], a.lo[1], a.lo[2], a.hi[0], a.hi[1], a.hi[2]) * 1.0f +
+                           area3(b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]) * 1.0f;
+                so.axis  = 0;
+                so.bin   = 0;
+                so.nleft = 1u;
+                leaf     = decide(so, max_leaf, nl, ax);
+                if (lane == 0) {
+                    if (leaf) write_leaf(nodes, nslots, err, c.slot, so, gb + c.b, cn);
+                    else write_inner(nodes, nslots, err, c.slot, so, nl);
+                }
+                if (!leaf && ax >= 0) {   // both children are one-primitive leaves
+                    if (lane == 0) {
+                        ids[c.b]     = sw ? id1 : id0;
+                        ids[c.b + 1] = sw ? id0 : id1;
+                        write_leaf(nodes, nslots, err, c.slot + 1u, a, gb + c.b, 1u);
+                        write_leaf(nodes, nslots, err, c.slot + 2u, b, gb + c.b + 1u, 1u);
+                    }
+                    keep_l = keep_r = false;
+                }
+                wave_sync();
+            } else if (sorted) {
                 // ---- exact SAH sweep over the sorted primitives (at most 16) ----
                 const int  row = lane >> 4, pos = lane & 15;   // row a: primitives sorted by axis a
                 const bool val = lane < 48 && (uint32_t)pos < cn;
