@@ -1600,12 +1600,26 @@ __global__ void k_sah_cost(const Bvh2Node *nodes, uint32_t nslots, double *acc, 
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
-// The build's scratch memory comes from a pool of its own per device that keeps up to
-// kPoolKeep bytes mapped between builds: the default pool hands freed memory back to the
-// driver at every synchronisation, and mapping a few hundred MB again costs ~0.2 ms per
-// build.  Scratch beyond kPoolKeep (builds of more than ~5M triangles) is still returned.
-constexpr uint64_t kPoolKeep   = 2ull << 30;
+// The build's scratch memory: one device buffer per device, kept between builds and grown
+// when a build needs more, up to kKeepBytes (≈ 4M triangles); a larger build allocates and
+// frees its own.  Allocating the scratch per build cost ≈ 0.5 ms of host time in 31 pool
+// allocations and ~0.25 ms of idle GPU when made while kernels were queued; with one pool
+// allocation per build, some builds still spent 35 ms (once ~1 s) before their first kernel.
+// Builds on one device take turns on the buffer.
+constexpr uint64_t kKeepBytes  = 2ull << 30;
 constexpr int      kMaxDevices = 64;
+
+struct ScratchArena {
+    std::mutex mu;
+    void      *base  = nullptr;
+    size_t     bytes = 0;
+};
+
+ScratchArena &device_arena(int dev)
+{
+    static ScratchArena arenas[kMaxDevices];   // device memory held until the process ends
+    return arenas[dev];
+}
 
 // The build's scratch buffers, carved out of one allocation in 256-byte aligned pieces.
 struct Arena {
@@ -1624,34 +1638,6 @@ struct Arena {
     }
 };
 
-hipError_t scratch_pool(hipMemPool_t *out)
-{
-    static std::mutex   mu;
-    static hipMemPool_t pools[kMaxDevices] = {};
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!pools[dev]) {
-        hipMemPoolProps props{};
-        props.allocType     = hipMemAllocationTypePinned;
-        props.handleTypes   = hipMemHandleTypeNone;
-        props.location.type = hipMemLocationTypeDevice;
-        props.location.id   = dev;
-        hipMemPool_t pool   = nullptr;
-        HIP_TRY(hipMemPoolCreate(&pool, &props));
-        uint64_t keep = kPoolKeep;
-        const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        if (e != hipSuccess) {
-            hipMemPoolDestroy(pool);
-            return e;
-        }
-        pools[dev] = pool;
-    }
-    *out = pools[dev];
-    return hipSuccess;
-}
-
 }  // namespace
 
 hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t max_leaf, hipStream_t s,
@@ -1663,8 +1649,11 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     if (ntri >= (1u << 27)) return hipErrorInvalidValue;
     if (max_leaf < 1) max_leaf = 1;
     if (max_leaf > 16) max_leaf = 16;
-    hipMemPool_t pool;
-    HIP_TRY(scratch_pool(&pool));
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    ScratchArena               &sa = device_arena(dev);
+    std::lock_guard<std::mutex> arena_lock(sa.mu);
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
@@ -1724,10 +1713,21 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
     A.add(&fa, sizeof(Frontier) * max_wide);
     A.add(&fb, sizeof(Frontier) * max_wide);
     A.add(&fcnt, sizeof(uint32_t) * ((size_t)max_wide + 6));
-    // one allocation for all scratch: a pool allocation costs the host ~10 us, and one made
-    // while kernels are queued has been measured to leave the GPU idle ~0.25 ms
-    void *arena = nullptr;
-    HIP_TRY(hipMallocFromPoolAsync(&arena, A.bytes, pool, s));
+    void *arena     = nullptr;
+    bool  own_arena = A.bytes > kKeepBytes;   // too large to keep: this build's own
+    if (own_arena) {
+        HIP_TRY(hipMalloc(&arena, A.bytes));
+    } else {
+        if (sa.bytes < A.bytes) {   // the previous build on this device has completed
+            if (sa.base) HIP_TRY(hipFree(sa.base));
+            sa.base  = nullptr;
+            sa.bytes = 0;
+            const size_t want = std::min<size_t>(kKeepBytes, A.bytes + A.bytes / 4);
+            HIP_TRY(hipMalloc(&sa.base, want));
+            sa.bytes = want;
+        }
+        arena = sa.base;
+    }
     A.place(arena);
     float4 *tri_out = nullptr;   // the triangles in leaf order: part of the result
 
@@ -1815,8 +1815,8 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
             HIP_TRY(hipStreamSynchronize(s));
             if (first && hc[5] != 0u) {
                 // a node slot or child range fell outside the tree: nothing past this point is safe
-                HIP_TRY(hipFreeAsync(arena, s));
                 HIP_TRY(hipStreamSynchronize(s));
+                if (own_arena) hipFree(arena);
                 hipFree(tri_out);
                 hipEventDestroy(ev0);
                 hipEventDestroy(ev1);
@@ -1909,8 +1909,7 @@ hipError_t build_bvh(const float *d_tri, uint32_t ntri, uint32_t width, uint32_t
                 rdz = root_node.hi[2] - root_node.lo[2];
     const double ra = 2.0 * ((double)rdx * rdy + (double)rdy * rdz + (double)rdz * rdx);
 
-    HIP_TRY(hipFreeAsync(arena, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if (own_arena) HIP_TRY(hipFree(arena));
     if (invalid) {
         hipFree(wide);
         hipFree(tri_out);
