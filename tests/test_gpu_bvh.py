@@ -45,6 +45,10 @@ def _check(spec, width):
     info = sc.bvh_info()
     sc.close()
     ctx.close()
+    _check_tree(spec, width, nodes, prims, info)
+
+
+def _check_tree(spec, width, nodes, prims, info):
     ntri = spec.ntri
     assert np.array_equal(np.sort(prims), np.arange(ntri))
     tri = spec.corners().reshape(-1, 3, 3).astype(np.float64)[prims]     # leaf order
@@ -86,3 +90,36 @@ def test_coincident_centroids():
     v[:] = v[0]                      # every triangle identical
     spec.vertices = v.reshape(-1, 3)
     _check(spec, 8)
+
+
+def test_concurrent_commits_share_the_build_scratch():
+    """Builds on one GPU take turns on the builder's scratch buffer: two threads committing
+    scenes of different sizes on two contexts at once both get complete, correct trees (the
+    second size also makes the buffer grow while the other thread may be building)."""
+    import threading
+
+    specs = [scenes.soup(30_000, seed=21), scenes.soup(120_000, seed=22)]
+    errors, exported = [], [None, None]
+
+    def work(k):
+        try:
+            ctx = Context(0)
+            sc = scene_from_spec(ctx, specs[k])
+            for _ in range(4):
+                sc.commit()
+            exported[k] = (sc.bvh_export(), sc.bvh_info())
+            sc.close()
+            ctx.close()
+        except Exception as e:   # reported by the main thread
+            errors.append(e)
+
+    print("bvh: two threads committing", flush=True)
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not errors, errors
+    for k in range(2):
+        (nodes, prims), info = exported[k]
+        _check_tree(specs[k], 8, nodes, prims, info)
