@@ -5,8 +5,7 @@
 //   1. binned SAH (16 bins per axis over centroid bounds) top-down.  Segments larger than
 //      one chunk (2048 primitives) are binned by many workgroups (LDS bins merged with
 //      global atomics) and partitioned stably chunk by chunk; smaller segments are built
-//      to completion by one workgroup each, with the primitive ids held in LDS, and their
-//      nodes of at most 16 primitives split by an exact SAH sweep over sorted centroids.
+//      to completion by one workgroup each, with the primitive ids held in LDS.
 //      Node slots are derived from segment ranges, so the tree is deterministic.
 //   2. collapse of the binary tree into a 4- or 8-wide tree: an SAH-optimal distribution of
 //      the binary subtrees over each wide node's child slots, small subtrees becoming
