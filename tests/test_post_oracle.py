@@ -90,6 +90,44 @@ def test_atrous_keeps_constants_and_smooths_noise():
     assert np.mean((out[..., :3] - 0.4) ** 2) < 0.1 * np.mean((noisy[..., :3] - 0.4) ** 2)
 
 
+def test_atrous_without_edge_stopping_is_the_b3_spline_cascade():
+    """With every edge-stopping width huge, each level is the plain 5x5 B3-spline kernel
+    (1, 4, 6, 4, 1) / 16 with holes of 2^level pixels: away from the border (62 pixels: the
+    sum of the levels' radii) the filter equals that cascade of convolutions computed by
+    scipy, and at the border a tap outside the image drops out with its weight renormalised
+    (a corner pixel of one level: the taps at offsets 0..2 in both directions)."""
+    from scipy.ndimage import convolve
+
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 1, (150, 140, 4))
+    flat = np.full(img.shape, 0.5)
+    out = P.atrous(img, flat, flat, levels=5, sc=1e9, sn=1e9, sa=1e9)[..., :3]
+    k1 = np.array([1, 4, 6, 4, 1], np.float64) / 16.0
+    want = img[..., :3].copy()
+    for lvl in range(5):
+        k = np.zeros(4 * (1 << lvl) + 1)
+        k[:: 1 << lvl] = k1
+        k2 = np.outer(k, k)
+        want = np.stack([convolve(want[..., ch], k2, mode="constant") for ch in range(3)], -1)
+    np.testing.assert_allclose(out[62:-62, 62:-62], want[62:-62, 62:-62], rtol=0, atol=1e-12)
+    one = P.atrous(img, flat, flat, levels=1, sc=1e9, sn=1e9, sa=1e9)[0, 0, :3]
+    w = np.outer(k1[2:], k1[2:])
+    np.testing.assert_allclose(one, np.einsum("yx,yxc->c", w, img[:3, :3, :3]) / w.sum(), rtol=1e-13)
+
+
+def test_atrous_with_sharp_normal_edges_is_the_identity():
+    """With a tiny normal width and a different normal at every pixel, every tap but the
+    pixel itself gets weight exp(-huge) = 0, so the output is the input ((w c) / w: within
+    an ulp)."""
+    rng = np.random.default_rng(4)
+    img = rng.uniform(0, 1, (20, 30, 4))
+    nrm = rng.normal(0, 1, (20, 30, 4))
+    nrm[..., :3] /= np.linalg.norm(nrm[..., :3], axis=-1, keepdims=True)
+    alb = np.full(img.shape, 0.5)
+    out = P.atrous(img, nrm, alb, levels=5, sn=1e-4)
+    np.testing.assert_allclose(out[..., :3], img[..., :3], rtol=3e-16, atol=0)
+
+
 def test_atrous_defaults_halve_display_error_of_a_16spp_render():
     """The default tolerances on the renderer's own output: a 16 spp Cornell box display
     image, guided by that render's normals and albedos, against a 1024 spp one."""
