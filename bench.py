@@ -242,7 +242,8 @@ def pmc_summary(d: Path) -> int:
     print(f"\n{rays} extension rays over {len(fetch)} dispatches: {total / rays:.1f} bytes per ray copy-scaled, "
           f"{total_g / rays:.1f} gather-scaled (algorithmic {BYTES_PER_RAY}), {total / len(fetch) / 1e6:.1f} MB "
           f"per dispatch on average copy-scaled; "
-          f"written to {PMC_FILE.relative_to(ROOT)} (source {summary['source_hash']}, commit {commit}).")
+          f"written to {PMC_FILE.relative_to(ROOT) if PMC_FILE.is_relative_to(ROOT) else PMC_FILE} "
+          f"(source {summary['source_hash']}, commit {commit}).")
     return 0
 
 
